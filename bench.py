@@ -1,0 +1,162 @@
+"""Benchmark: cell-RK-stage updates/s of the FP64 device path on one MI355X.
+
+Workload (default): the 15-degree compression ramp, 512x256 quads with a
+seed-chosen 50% split into triangles (196,608 cells, 328,448 faces), Euler,
+second order, SQUEEZE limiter, LOCAL time step, CFL 0.8, M=2.5.  One "step"
+is one SSP-RK3 iteration (3 residual evaluations + stage updates) over the
+whole mesh with all inputs resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ramp|bump|ogrid]
+
+Prints one JSON line.  `value` = cells * 3 * K / wall seconds of the K timed
+steps (the library call returns after its stream has drained).  The
+`roofline` object prices the dominant kernel's algorithmic bytes per launch
+against its average launch time measured with HIP events on the library's
+stream; `cpu_baseline` is the reference solver built from its own sources
+(or, where that build is absent, the C oracle) timed on one host core over a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cfd-2d-ns-fv_amd"))
+
+import numpy as np  # noqa: E402
+
+from fv2d import gridgen as gg  # noqa: E402
+from fv2d import inputs  # noqa: E402
+from fv2d.lib import Solver  # noqa: E402
+
+METRIC = "cell-RK-stage updates/s, FP32-mixed vs FP64 at stated error; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(name: str):
+    if name == "ramp":
+        u, v = inputs.mach_velocity(2.5)
+        return ("ramp512x256-split50 Euler o2 SQUEEZE LOCAL CFL0.8 M2.5", gg.ramp(512, 256),
+                inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE", dt_type="LOCAL", CFL=0.8))
+    if name == "bump":
+        u, v = inputs.mach_velocity(0.5)
+        return ("bump64x32 Euler o1 GLOBAL CFL0.5 M0.5", gg.bump(64, 32),
+                inputs.Case(u=u, v=v, poly=0, dt_type="GLOBAL", CFL=0.5))
+    if name == "ogrid":
+        u, v = inputs.mach_velocity(0.2)
+        return ("ogrid1024x512 NS Re100 o2 SQUEEZE GLOBAL CFL0.5 M0.2", gg.ogrid(1024, 512),
+                inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE", eqn="NS", mu=1.225 * u / 100.0,
+                            BC1="NO_SLIP_WALL", dt_type="GLOBAL", CFL=0.5))
+    raise SystemExit(f"unknown config {name}")
+
+
+def algorithmic_bytes(case, nc: int, nf: int, s: int = 8, g: int = 8, i: int = 4):
+    """Compulsory HBM bytes per launch of each kernel (minimal-traffic model:
+    every array read or written once per pass, neighbour re-reads free,
+    derived geometry recomputed from centroids, face centres, normals,
+    lengths, volumes and LSQ inverses)."""
+    if case.poly == 0:
+        # one pass: Q (4s) + Q_n (8s/3) + dt (s) + write Q (4s) + sigma (s/3) + V (g)
+        b = nc * (12 * s + g) + nf * (2 * i + 2 * g)
+        return {"pass_b": b}
+    a = nc * (24 * s + 5 * g) + nf * (2 * i + 2 * g)
+    if case.limiter != "SQUEEZE":
+        a = nc * (12 * s + 5 * g) + nf * (2 * i + 2 * g)
+    b = nc * (20 * s + 3 * g) + nf * (2 * i + 4 * g)
+    return {"pass_a": a, "pass_b": b}
+
+
+def cpu_baseline(grid, case, nc: int, target_s: float = 15.0):
+    """Time the reference solver (oracle/_ref/ref_harness) or the C oracle on
+    one core over a bounded number of steps of the same workload."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if os.path.exists(harness):
+        d = tempfile.mkdtemp()
+        grid.write(os.path.join(d, "grid.in"))
+        c = inputs.Case(**{**case.__dict__, "grid_file": "grid.in"})
+        c.write(os.path.join(d, "input.in"))
+
+        def run(n):
+            r = subprocess.run(["taskset", "-c", "0", harness, "input.in", "out.bin", str(n),
+                                "--dump", "0", "--time"], cwd=d, capture_output=True, text=True,
+                               check=True)
+            return float(r.stdout.strip().splitlines()[-1].split()[1])
+
+        t1 = run(2)
+        n = max(2, min(400, int(target_s / max(t1 / 2, 1e-6))))
+        t = run(n)
+        kind, what = "reference", f"reference solver built from its own sources, {n} steps"
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        o = O.Oracle(grid, case)
+        t0 = time.perf_counter()
+        o.step(2)
+        t1 = time.perf_counter() - t0
+        n = max(2, min(400, int(target_s / max(t1 / 2, 1e-6))))
+        t0 = time.perf_counter()
+        o.step(n)
+        t = time.perf_counter() - t0
+        kind, what = "port", f"C oracle (serial restatement), {n} steps"
+    return {"value": nc * 3 * n / t, "unit": "cell-stage/s", "cores": 1, "kind": kind,
+            "sample": f"{what} of the same workload ({t:.2f} s of step loop)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--config", default="ramp")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(instances or mesh partitions would shard across GPUs, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    name, grid, case = workload(args.config)
+    nc, nf = grid.num_cells, grid.num_faces
+    s = Solver(grid, case)
+    s.init_freestream()
+    s.step(args.warmup)
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    wall = time.perf_counter() - t0
+    value = nc * 3 * args.steps / wall
+
+    # per-kernel HIP-event timing on the library's stream
+    ms, cnt = s.time_steps(args.profile_steps)
+    kinds = {1: "pass_a", 2: "pass_b"}
+    per_launch = {kinds[k]: ms[k] / cnt[k] for k in (1, 2) if cnt[k]}
+    totals = {kinds[k]: ms[k] for k in (1, 2) if cnt[k]}
+    algo = algorithmic_bytes(case, nc, nf)
+    dom = max(totals, key=totals.get)
+    achieved = algo[dom] / (per_launch[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "algorithmic_bytes_per_launch": int(algo[dom]),
+            "avg_launch_us": round(per_launch[dom] * 1e3, 2),
+            "all_kernels_avg_us": {k: round(v * 1e3, 2) for k, v in per_launch.items()}}
+    finite = bool(np.isfinite(s.state()).all())
+    out = {"metric": METRIC, "value": value, "unit": "cell-stage/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": name, "cells": nc, "faces": nf, "precision": "FP64 plain",
+                      "state_finite": finite},
+           "roofline": roof}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(grid, case, nc)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,69 @@
+// fv_kernels.hpp — argument blocks and launchers of the FP64 FV kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace fvk {
+
+// Read-only mesh data on the device (new numbering; see mesh_prep.hpp).
+struct DevMesh {
+  int32_t nc;                  // real cells
+  int32_t ncp;                 // padded cells (multiple of 64)
+  const int32_t* chunk_off;    // slot offset of each 64-cell chunk
+  const uint8_t* deg;          // faces per cell
+  const int32_t* code;         // per slot: neighbour/owner or boundary kind
+  const double* s_nx;          // per slot: face unit normal (out of the owner)
+  const double* s_ny;
+  const double* s_area;        // per slot: face length
+  const double* s_fcx;         // per slot: face centre
+  const double* s_fcy;
+  const double* ccx;           // per cell: centroid
+  const double* ccy;
+  const double* vol;           // per cell: area
+  const double* ixx;           // per cell: LSQ inverse (readmesh.cpp:316-320)
+  const double* iyy;
+  const double* ixy;
+};
+
+// Gas and freestream constants (class_flow, class_Q members).
+struct Flow {
+  double gamma, gm1, Rgas, mu, k;
+  double cfl2;                 // 2*CFL
+  double qinf[4];              // freestream conserved state (class_flow::updateQ)
+  double rho, u, v, P;         // freestream primitives
+  double vn0, c0;              // initial face |Vn_avg| and c_avg (class_q.hpp:61-62)
+};
+
+// One RK stage.  State arrays are SoA blocks of ncp doubles: q[k*ncp + cell].
+struct StageArgs {
+  DevMesh m;
+  Flow fl;
+  int32_t stage;               // 1, 2, 3
+  int32_t dt_global;           // time_step_type GLOBAL
+  int32_t sentinel;            // lagged face states are the -101 initial values
+  int32_t fs_prims;            // stage input's primitives are the freestream ones
+  int32_t store_res;           // write the residual (stage 3 of a reported step)
+  int32_t xcd_blocks;          // grid blocks per XCD for the remap (0 = off)
+  const double* qc;            // stage input state (4 blocks)
+  const double* qp;            // previous stage's input state (lagged limiter)
+  const double* gp;            // previous stage's limited gradient (8 blocks)
+  double* g;                   // this stage's limited gradient (8 blocks)
+  const double* qn;            // state at the start of the step
+  double* qout;                // stage output state
+  double* sigma;               // per-cell spectral sum of the last stage 3
+  unsigned long long* smax_cur;   // max spectral sum used by this step (GLOBAL)
+  unsigned long long* smax_next;  // max spectral sum written by stage 3
+  double* res;                 // residual (4 blocks)
+};
+
+enum PassAKind { kPassA_Plain = 0, kPassA_Squeeze = 1 };
+
+hipError_t launch_pass_a(const StageArgs& a, int squeeze, int block, hipStream_t s);
+hipError_t launch_pass_b(const StageArgs& a, int order, int ns, int block, hipStream_t s);
+hipError_t launch_init_sigma(const StageArgs& a, int block, hipStream_t s);
+// per-block partial sums of R_k^2: out[k*nblocks + b]
+hipError_t launch_res_sq(const double* res, int32_t nc, int32_t ncp, double* partial,
+                         int nblocks, int block, hipStream_t s);
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
+
+}  // namespace fvk

@@ -1,0 +1,474 @@
+// fvk_api.hip — the C ABI (include/fvk.h): context, device memory, the SSP-RK3
+// driver (ssp_rk3.cpp:18-77 as a sequence of kernel launches on one stream).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fv_kernels.hpp"
+#include "fvk.h"
+#include "mesh_prep.hpp"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+struct fvk_ctx {
+  std::string err;
+  fvk_options opt{};
+  fvk::HostMesh hm;
+  hipStream_t stream = nullptr;
+  std::vector<void*> bufs;   // device allocations, freed by fvk_destroy
+  fvk::DevMesh dm{};
+  fvk::Flow fl{};
+  double* q[3] = {nullptr, nullptr, nullptr};   // Qn, Q1, Q2 (4*ncp each)
+  double* g[3] = {nullptr, nullptr, nullptr};   // stage 1..3 limited gradients (8*ncp each)
+  double* sigma = nullptr;
+  double* res = nullptr;
+  unsigned long long* smax = nullptr;           // [2]
+  double* partial = nullptr;
+  int block = 256;
+  int xcd_blocks = 0;
+  int res_blocks = 0;
+  int smax_cur = 0;
+  bool have_state = false;
+  bool sentinel = true;
+  bool fs_prims = false;
+  bool res_valid = false;
+};
+
+namespace {
+
+int set_err(fvk_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+int hip_err(fvk_ctx* c, hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return set_err(c, FVK_ERR_HIP, m);
+}
+
+#define FVK_HIP(ctx, call)                         \
+  do {                                             \
+    hipError_t e_ = (call);                        \
+    if (e_ != hipSuccess) return hip_err(ctx, e_, #call); \
+  } while (0)
+
+template <typename T>
+int dev_alloc(fvk_ctx* c, T** out, size_t count) {
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, count * sizeof(T) + 256);
+  if (e != hipSuccess) return hip_err(c, e, "hipMalloc");
+  c->bufs.push_back(p);
+  *out = static_cast<T*>(p);
+  return FVK_OK;
+}
+
+template <typename T>
+int upload(fvk_ctx* c, const std::vector<T>& v, const T** out) {
+  T* d = nullptr;
+  int rc = dev_alloc(c, &d, v.size());
+  if (rc) return rc;
+  if (!v.empty()) {
+    hipError_t e = hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_err(c, e, "hipMemcpy H2D");
+  }
+  *out = d;
+  return FVK_OK;
+}
+
+int check_options(const fvk_options& o, std::string* msg) {
+  if (o.order != 1 && o.order != 2) { *msg = "order must be 1 or 2 (solution_poly_order 0 or 1)"; return FVK_ERR_OPTION; }
+  if (o.limiter < 0 || o.limiter > 2) { *msg = "limiter must be NONE, MINMOD or SQUEEZE"; return FVK_ERR_OPTION; }
+  if (o.eqn != 1 && o.eqn != 2) { *msg = "eqn must be EULER or NS"; return FVK_ERR_OPTION; }
+  if (o.flux_solver == 2) { *msg = "flux_solver ROE is not implemented by the reference solver library"; return FVK_ERR_UNSUPPORTED; }
+  if (o.flux_solver != 1) { *msg = "flux_solver must be RUSANOV"; return FVK_ERR_OPTION; }
+  if (o.time_scheme != 1 && o.time_scheme != 2) { *msg = "time_scheme must be SSP_RK2 or SSP_RK3"; return FVK_ERR_OPTION; }
+  if (o.time_step_type != 1 && o.time_step_type != 2) { *msg = "time_step_type must be GLOBAL or LOCAL"; return FVK_ERR_OPTION; }
+  if (o.bc1 < -4 || o.bc1 > -1 || o.bc2 < -4 || o.bc2 > -1) { *msg = "BC1/BC2 must be FREESTREAM, SLIP_WALL, EXTRAPOLATION or NO_SLIP_WALL"; return FVK_ERR_OPTION; }
+  if (o.precision != FVK_FP64) { *msg = "precision must be FVK_FP64"; return FVK_ERR_OPTION; }
+  if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64)) {
+    *msg = "block_size must be 0 or a multiple of 64 in [64, 256]";
+    return FVK_ERR_OPTION;
+  }
+  if (!(o.rho > 0) || !(o.P > 0) || !(o.gamma > 1)) { *msg = "freestream needs rho > 0, P > 0, gamma > 1"; return FVK_ERR_OPTION; }
+  return FVK_OK;
+}
+
+fvk::StageArgs base_args(fvk_ctx* c) {
+  fvk::StageArgs a{};
+  a.m = c->dm;
+  a.fl = c->fl;
+  a.dt_global = c->opt.time_step_type == 1;
+  a.xcd_blocks = c->xcd_blocks;
+  a.sigma = c->sigma;
+  a.res = c->res;
+  a.smax_cur = c->smax + c->smax_cur;
+  a.smax_next = c->smax + (1 - c->smax_cur);
+  return a;
+}
+
+// Spectral sums |V_inf| + c_inf on every face and their max: the lagged face
+// data of a fresh start (class_q.hpp:61-62, main.cpp:117-118).
+int arm_start(fvk_ctx* c) {
+  c->smax_cur = 0;
+  FVK_HIP(c, fvk::launch_fill(reinterpret_cast<double*>(c->smax), 2, 0.0, c->stream));
+  fvk::StageArgs a = base_args(c);
+  FVK_HIP(c, fvk::launch_init_sigma(a, c->block, c->stream));
+  c->sentinel = true;
+  c->res_valid = false;
+  c->have_state = true;
+  FVK_HIP(c, hipStreamSynchronize(c->stream));
+  return FVK_OK;
+}
+
+enum { kKindA = 1, kKindB = 2 };
+
+// One ssp_rk3 call.  timer(kind, launch) lets fvk_time_steps bracket launches.
+template <typename Launch>
+int run_step(fvk_ctx* c, bool store_res, Launch&& launch) {
+  const fvk_options& o = c->opt;
+  for (int stage = 1; stage <= 3; ++stage) {
+    fvk::StageArgs a = base_args(c);
+    a.stage = stage;
+    a.sentinel = (stage == 1 && c->sentinel) ? 1 : 0;
+    a.fs_prims = (stage == 1 && c->fs_prims) ? 1 : 0;
+    a.store_res = (stage == 3 && store_res) ? 1 : 0;
+    a.qn = c->q[0];
+    if (stage == 1) { a.qc = c->q[0]; a.qp = c->q[2]; a.gp = c->g[2]; a.g = c->g[0]; a.qout = c->q[1]; }
+    if (stage == 2) { a.qc = c->q[1]; a.qp = c->q[0]; a.gp = c->g[0]; a.g = c->g[1]; a.qout = c->q[2]; }
+    if (stage == 3) { a.qc = c->q[2]; a.qp = c->q[1]; a.gp = c->g[1]; a.g = c->g[2]; a.qout = c->q[0]; }
+    if (o.order == 2) {
+      int rc = launch(kKindA, [&]() { return fvk::launch_pass_a(a, o.limiter == 2, c->block, c->stream); });
+      if (rc) return rc;
+    }
+    int rc = launch(kKindB, [&]() { return fvk::launch_pass_b(a, o.order, o.eqn == 2, c->block, c->stream); });
+    if (rc) return rc;
+  }
+  c->sentinel = false;
+  c->fs_prims = false;
+  c->smax_cur = 1 - c->smax_cur;
+  if (store_res) c->res_valid = true;
+  return FVK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void fvk_default_options(fvk_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->order = 1;
+  o->limiter = 0;
+  o->eqn = 1;
+  o->flux_solver = 1;
+  o->time_scheme = 2;
+  o->time_step_type = 1;
+  o->bc1 = FVK_BC_SLIP_WALL;
+  o->bc2 = FVK_BC_FREESTREAM;
+  o->cfl = 0.5;
+  o->gamma = 1.4;
+  o->R = 287.0;
+  o->mu = 0.0;
+  o->Pr = 0.72;
+  o->P = 1e5;
+  o->rho = 1.225;
+  o->u = 0.0;
+  o->v = 0.0;
+  o->precision = FVK_FP64;
+  o->renumber = 1;
+  o->block_size = 0;
+  o->xcd_remap = 1;
+}
+
+const char* fvk_last_error(const fvk_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int fvk_create(const fvk_mesh* mesh, const fvk_options* opt, fvk_ctx** out) {
+  g_create_error.clear();
+  if (!mesh || !opt || !out) return set_err(nullptr, FVK_ERR_ARG, "fvk_create: NULL argument");
+  *out = nullptr;
+  std::string msg;
+  int rc = check_options(*opt, &msg);
+  if (rc) return set_err(nullptr, rc, msg);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_err(nullptr, FVK_ERR_HIP, "fvk_create: no HIP device");
+  fvk_ctx* c = new fvk_ctx();
+  c->opt = *opt;
+  rc = fvk::prepare_mesh(*mesh, *opt, &c->hm, &msg);
+  if (rc) {
+    delete c;
+    return set_err(nullptr, rc, msg);
+  }
+  if (!c->hm.warnings.empty()) fputs(c->hm.warnings.c_str(), stderr);
+  const fvk::HostMesh& h = c->hm;
+  c->block = opt->block_size ? opt->block_size : 256;
+  const int nblk = (h.nc + c->block - 1) / c->block;
+  c->xcd_blocks = opt->xcd_remap ? (nblk + 7) / 8 : 0;
+  c->res_blocks = (h.nc + 255) / 256;
+  hipError_t e = hipStreamCreate(&c->stream);
+  if (e != hipSuccess) {
+    hip_err(nullptr, e, "hipStreamCreate");
+    delete c;
+    return FVK_ERR_HIP;
+  }
+  auto fail = [&](int code) {
+    g_create_error = c->err;
+    fvk_destroy(c);
+    return code;
+  };
+  fvk::DevMesh& d = c->dm;
+  d.nc = h.nc;
+  d.ncp = h.ncp;
+  if ((rc = upload(c, h.chunk_off, &d.chunk_off))) return fail(rc);
+  if ((rc = upload(c, h.deg, &d.deg))) return fail(rc);
+  if ((rc = upload(c, h.code, &d.code))) return fail(rc);
+  if ((rc = upload(c, h.s_nx, &d.s_nx))) return fail(rc);
+  if ((rc = upload(c, h.s_ny, &d.s_ny))) return fail(rc);
+  if ((rc = upload(c, h.s_area, &d.s_area))) return fail(rc);
+  if ((rc = upload(c, h.s_fcx, &d.s_fcx))) return fail(rc);
+  if ((rc = upload(c, h.s_fcy, &d.s_fcy))) return fail(rc);
+  if ((rc = upload(c, h.c_ccx, &d.ccx))) return fail(rc);
+  if ((rc = upload(c, h.c_ccy, &d.ccy))) return fail(rc);
+  if ((rc = upload(c, h.c_vol, &d.vol))) return fail(rc);
+  if ((rc = upload(c, h.c_ixx, &d.ixx))) return fail(rc);
+  if ((rc = upload(c, h.c_iyy, &d.iyy))) return fail(rc);
+  if ((rc = upload(c, h.c_ixy, &d.ixy))) return fail(rc);
+  const size_t ncp = h.ncp;
+  for (int i = 0; i < 3; ++i) {
+    if ((rc = dev_alloc(c, &c->q[i], 4 * ncp))) return fail(rc);
+    if ((rc = dev_alloc(c, &c->g[i], 8 * ncp))) return fail(rc);
+  }
+  if ((rc = dev_alloc(c, &c->sigma, ncp))) return fail(rc);
+  if ((rc = dev_alloc(c, &c->res, 4 * ncp))) return fail(rc);
+  if ((rc = dev_alloc(c, &c->smax, 2))) return fail(rc);
+  if ((rc = dev_alloc(c, &c->partial, 4 * (size_t)c->res_blocks))) return fail(rc);
+  for (int i = 0; i < 3; ++i) {
+    if ((e = fvk::launch_fill(c->q[i], 4 * (int64_t)ncp, 0.0, c->stream)) != hipSuccess) { hip_err(c, e, "fill"); return fail(FVK_ERR_HIP); }
+    if ((e = fvk::launch_fill(c->g[i], 8 * (int64_t)ncp, 0.0, c->stream)) != hipSuccess) { hip_err(c, e, "fill"); return fail(FVK_ERR_HIP); }
+  }
+  if ((e = fvk::launch_fill(c->res, 4 * (int64_t)ncp, 0.0, c->stream)) != hipSuccess) { hip_err(c, e, "fill"); return fail(FVK_ERR_HIP); }
+  if ((e = fvk::launch_fill(c->sigma, (int64_t)ncp, 0.0, c->stream)) != hipSuccess) { hip_err(c, e, "fill"); return fail(FVK_ERR_HIP); }
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) { hip_err(c, e, "sync"); return fail(FVK_ERR_HIP); }
+
+  // class_flow (class_flow.hpp:28-46) and class_Q constants
+  const fvk_options& o = *opt;
+  fvk::Flow& fl = c->fl;
+  fl.gamma = o.gamma;
+  fl.gm1 = o.gamma - 1;
+  fl.Rgas = o.R;
+  fl.mu = o.mu;
+  const double Cp = o.R * o.gamma / (o.gamma - 1);
+  fl.k = (o.mu * Cp) / o.Pr;
+  fl.cfl2 = 2 * o.cfl;
+  fl.qinf[0] = o.rho;
+  fl.qinf[1] = o.rho * o.u;
+  fl.qinf[2] = o.rho * o.v;
+  fl.qinf[3] = (o.P / (o.gamma - 1)) + 0.5 * o.rho * (o.u * o.u + o.v * o.v);
+  fl.rho = o.rho;
+  fl.u = o.u;
+  fl.v = o.v;
+  fl.P = o.P;
+  fl.vn0 = std::sqrt(o.v * o.v + o.u * o.u);
+  fl.c0 = std::sqrt(o.gamma * (o.P / o.rho));
+  *out = c;
+  return FVK_OK;
+}
+
+int fvk_destroy(fvk_ctx* c) {
+  if (!c) return FVK_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->bufs) (void)hipFree(p);
+  c->bufs.clear();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return FVK_OK;
+}
+
+int fvk_init_freestream(fvk_ctx* c) {
+  if (!c) return FVK_ERR_ARG;
+  const size_t ncp = c->hm.ncp;
+  for (int k = 0; k < 4; ++k)
+    FVK_HIP(c, fvk::launch_fill(c->q[0] + k * ncp, (int64_t)ncp, c->fl.qinf[k], c->stream));
+  int rc = arm_start(c);
+  c->fs_prims = true;
+  return rc;
+}
+
+int fvk_set_state(fvk_ctx* c, const double* q) {
+  if (!c) return FVK_ERR_ARG;
+  if (!q) return set_err(c, FVK_ERR_ARG, "fvk_set_state: NULL state");
+  const fvk::HostMesh& h = c->hm;
+  std::vector<double> buf(4 * (size_t)h.ncp, 1.0);
+  for (int k = 0; k < 4; ++k)
+    for (int32_t i = 0; i < h.nc; ++i) buf[k * (size_t)h.ncp + i] = q[k * (size_t)h.nc + h.perm[i]];
+  FVK_HIP(c, hipMemcpyAsync(c->q[0], buf.data(), buf.size() * 8, hipMemcpyHostToDevice, c->stream));
+  int rc = arm_start(c);
+  c->fs_prims = false;
+  return rc;
+}
+
+int fvk_step(fvk_ctx* c, int32_t nsteps) {
+  if (!c) return FVK_ERR_ARG;
+  if (!c->have_state) return set_err(c, FVK_ERR_ARG, "fvk_step: no state (call fvk_init_freestream or fvk_set_state)");
+  if (nsteps < 0) return set_err(c, FVK_ERR_ARG, "fvk_step: negative step count");
+  auto plain = [c](int, auto&& fn) -> int {
+    hipError_t e = fn();
+    return e == hipSuccess ? FVK_OK : hip_err(c, e, "kernel launch");
+  };
+  for (int32_t n = 0; n < nsteps; ++n) {
+    int rc = run_step(c, n == nsteps - 1, plain);
+    if (rc) return rc;
+  }
+  FVK_HIP(c, hipStreamSynchronize(c->stream));
+  return FVK_OK;
+}
+
+int fvk_get_state(fvk_ctx* c, double* q, double* grad) {
+  if (!c) return FVK_ERR_ARG;
+  if (!q && !grad) return set_err(c, FVK_ERR_ARG, "fvk_get_state: nothing requested");
+  const fvk::HostMesh& h = c->hm;
+  const size_t ncp = h.ncp;
+  if (q) {
+    std::vector<double> buf(4 * ncp);
+    FVK_HIP(c, hipMemcpy(buf.data(), c->q[0], buf.size() * 8, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 4; ++k)
+      for (int32_t i = 0; i < h.nc; ++i) q[k * (size_t)h.nc + h.perm[i]] = buf[k * ncp + i];
+  }
+  if (grad) {
+    std::vector<double> buf(8 * ncp);
+    FVK_HIP(c, hipMemcpy(buf.data(), c->g[0], buf.size() * 8, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 8; ++k)
+      for (int32_t i = 0; i < h.nc; ++i) grad[k * (size_t)h.nc + h.perm[i]] = buf[k * ncp + i];
+  }
+  return FVK_OK;
+}
+
+int fvk_get_residual(fvk_ctx* c, double* r) {
+  if (!c) return FVK_ERR_ARG;
+  if (!r) return set_err(c, FVK_ERR_ARG, "fvk_get_residual: NULL output");
+  const fvk::HostMesh& h = c->hm;
+  std::vector<double> buf(4 * (size_t)h.ncp);
+  FVK_HIP(c, hipMemcpy(buf.data(), c->res, buf.size() * 8, hipMemcpyDeviceToHost));
+  for (int k = 0; k < 4; ++k)
+    for (int32_t i = 0; i < h.nc; ++i) r[k * (size_t)h.nc + h.perm[i]] = buf[k * (size_t)h.ncp + i];
+  return FVK_OK;
+}
+
+int fvk_residual_l2(fvk_ctx* c, double out[4]) {
+  if (!c || !out) return FVK_ERR_ARG;
+  const fvk::HostMesh& h = c->hm;
+  FVK_HIP(c, fvk::launch_res_sq(c->res, h.nc, h.ncp, c->partial, c->res_blocks, 256, c->stream));
+  std::vector<double> part(4 * (size_t)c->res_blocks);
+  FVK_HIP(c, hipMemcpyAsync(part.data(), c->partial, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  FVK_HIP(c, hipStreamSynchronize(c->stream));
+  bool finite = true;
+  for (int k = 0; k < 4; ++k) {
+    // Neumaier-compensated sum of the block partials
+    double s = 0.0, comp = 0.0;
+    for (int b = 0; b < c->res_blocks; ++b) {
+      const double x = part[k * (size_t)c->res_blocks + b];
+      const double t = s + x;
+      if (std::fabs(s) >= std::fabs(x)) comp += (s - t) + x;
+      else comp += (x - t) + s;
+      s = t;
+    }
+    out[k] = std::sqrt(s + comp);
+    if (!std::isfinite(out[k])) finite = false;
+  }
+  if (!finite) return set_err(c, FVK_ERR_NONFINITE, "residual norm is not finite");
+  return FVK_OK;
+}
+
+int fvk_get_geometry(fvk_ctx* c, const char* name, double* out) {
+  if (!c || !name || !out) return FVK_ERR_ARG;
+  const fvk::HostMesh& h = c->hm;
+  const std::vector<double>* v = nullptr;
+  const std::string n = name;
+  if (n == "face_area") v = &h.area;
+  else if (n == "face_nx") v = &h.nx;
+  else if (n == "face_ny") v = &h.ny;
+  else if (n == "face_centerx") v = &h.fcx;
+  else if (n == "face_centery") v = &h.fcy;
+  else if (n == "cell_vol") v = &h.vol;
+  else if (n == "cell_centerx") v = &h.ccx;
+  else if (n == "cell_centery") v = &h.ccy;
+  else if (n == "Ixx") v = &h.ixx;
+  else if (n == "Iyy") v = &h.iyy;
+  else if (n == "Ixy") v = &h.ixy;
+  if (!v) return set_err(c, FVK_ERR_ARG, "fvk_get_geometry: unknown array " + n);
+  std::memcpy(out, v->data(), v->size() * sizeof(double));
+  return FVK_OK;
+}
+
+int fvk_geometry(const fvk_mesh* mesh, const fvk_options* opt, const char* name, double* out) {
+  g_create_error.clear();
+  if (!mesh || !opt || !name || !out) return set_err(nullptr, FVK_ERR_ARG, "fvk_geometry: NULL argument");
+  fvk_ctx tmp;
+  std::string msg;
+  int rc = fvk::prepare_mesh(*mesh, *opt, &tmp.hm, &msg);
+  if (rc) return set_err(nullptr, rc, msg);
+  rc = fvk_get_geometry(&tmp, name, out);
+  if (rc) g_create_error = tmp.err;
+  return rc;
+}
+
+int fvk_get_info(fvk_ctx* c, int64_t info[6]) {
+  if (!c || !info) return FVK_ERR_ARG;
+  const fvk::HostMesh& h = c->hm;
+  info[0] = h.nc;
+  info[1] = h.nf;
+  info[2] = h.nb;
+  info[3] = h.chunk_off.empty() ? 0 : h.chunk_off.back();
+  info[4] = h.max_deg;
+  info[5] = h.num_wall;
+  return FVK_OK;
+}
+
+int fvk_time_steps(fvk_ctx* c, int32_t nsteps, double out_ms[4], int64_t counts[4]) {
+  if (!c || !out_ms || !counts) return FVK_ERR_ARG;
+  if (!c->have_state) return set_err(c, FVK_ERR_ARG, "fvk_time_steps: no state");
+  for (int i = 0; i < 4; ++i) { out_ms[i] = 0; counts[i] = 0; }
+  const int nev = 2 * 6 * (nsteps > 0 ? nsteps : 1) + 2;
+  std::vector<hipEvent_t> ev(nev);
+  for (auto& e : ev) FVK_HIP(c, hipEventCreate(&e));
+  std::vector<int> kinds;
+  int used = 2;
+  FVK_HIP(c, hipEventRecord(ev[0], c->stream));
+  auto timed = [&](int kind, auto&& fn) -> int {
+    hipError_t e = hipEventRecord(ev[used], c->stream);
+    if (e == hipSuccess) e = fn();
+    if (e == hipSuccess) e = hipEventRecord(ev[used + 1], c->stream);
+    if (e != hipSuccess) return hip_err(c, e, "timed launch");
+    kinds.push_back(kind);
+    used += 2;
+    return FVK_OK;
+  };
+  int rc = FVK_OK;
+  for (int32_t n = 0; n < nsteps && !rc; ++n) rc = run_step(c, n == nsteps - 1, timed);
+  if (!rc) {
+    FVK_HIP(c, hipEventRecord(ev[1], c->stream));
+    FVK_HIP(c, hipEventSynchronize(ev[1]));
+    float ms = 0;
+    FVK_HIP(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+    out_ms[0] = ms;
+    for (size_t i = 0; i < kinds.size(); ++i) {
+      float t = 0;
+      FVK_HIP(c, hipEventElapsedTime(&t, ev[2 + 2 * i], ev[3 + 2 * i]));
+      out_ms[kinds[i]] += t;
+      counts[kinds[i]] += 1;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
+}  // extern "C"

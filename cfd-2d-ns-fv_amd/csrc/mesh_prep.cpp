@@ -1,0 +1,357 @@
+// mesh_prep.cpp — geometry (bit-identical to src/base/readmesh.cpp) and the
+// RCM + sliced-ELL device layout.  Built with -ffp-contract=off: every FP64
+// expression below is evaluated exactly in the order written.
+#include "mesh_prep.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+
+namespace fvk {
+namespace {
+
+int bc_kind_of(const fvk_options& o, int32_t c2) {
+  if (c2 == -1) return o.bc1;  // readmesh.cpp:61-63
+  if (c2 == -2) return o.bc2;  // readmesh.cpp:64-66
+  return c2;                   // readmesh.cpp:67-69: raw -3 / -4
+}
+
+// Face normals/areas/centres (readmesh.cpp:117-139).
+void face_geometry(const fvk_mesh& m, HostMesh& h) {
+  h.area.resize(h.nf); h.nx.resize(h.nf); h.ny.resize(h.nf);
+  h.fcx.resize(h.nf); h.fcy.resize(h.nf);
+  for (int32_t f = 0; f < h.nf; ++f) {
+    const double xa = m.x[m.face_point1[f]], ya = m.y[m.face_point1[f]];
+    const double xb = m.x[m.face_point2[f]], yb = m.y[m.face_point2[f]];
+    const double dx = xb - xa, dy = yb - ya;
+    const double len = std::sqrt(dx * dx + dy * dy);
+    h.area[f] = len;
+    h.nx[f] = dy / len;
+    h.ny[f] = -dx / len;
+    h.fcx[f] = dx / 2 + xa;
+    h.fcy[f] = dy / 2 + ya;
+  }
+}
+
+// Cell volume and centroid (readmesh.cpp:147-228): triangles fanned about the
+// mean of the cell's distinct vertices.  The distinct-vertex search scans all
+// face-count slots, which start at (0,0), exactly as the reference does.
+void cell_geometry(const fvk_mesh& m, HostMesh& h) {
+  h.vol.resize(h.nc); h.ccx.resize(h.nc); h.ccy.resize(h.nc);
+  std::vector<double> vx(h.max_deg), vy(h.max_deg);
+  for (int32_t c = 0; c < h.nc; ++c) {
+    const int32_t b = h.cf_ptr[c], n = h.cf_ptr[c + 1] - b;
+    std::fill(vx.begin(), vx.begin() + n, 0.0);
+    std::fill(vy.begin(), vy.begin() + n, 0.0);
+    int32_t found = 0;
+    auto add_vertex = [&](double px, double py) {
+      for (int32_t z = 0; z < n; ++z)
+        if (vx[z] == px && vy[z] == py) return;
+      if (found < n) { vx[found] = px; vy[found] = py; ++found; }
+    };
+    for (int32_t j = 0; j < n; ++j) {
+      const int32_t f = h.cf_idx[b + j];
+      add_vertex(m.x[m.face_point1[f]], m.y[m.face_point1[f]]);
+      add_vertex(m.x[m.face_point2[f]], m.y[m.face_point2[f]]);
+    }
+    double sx = 0, sy = 0;
+    for (int32_t j = 0; j < n; ++j) { sx += vx[j]; sy += vy[j]; }
+    const double mx = sx / n, my = sy / n;
+    double vsum = 0, wx = 0, wy = 0;
+    for (int32_t j = 0; j < n; ++j) {
+      const int32_t f = h.cf_idx[b + j];
+      const double x1 = m.x[m.face_point1[f]], y1 = m.y[m.face_point1[f]];
+      const double x2 = m.x[m.face_point2[f]], y2 = m.y[m.face_point2[f]];
+      const double tri = 0.5 * std::fabs(x1 * (y2 - my) + x2 * (my - y1) + mx * (y1 - y2));
+      vsum += tri;
+      const double tx = (x1 + x2 + mx) / 3, ty = (y1 + y2 + my) / 3;
+      wx += tx * tri;
+      wy += ty * tri;
+    }
+    h.vol[c] = vsum;
+    h.ccx[c] = wx / vsum;
+    h.ccy[c] = wy / vsum;
+  }
+}
+
+// Mirror image of the owner centroid about a boundary face (readmesh.cpp:238-243).
+inline void ghost_centroid(const HostMesh& h, int32_t f, double* gx, double* gy) {
+  const int32_t c = h.fc1[f];
+  const double rx = h.fcx[f] - h.ccx[c];
+  const double ry = h.fcy[f] - h.ccy[c];
+  const double t = 2.0 * (rx * h.nx[f] + ry * h.ny[f]);
+  *gx = h.ccx[c] + t * h.nx[f];
+  *gy = h.ccy[c] + t * h.ny[f];
+}
+
+// LSQ normal-matrix inverse including ghost neighbours (readmesh.cpp:268-320),
+// accumulated per cell over its faces in increasing face index.
+void lsq_geometry(HostMesh& h) {
+  std::vector<double> sxx(h.nc, 0.0), syy(h.nc, 0.0), sxy(h.nc, 0.0);
+  for (int32_t f = 0; f < h.nf; ++f) {
+    const int32_t c1 = h.fc1[f], c2 = h.fc2[f];
+    const double xi = h.ccx[c1], yi = h.ccy[c1];
+    double xj, yj;
+    if (c2 >= 0) { xj = h.ccx[c2]; yj = h.ccy[c2]; }
+    else ghost_centroid(h, f, &xj, &yj);
+    sxx[c1] += (xi - xj) * (xi - xj);
+    syy[c1] += (yi - yj) * (yi - yj);
+    sxy[c1] += ((xi - xj) * (yi - yj));
+    if (c2 >= 0) {
+      sxx[c2] += (xj - xi) * (xj - xi);
+      syy[c2] += (yj - yi) * (yj - yi);
+      sxy[c2] += ((xj - xi) * (yj - yi));
+    }
+  }
+  h.ixx.resize(h.nc); h.iyy.resize(h.nc); h.ixy.resize(h.nc);
+  for (int32_t c = 0; c < h.nc; ++c) {
+    const double det = sxx[c] * syy[c] - (sxy[c] * sxy[c]);
+    h.ixx[c] = sxx[c] / det;
+    h.iyy[c] = syy[c] / det;
+    h.ixy[c] = sxy[c] / det;
+  }
+}
+
+// Reverse Cuthill-McKee over the interior-face graph.  Each connected part
+// starts from a pseudo-peripheral cell (two BFS sweeps from a min-degree cell).
+std::vector<int32_t> rcm_order(const HostMesh& h) {
+  const int32_t n = h.nc;
+  std::vector<int32_t> aptr(n + 1, 0), adj;
+  for (int32_t c = 0; c < n; ++c) {
+    int32_t d = 0;
+    for (int32_t j = h.cf_ptr[c]; j < h.cf_ptr[c + 1]; ++j)
+      if (h.fc2[h.cf_idx[j]] >= 0) ++d;
+    aptr[c + 1] = aptr[c] + d;
+  }
+  adj.resize(aptr[n]);
+  for (int32_t c = 0; c < n; ++c) {
+    int32_t k = aptr[c];
+    for (int32_t j = h.cf_ptr[c]; j < h.cf_ptr[c + 1]; ++j) {
+      const int32_t f = h.cf_idx[j];
+      if (h.fc2[f] < 0) continue;
+      adj[k++] = (h.fc1[f] == c) ? h.fc2[f] : h.fc1[f];
+    }
+  }
+  auto degree = [&](int32_t c) { return aptr[c + 1] - aptr[c]; };
+  std::vector<int32_t> level(n, -1), order;
+  order.reserve(n);
+  std::vector<char> placed(n, 0);
+  std::vector<int32_t> queue;
+  queue.reserve(n);
+  // BFS from s; returns the last level's min-degree cell and the depth
+  auto bfs_far = [&](int32_t s, std::vector<int32_t>& touched) {
+    queue.clear();
+    queue.push_back(s);
+    level[s] = 0;
+    touched.push_back(s);
+    size_t head = 0;
+    int32_t far = s;
+    while (head < queue.size()) {
+      const int32_t c = queue[head++];
+      if (level[c] > level[far] || (level[c] == level[far] && degree(c) < degree(far))) far = c;
+      for (int32_t k = aptr[c]; k < aptr[c + 1]; ++k)
+        if (level[adj[k]] < 0) {
+          level[adj[k]] = level[c] + 1;
+          queue.push_back(adj[k]);
+          touched.push_back(adj[k]);
+        }
+    }
+    return std::make_pair(far, level[far]);
+  };
+  std::vector<int32_t> by_degree(n);
+  std::iota(by_degree.begin(), by_degree.end(), 0);
+  std::stable_sort(by_degree.begin(), by_degree.end(),
+                   [&](int32_t a, int32_t b) { return degree(a) < degree(b); });
+  std::vector<int32_t> touched, nbrs;
+  for (int32_t start : by_degree) {
+    if (placed[start]) continue;
+    int32_t s = start, depth = -1;
+    for (int sweep = 0; sweep < 3; ++sweep) {
+      touched.clear();
+      auto r = bfs_far(s, touched);
+      for (int32_t c : touched) level[c] = -1;
+      if (r.second <= depth) break;
+      depth = r.second;
+      s = r.first;
+    }
+    // Cuthill-McKee BFS from s
+    size_t first = order.size();
+    order.push_back(s);
+    placed[s] = 1;
+    for (size_t head = first; head < order.size(); ++head) {
+      const int32_t c = order[head];
+      nbrs.clear();
+      for (int32_t k = aptr[c]; k < aptr[c + 1]; ++k)
+        if (!placed[adj[k]]) { nbrs.push_back(adj[k]); placed[adj[k]] = 1; }
+      std::stable_sort(nbrs.begin(), nbrs.end(),
+                       [&](int32_t a, int32_t b) { return degree(a) < degree(b); });
+      order.insert(order.end(), nbrs.begin(), nbrs.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+}  // namespace
+
+int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::string* err) {
+  HostMesh& h = *out;
+  h = HostMesh();
+  char buf[256];
+  if (m.num_points <= 0 || m.num_faces <= 0 || m.num_cells <= 0 || !m.x || !m.y ||
+      !m.face_point1 || !m.face_point2 || !m.face_cell1 || !m.face_cell2) {
+    *err = "mesh: empty or NULL arrays";
+    return FVK_ERR_ARG;
+  }
+  if (m.num_cells >= (1 << 29)) {
+    *err = "mesh: more than 2^29 cells";
+    return FVK_ERR_MESH;
+  }
+  h.np = m.num_points; h.nf = m.num_faces; h.nc = m.num_cells;
+  h.fc1.resize(h.nf); h.fc2.resize(h.nf);
+  std::vector<int32_t> cnt(h.nc, 0);
+  for (int32_t f = 0; f < h.nf; ++f) {
+    const int32_t p1 = m.face_point1[f], p2 = m.face_point2[f];
+    const int32_t c1 = m.face_cell1[f], c2 = m.face_cell2[f];
+    if (p1 < 0 || p1 >= h.np || p2 < 0 || p2 >= h.np || p1 == p2 || c1 < 0 || c1 >= h.nc ||
+        c2 >= h.nc || c2 < -4 || c2 == c1) {
+      snprintf(buf, sizeof buf, "mesh: face %d has an out-of-range or degenerate entry", f);
+      *err = buf;
+      return FVK_ERR_MESH;
+    }
+    h.fc1[f] = c1;
+    h.fc2[f] = (c2 >= 0) ? c2 : bc_kind_of(o, c2);
+    if (c2 == -1) ++h.num_bc1;
+    if (c2 == -2) ++h.num_bc2;
+    if (c2 < 0) ++h.nb;
+    cnt[c1]++;
+    if (c2 >= 0) cnt[c2]++;
+  }
+  for (int32_t f = 0; f < h.nf; ++f)
+    if (h.fc2[f] < 0 && (h.fc2[f] < -4 || h.fc2[f] > -1)) {
+      *err = "options: BC kind must be -1..-4";
+      return FVK_ERR_OPTION;
+    }
+  h.cf_ptr.assign(h.nc + 1, 0);
+  for (int32_t c = 0; c < h.nc; ++c) {
+    if (cnt[c] < 3) {
+      snprintf(buf, sizeof buf, "mesh: cell %d has %d faces (needs at least 3)", c, cnt[c]);
+      *err = buf;
+      return FVK_ERR_MESH;
+    }
+    if (cnt[c] > 255) {
+      snprintf(buf, sizeof buf, "mesh: cell %d has %d faces (at most 255)", c, cnt[c]);
+      *err = buf;
+      return FVK_ERR_MESH;
+    }
+    h.cf_ptr[c + 1] = h.cf_ptr[c] + cnt[c];
+    h.max_deg = std::max(h.max_deg, cnt[c]);
+  }
+  h.cf_idx.resize(h.cf_ptr[h.nc]);
+  std::fill(cnt.begin(), cnt.end(), 0);
+  for (int32_t f = 0; f < h.nf; ++f) {
+    const int32_t c1 = h.fc1[f], c2 = h.fc2[f];
+    h.cf_idx[h.cf_ptr[c1] + cnt[c1]++] = f;
+    if (c2 >= 0) h.cf_idx[h.cf_ptr[c2] + cnt[c2]++] = f;
+  }
+  // wall count, readmesh.cpp:356-360
+  if (o.bc1 == FVK_BC_SLIP_WALL || o.bc1 == FVK_BC_NO_SLIP_WALL) h.num_wall = h.num_bc1;
+  else if (o.bc2 == FVK_BC_SLIP_WALL || o.bc2 == FVK_BC_NO_SLIP_WALL) h.num_wall = h.num_bc2;
+
+  face_geometry(m, h);
+  cell_geometry(m, h);
+  lsq_geometry(h);
+
+  // The reference's no-slip gradient ghost reads the state of the cell whose
+  // index equals the face index (cell_gradient.cpp:63-66).
+  int32_t noslip_mismatch = 0;
+  for (int32_t f = 0; f < h.nf; ++f) {
+    if (h.fc2[f] != FVK_BC_NO_SLIP_WALL) continue;
+    if (f >= h.nc) {
+      if (o.order == 2) {
+        snprintf(buf, sizeof buf,
+                 "mesh: no-slip face %d has no cell of the same index; second-order no-slip "
+                 "walls need boundary faces first with wall face k owned by cell k",
+                 f);
+        *err = buf;
+        return FVK_ERR_MESH;
+      }
+    } else if (h.fc1[f] != f) {
+      ++noslip_mismatch;
+    }
+  }
+  if (noslip_mismatch && o.order == 2) {
+    snprintf(buf, sizeof buf,
+             "warning: %d no-slip faces are not owned by the cell of the same index; the "
+             "gradient ghost uses that cell's state, as the reference does\n",
+             noslip_mismatch);
+    h.warnings += buf;
+  }
+  int32_t first_interior = -1;
+  for (int32_t f = 0; f < h.nf; ++f)
+    if (h.fc2[f] >= 0) { first_interior = f; break; }
+  for (int32_t f = first_interior >= 0 ? first_interior : h.nf; f < h.nf; ++f)
+    if (h.fc2[f] < 0) {
+      h.warnings += "warning: boundary faces are not all first (the reference reader reads "
+                    "past its boundary arrays on such grids)\n";
+      break;
+    }
+
+  // renumbering
+  if (o.renumber) h.perm = rcm_order(h);
+  else { h.perm.resize(h.nc); std::iota(h.perm.begin(), h.perm.end(), 0); }
+  h.iperm.assign(h.nc, 0);
+  for (int32_t i = 0; i < h.nc; ++i) h.iperm[h.perm[i]] = i;
+
+  // sliced-ELL slot table
+  h.ncp = (h.nc + kChunk - 1) / kChunk * kChunk;
+  const int32_t nchunks = h.ncp / kChunk;
+  h.deg.assign(h.ncp, 0);
+  for (int32_t i = 0; i < h.nc; ++i) {
+    const int32_t c = h.perm[i];
+    h.deg[i] = (uint8_t)(h.cf_ptr[c + 1] - h.cf_ptr[c]);
+  }
+  h.chunk_off.assign(nchunks + 1, 0);
+  for (int32_t k = 0; k < nchunks; ++k) {
+    int32_t mx = 0;
+    for (int32_t l = 0; l < kChunk; ++l) mx = std::max<int32_t>(mx, h.deg[k * kChunk + l]);
+    h.chunk_off[k + 1] = h.chunk_off[k] + mx * kChunk;
+  }
+  const int64_t nslots = h.chunk_off[nchunks];
+  h.code.assign(nslots, 0);
+  h.s_nx.assign(nslots, 0.0); h.s_ny.assign(nslots, 0.0); h.s_area.assign(nslots, 0.0);
+  h.s_fcx.assign(nslots, 0.0); h.s_fcy.assign(nslots, 0.0);
+  for (int32_t i = 0; i < h.nc; ++i) {
+    const int32_t c = h.perm[i];
+    const int32_t base = h.chunk_off[i / kChunk] + (i % kChunk);
+    for (int32_t j = h.cf_ptr[c]; j < h.cf_ptr[c + 1]; ++j) {
+      const int32_t f = h.cf_idx[j];
+      const int64_t s = base + (int64_t)(j - h.cf_ptr[c]) * kChunk;
+      const int32_t c2 = h.fc2[f];
+      if (c2 >= 0) {
+        const bool owner = (h.fc1[f] == c);
+        h.code[s] = slot_code_interior(h.iperm[owner ? c2 : h.fc1[f]], owner);
+      } else {
+        const int kind = -c2 - 1;
+        const int32_t src = (c2 == FVK_BC_NO_SLIP_WALL && f < h.nc) ? h.iperm[f] : 0;
+        h.code[s] = slot_code_boundary(src, kind);
+      }
+      h.s_nx[s] = h.nx[f]; h.s_ny[s] = h.ny[f]; h.s_area[s] = h.area[f];
+      h.s_fcx[s] = h.fcx[f]; h.s_fcy[s] = h.fcy[f];
+    }
+  }
+  auto permute = [&](const std::vector<double>& src, std::vector<double>& dst, double pad) {
+    dst.assign(h.ncp, pad);
+    for (int32_t i = 0; i < h.nc; ++i) dst[i] = src[h.perm[i]];
+  };
+  permute(h.ccx, h.c_ccx, 0.0);
+  permute(h.ccy, h.c_ccy, 0.0);
+  permute(h.vol, h.c_vol, 1.0);
+  permute(h.ixx, h.c_ixx, 0.0);
+  permute(h.iyy, h.c_iyy, 0.0);
+  permute(h.ixy, h.c_ixy, 0.0);
+  return FVK_OK;
+}
+
+}  // namespace fvk

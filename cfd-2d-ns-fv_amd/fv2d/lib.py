@@ -1,0 +1,222 @@
+"""ctypes binding of libfvk.so (include/fvk.h).
+
+The binding is deliberately thin: it only converts numpy arrays to the C ABI's
+plain pointers.  It never falls back to any CPU path: if libfvk.so is missing
+it raises, and every solver call runs the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libfvk.so")
+CLI_PATH = os.path.join(PKG_DIR, "lib", "fv2d")
+
+FVK_OK = 0
+ERRORS = {-1: "FVK_ERR_ARG", -2: "FVK_ERR_MESH", -3: "FVK_ERR_OPTION", -4: "FVK_ERR_HIP",
+          -5: "FVK_ERR_NONFINITE", -6: "FVK_ERR_UNSUPPORTED"}
+BC_CODES = {"FREESTREAM": -1, "SLIP_WALL": -2, "EXTRAPOLATION": -3, "NO_SLIP_WALL": -4}
+
+# every symbol include/fvk.h declares
+EXPORTS = ["fvk_default_options", "fvk_create", "fvk_destroy", "fvk_last_error",
+           "fvk_init_freestream", "fvk_set_state", "fvk_step", "fvk_get_state",
+           "fvk_get_residual", "fvk_residual_l2", "fvk_get_geometry", "fvk_geometry",
+           "fvk_get_info", "fvk_time_steps"]
+
+
+class FvkError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class FvkMesh(ctypes.Structure):
+    _fields_ = [("num_points", ctypes.c_int32), ("num_faces", ctypes.c_int32),
+                ("num_cells", ctypes.c_int32), ("reserved_", ctypes.c_int32),
+                ("x", ctypes.POINTER(ctypes.c_double)), ("y", ctypes.POINTER(ctypes.c_double)),
+                ("face_point1", ctypes.POINTER(ctypes.c_int32)),
+                ("face_point2", ctypes.POINTER(ctypes.c_int32)),
+                ("face_cell1", ctypes.POINTER(ctypes.c_int32)),
+                ("face_cell2", ctypes.POINTER(ctypes.c_int32))]
+
+
+class FvkOptions(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int32), ("limiter", ctypes.c_int32), ("eqn", ctypes.c_int32),
+                ("flux_solver", ctypes.c_int32), ("time_scheme", ctypes.c_int32),
+                ("time_step_type", ctypes.c_int32), ("bc1", ctypes.c_int32), ("bc2", ctypes.c_int32),
+                ("cfl", ctypes.c_double), ("gamma", ctypes.c_double), ("R", ctypes.c_double),
+                ("mu", ctypes.c_double), ("Pr", ctypes.c_double), ("P", ctypes.c_double),
+                ("rho", ctypes.c_double), ("u", ctypes.c_double), ("v", ctypes.c_double),
+                ("precision", ctypes.c_int32), ("renumber", ctypes.c_int32),
+                ("block_size", ctypes.c_int32), ("xcd_remap", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libfvk.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with `make -C {PKG_DIR}`")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        vp = ctypes.c_void_p
+        dp = P(ctypes.c_double)
+        L.fvk_default_options.argtypes = [P(FvkOptions)]
+        L.fvk_default_options.restype = None
+        L.fvk_create.argtypes = [P(FvkMesh), P(FvkOptions), P(vp)]
+        L.fvk_destroy.argtypes = [vp]
+        L.fvk_last_error.argtypes = [vp]
+        L.fvk_last_error.restype = ctypes.c_char_p
+        L.fvk_init_freestream.argtypes = [vp]
+        L.fvk_set_state.argtypes = [vp, dp]
+        L.fvk_step.argtypes = [vp, ctypes.c_int32]
+        L.fvk_get_state.argtypes = [vp, dp, dp]
+        L.fvk_get_residual.argtypes = [vp, dp]
+        L.fvk_residual_l2.argtypes = [vp, dp]
+        L.fvk_get_geometry.argtypes = [vp, ctypes.c_char_p, dp]
+        L.fvk_geometry.argtypes = [P(FvkMesh), P(FvkOptions), ctypes.c_char_p, dp]
+        L.fvk_get_info.argtypes = [vp, P(ctypes.c_int64)]
+        L.fvk_time_steps.argtypes = [vp, ctypes.c_int32, dp, P(ctypes.c_int64)]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def make_options(case, **overrides) -> FvkOptions:
+    """FvkOptions from a fv2d.inputs.Case (read_inputs.cpp enum codes)."""
+    o = FvkOptions()
+    lib().fvk_default_options(ctypes.byref(o))
+    o.order = case.poly + 1
+    o.limiter = {"NONE": 0, "MINMOD": 1, "SQUEEZE": 2}[case.limiter]
+    o.eqn = {"EULER": 1, "NS": 2}[case.eqn]
+    o.flux_solver = {"RUSANOV": 1, "ROE": 2}[case.flux]
+    o.time_scheme = {"SSP_RK2": 1, "SSP_RK3": 2}[case.scheme]
+    o.time_step_type = {"GLOBAL": 1, "LOCAL": 2}[case.dt_type]
+    o.bc1 = BC_CODES[case.BC1]
+    o.bc2 = BC_CODES[case.BC2]
+    o.cfl = case.CFL
+    o.gamma, o.R, o.mu, o.Pr = case.gamma, case.R, case.mu, case.Pr
+    o.P, o.rho, o.u, o.v = case.P, case.rho, case.u, case.v
+    for k, v in overrides.items():
+        setattr(o, k, v)
+    return o
+
+
+class _MeshArrays:
+    def __init__(self, grid):
+        self.x = np.ascontiguousarray(grid.x, dtype=np.float64)
+        self.y = np.ascontiguousarray(grid.y, dtype=np.float64)
+        self.p1 = np.ascontiguousarray(grid.face_p1, dtype=np.int32)
+        self.p2 = np.ascontiguousarray(grid.face_p2, dtype=np.int32)
+        self.c1 = np.ascontiguousarray(grid.face_c1, dtype=np.int32)
+        self.c2 = np.ascontiguousarray(grid.reader_c2(), dtype=np.int32)
+        self.m = FvkMesh(grid.num_points, grid.num_faces, grid.num_cells, 0, _dp(self.x),
+                         _dp(self.y), _ip(self.p1), _ip(self.p2), _ip(self.c1), _ip(self.c2))
+
+
+def host_geometry(grid, case, name: str) -> np.ndarray:
+    """Geometry array from the host-only fvk_geometry (no device needed)."""
+    L = lib()
+    ma = _MeshArrays(grid)
+    opt = make_options(case)
+    n = grid.num_cells if name in ("cell_vol", "cell_centerx", "cell_centery", "Ixx", "Iyy", "Ixy") \
+        else grid.num_faces
+    out = np.empty(n, dtype=np.float64)
+    rc = L.fvk_geometry(ctypes.byref(ma.m), ctypes.byref(opt), name.encode(), _dp(out))
+    if rc != FVK_OK:
+        raise FvkError(rc, L.fvk_last_error(None).decode())
+    return out
+
+
+class Solver:
+    """One device solver context (fvk_ctx)."""
+
+    def __init__(self, grid, case, **overrides):
+        L = lib()
+        self._ma = _MeshArrays(grid)
+        self.opt = make_options(case, **overrides)
+        h = ctypes.c_void_p()
+        rc = L.fvk_create(ctypes.byref(self._ma.m), ctypes.byref(self.opt), ctypes.byref(h))
+        if rc != FVK_OK:
+            raise FvkError(rc, L.fvk_last_error(None).decode())
+        self.h = h
+        self.nc = grid.num_cells
+        self.nf = grid.num_faces
+
+    def _check(self, rc):
+        if rc != FVK_OK:
+            raise FvkError(rc, lib().fvk_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fvk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def init_freestream(self):
+        self._check(lib().fvk_init_freestream(self.h))
+
+    def set_state(self, q: np.ndarray):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(4 * self.nc)
+        self._check(lib().fvk_set_state(self.h, _dp(q)))
+
+    def step(self, n: int = 1):
+        self._check(lib().fvk_step(self.h, n))
+
+    def state(self) -> np.ndarray:
+        q = np.empty(4 * self.nc)
+        self._check(lib().fvk_get_state(self.h, _dp(q), None))
+        return q.reshape(4, self.nc)
+
+    def state_and_grad(self):
+        q = np.empty(4 * self.nc)
+        g = np.empty(8 * self.nc)
+        self._check(lib().fvk_get_state(self.h, _dp(q), _dp(g)))
+        return q.reshape(4, self.nc), g.reshape(8, self.nc)
+
+    def residual(self) -> np.ndarray:
+        r = np.empty(4 * self.nc)
+        self._check(lib().fvk_get_residual(self.h, _dp(r)))
+        return r.reshape(4, self.nc)
+
+    def residual_l2(self) -> np.ndarray:
+        out = np.empty(4)
+        rc = lib().fvk_residual_l2(self.h, _dp(out))
+        if rc not in (FVK_OK, -5):
+            self._check(rc)
+        return out
+
+    def geometry(self, name: str) -> np.ndarray:
+        n = self.nc if name in ("cell_vol", "cell_centerx", "cell_centery", "Ixx", "Iyy", "Ixy") \
+            else self.nf
+        out = np.empty(n)
+        self._check(lib().fvk_get_geometry(self.h, name.encode(), _dp(out)))
+        return out
+
+    def info(self) -> dict:
+        a = (ctypes.c_int64 * 6)()
+        self._check(lib().fvk_get_info(self.h, a))
+        keys = ["num_cells", "num_faces", "num_boundary_faces", "num_slots", "max_faces",
+                "num_wall"]
+        return dict(zip(keys, list(a)))
+
+    def time_steps(self, n: int):
+        ms = np.zeros(4)
+        cnt = (ctypes.c_int64 * 4)()
+        self._check(lib().fvk_time_steps(self.h, n, _dp(ms), cnt))
+        return ms, list(cnt)

@@ -1,0 +1,137 @@
+/* fvk.h — C ABI of the MI355X explicit unstructured FV residual-and-update engine.
+ *
+ * The library replaces the reference's solver library seam: the free functions
+ * of the src/include headers that src/base/main.cpp calls every iteration
+ * (ssp_rk3 -> res -> compute_gradient / squeeze / reconstruction / rusanov),
+ * together with the per-run set-up that main.cpp does around them.  Plain C
+ * types only; all calls are synchronous with respect to host buffers; one
+ * context per host thread; the device memory belongs to the context.
+ *
+ * Numbering: every array crossing this boundary is in the caller's (grid
+ * file) numbering.  The library renumbers cells internally.
+ *
+ * Errors: every call returns FVK_OK (0) or a negative FVK_ERR_* code and never
+ * exits; fvk_last_error() gives the message.
+ */
+#ifndef FVK_H
+#define FVK_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVK_OK 0
+#define FVK_ERR_ARG (-1)          /* NULL pointer, bad size */
+#define FVK_ERR_MESH (-2)         /* index out of range, degenerate cell */
+#define FVK_ERR_OPTION (-3)       /* option value the reference does not define */
+#define FVK_ERR_HIP (-4)          /* HIP runtime error (no device, launch failure) */
+#define FVK_ERR_NONFINITE (-5)    /* non-finite residual norm */
+#define FVK_ERR_UNSUPPORTED (-6)  /* option the reference declares but never ran (ROE) */
+
+/* BC kinds, read_inputs.cpp:171-205 (struct_BC values) */
+#define FVK_BC_FREESTREAM (-1)
+#define FVK_BC_SLIP_WALL (-2)
+#define FVK_BC_EXTRAPOLATION (-3)
+#define FVK_BC_NO_SLIP_WALL (-4)
+
+/* precision forms */
+#define FVK_FP64 0         /* plain FP64: reproduces the reference */
+
+typedef struct fvk_ctx fvk_ctx;
+
+/* The mesh as src/base/readmesh.cpp:31-85 reads it (class_mesh.hpp:7-14),
+ * 0-based.  face_cell2 < 0 is the file marker minus one, before the BC kind is
+ * substituted: -1 = BC1 group, -2 = BC2 group, -3 = EXTRAPOLATION and
+ * -4 = NO_SLIP_WALL (readmesh.cpp:59-69).  Cells may have any number (>= 3) of
+ * faces. */
+typedef struct fvk_mesh {
+  int32_t num_points;
+  int32_t num_faces;
+  int32_t num_cells;
+  int32_t reserved_;
+  const double* x;
+  const double* y;
+  const int32_t* face_point1;
+  const int32_t* face_point2;
+  const int32_t* face_cell1;
+  const int32_t* face_cell2;
+} fvk_mesh;
+
+/* The input.in options (struct_inputs.hpp, struct_time.hpp, struct_BC.hpp,
+ * class_flow.hpp) with read_inputs.cpp's enum codes. */
+typedef struct fvk_options {
+  int32_t order;           /* solution_poly_order + 1: 1 or 2 */
+  int32_t limiter;         /* 0 NONE, 1 MINMOD (a no-op, res.cpp:36-38), 2 SQUEEZE */
+  int32_t eqn;             /* 1 EULER, 2 NS */
+  int32_t flux_solver;     /* 1 RUSANOV (2 ROE -> FVK_ERR_UNSUPPORTED) */
+  int32_t time_scheme;     /* 1 SSP_RK2 (runs SSP-RK3, main.cpp:123-126), 2 SSP_RK3 */
+  int32_t time_step_type;  /* 1 GLOBAL, 2 LOCAL */
+  int32_t bc1;             /* kind of the BC1 group (FVK_BC_*) */
+  int32_t bc2;             /* kind of the BC2 group (FVK_BC_*) */
+  double cfl;
+  double gamma, R, mu, Pr; /* [fluid] */
+  double P, rho, u, v;     /* [init] freestream */
+  int32_t precision;       /* FVK_FP64 */
+  int32_t renumber;        /* 1: reverse Cuthill-McKee cell order (default), 0: file order */
+  int32_t block_size;      /* 0: default (256); else 64, 128, 192 or 256 */
+  int32_t xcd_remap;       /* 1: contiguous cell ranges per XCD (default), 0: off */
+} fvk_options;
+
+/* Fill *opt with the defaults above for the given BC kinds. */
+void fvk_default_options(fvk_options* opt);
+
+/* Set-up: geometry of readmesh.cpp:113-411 (bit-identical), device layout and
+ * memory.  Replaces readmesh's geometry half and main.cpp:75-80. */
+int fvk_create(const fvk_mesh* mesh, const fvk_options* opt, fvk_ctx** out);
+int fvk_destroy(fvk_ctx* ctx);
+/* Message of the last failing call on ctx (or of the last fvk_create when
+ * ctx is NULL). */
+const char* fvk_last_error(const fvk_ctx* ctx);
+
+/* Uniform freestream start: Qbar.init + updateQ and the face-state
+ * initialisation of main.cpp:107-118 (limiter sees the -101 sentinel, the
+ * first time step uses |V_inf| and c_inf on every face). */
+int fvk_init_freestream(fvk_ctx* ctx);
+/* Restart start, main.cpp:82-106 and 114-118: conserved state q[4*N_c]
+ * (p1 block, p2 block, p3 block, p4 block), primitives recomputed, the face
+ * states re-initialised as on a fresh start. */
+int fvk_set_state(fvk_ctx* ctx, const double* q);
+
+/* nsteps calls of ssp_rk3 (ssp_rk3.cpp:18-77).  The stage-3 residual of the
+ * last step is kept for fvk_get_residual / fvk_residual_l2. */
+int fvk_step(fvk_ctx* ctx, int32_t nsteps);
+
+/* Conserved state q[4*N_c]; grad[8*N_c] (nullable) = the stage-1 limited
+ * gradients Qxp1..4, Qyp1..4 that Qbar holds after ssp_rk3 and that
+ * save_sol.cpp:34-37,123-142 reads (zeros at first order). */
+int fvk_get_state(fvk_ctx* ctx, double* q, double* grad);
+/* Stage-3 residual r[4*N_c] of the last step (class_residual after ssp_rk3). */
+int fvk_get_residual(fvk_ctx* ctx, double* r);
+/* sqrt(sum_i R_k,i^2), k = 1..4, of that residual (save_sol.cpp:79-89);
+ * FVK_ERR_NONFINITE if any is not finite (out[] is still written). */
+int fvk_residual_l2(fvk_ctx* ctx, double out[4]);
+
+/* Geometry arrays of class_mesh, original numbering, for checks:
+ * "face_area", "face_nx", "face_ny", "face_centerx", "face_centery" (N_f);
+ * "cell_vol", "cell_centerx", "cell_centery", "Ixx", "Iyy", "Ixy" (N_c). */
+int fvk_get_geometry(fvk_ctx* ctx, const char* name, double* out);
+
+/* Host-only (no device needed): the same geometry straight from a mesh, i.e.
+ * the geometry half of readmesh.cpp:113-320. */
+int fvk_geometry(const fvk_mesh* mesh, const fvk_options* opt, const char* name, double* out);
+
+/* Sizes: [num_cells, num_faces, num_boundary_faces, num_slots, max_faces_per_cell,
+ * num_wall_faces (readmesh.cpp:356-360)]. */
+int fvk_get_info(fvk_ctx* ctx, int64_t info[6]);
+
+/* Timing: run nsteps with a HIP event pair around every launch on the
+ * context's stream.  out_ms[0] = wall time of the nsteps (event-timed),
+ * out_ms[1..] = per kernel kind total time: [1] pass A (gradient+limiter),
+ * [2] pass B (flux+residual+update), [3] other.  counts[0..3] launches. */
+int fvk_time_steps(fvk_ctx* ctx, int32_t nsteps, double out_ms[4], int64_t counts[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,163 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every
+declared symbol, the host geometry is bit-identical to the reference reader,
+mesh/option validation, the grid generators and the grid/input file formats."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from fv2d import gridgen as gg
+from fv2d import inputs
+from fv2d import lib as F
+from fvtest import Golden, golden_names, mismatch, same
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "fvk.h")).read()
+    declared = set(re.findall(r"\b(fvk_[a-z_0-9]+)\s*\(", hdr))
+    assert declared == set(F.EXPORTS)
+    L = ctypes.CDLL(F.LIB_PATH)
+    for name in declared:
+        assert hasattr(L, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", F.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    for name in declared:
+        assert re.search(rf"\bT {name}$", out, re.M), name
+
+
+def test_struct_layout_matches_header():
+    # offsets the C compiler gives the header structs
+    src = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "fvk.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(fvk_mesh), offsetof(fvk_mesh, face_cell2),
+         sizeof(fvk_options), offsetof(fvk_options, xcd_remap));
+  return 0;
+}
+"""
+    d = os.path.join(ROOT, "_work", "abi")
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "abi.c"), "w") as f:
+        f.write(src)
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), "-o",
+                           os.path.join(d, "abi"), os.path.join(d, "abi.c")])
+    got = subprocess.run([os.path.join(d, "abi")], capture_output=True, text=True).stdout.split()
+    assert [int(v) for v in got] == [ctypes.sizeof(F.FvkMesh), F.FvkMesh.face_cell2.offset,
+                                     ctypes.sizeof(F.FvkOptions), F.FvkOptions.xcd_remap.offset]
+
+
+GEOM_FACE = ["face_area", "face_nx", "face_ny", "face_centerx", "face_centery"]
+GEOM_CELL = ["cell_vol", "cell_centerx", "cell_centery", "Ixx", "Iyy", "Ixy"]
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_host_geometry_bit_identical_to_reference(name):
+    g = Golden(name)
+    for key in GEOM_FACE + GEOM_CELL:
+        a = F.host_geometry(g.grid, g.case, key)
+        ref = g.a[f"geom/{key}"]
+        assert same(a, ref), f"{key}: {mismatch(a, ref)}"
+        assert np.array_equal(a.view(np.uint64), ref.view(np.uint64)), key
+
+
+def _mesh_err(grid, case):
+    with pytest.raises(F.FvkError) as e:
+        F.host_geometry(grid, case, "cell_vol")
+    return e.value
+
+
+def test_mesh_validation():
+    case = inputs.Case()
+    g = gg.box(3, 3)
+    bad = gg.Grid(**{**g.__dict__, "face_p1": g.face_p1.copy()})
+    bad.face_p1[4] = g.num_points + 5
+    assert _mesh_err(bad, case).code == -2
+    bad = gg.Grid(**{**g.__dict__, "face_c1": g.face_c1.copy()})
+    bad.face_c1[0] = -7
+    assert _mesh_err(bad, case).code == -2
+    # a cell left with two faces
+    keep = np.ones(g.num_faces, bool)
+    c0 = np.nonzero((g.face_c1 == 0) | (g.face_c2 == 0))[0]
+    keep[c0[:2]] = False
+    bad = gg.Grid(x=g.x, y=g.y, face_p1=g.face_p1[keep], face_p2=g.face_p2[keep],
+                  face_c1=g.face_c1[keep], face_c2=g.face_c2[keep], bc_marker=g.bc_marker[keep],
+                  num_cells=g.num_cells)
+    assert "faces" in str(_mesh_err(bad, case))
+
+
+def test_noslip_second_order_needs_face_index_cell():
+    # a no-slip face whose index exceeds the cell count: the reference would
+    # read past its state arrays (cell_gradient.cpp:63-66); refused here
+    g = gg.box(2, 2, markers=(-1, -1, -1, -3))
+    idx = np.nonzero(g.bc_marker == -3)[0]
+    assert idx.max() >= g.num_cells
+    case = inputs.Case(poly=1, BC1="SLIP_WALL", BC2="FREESTREAM")
+    assert _mesh_err(g, case).code == -2
+    F.host_geometry(g, inputs.Case(poly=0), "cell_vol")   # first order never reads it
+
+
+def test_polygons_with_more_than_six_sides():
+    # a regular nonagon surrounded by triangles: closed-form area and centroid
+    n = 9
+    th = 2 * np.pi * np.arange(n) / n
+    x = np.concatenate([np.cos(th), 2 * np.cos(th + np.pi / n)])
+    y = np.concatenate([np.sin(th), 2 * np.sin(th + np.pi / n)])
+    cells = [list(range(n))]
+    for k in range(n):
+        cells.append([k, (k + 1) % n, n + k])
+    padded = np.full((len(cells), n), -1)
+    nv = np.array([len(c) for c in cells])
+    for i, c in enumerate(cells):
+        padded[i, :len(c)] = c
+    g = gg.build_grid(x, y, padded, nv, lambda a, b: np.full(a.shape[0], -1), "nonagon")
+    vol = F.host_geometry(g, inputs.Case(BC1="FREESTREAM"), "cell_vol")
+    assert abs(vol[0] - 0.5 * n * np.sin(2 * np.pi / n)) < 1e-14
+    cx = F.host_geometry(g, inputs.Case(BC1="FREESTREAM"), "cell_centerx")
+    assert abs(cx[0]) < 1e-15
+
+
+def test_generators_are_canonical_and_sized():
+    b = gg.bump()
+    assert (b.num_cells, b.num_faces, int((b.face_c2 < 0).sum())) == (2048, 4192, 192)
+    r = gg.ramp()
+    assert (r.num_cells, r.num_faces, int((r.face_c2 < 0).sum())) == (196608, 328448, 1536)
+    o = gg.ogrid(64, 32)
+    for g in (b, r, o):
+        assert gg.check_canonical(g) == []
+    # wall face k is owned by cell k on the O-grid (no-slip gradient ghost)
+    wall = np.nonzero(o.bc_marker == 0)[0]
+    assert np.array_equal(o.face_c1[wall], wall)
+    # same seed, same grid
+    assert np.array_equal(gg.ramp(32, 16).face_p1, gg.ramp(32, 16).face_p1)
+
+
+def test_grid_file_round_trip(tmp_path):
+    g = gg.box(5, 4, jitter=0.3, split_fraction=0.5)
+    p = tmp_path / "g.in"
+    g.write(str(p))
+    h = gg.read_grid(str(p))
+    for k in ("x", "y"):
+        assert np.array_equal(getattr(g, k), getattr(h, k))
+    for k in ("face_p1", "face_p2", "face_c1", "face_c2", "bc_marker"):
+        assert np.array_equal(getattr(g, k), getattr(h, k))
+
+
+def test_reference_grid_file_reads():
+    """grids/grid_file.in ships with the reference; its header and counts are
+    recorded here (7026 points, 15222 faces, 8196 cells, 219 + 40 BC faces)."""
+    path = "/root/reference/grids/grid_file.in"
+    if not os.path.exists(path):
+        pytest.skip("reference tree not present")
+    g = gg.read_grid(path)
+    assert (g.num_points, g.num_faces, g.num_cells) == (7026, 15222, 8196)
+    assert int((g.bc_marker == 0).sum()) == 219 and int((g.bc_marker == -1).sum()) == 40
+    assert gg.check_canonical(g) == []
+    vol = F.host_geometry(g, inputs.Case(BC1="NO_SLIP_WALL"), "cell_vol")
+    assert np.all(vol > 0)
