@@ -108,6 +108,25 @@ def cpu_baseline(grid, case, nc: int, target_s: float = 15.0):
             "sample": f"{what} of the same workload ({t:.2f} s of step loop)"}
 
 
+def copy_calibration(mbytes: int) -> None:
+    """One device-to-device copy of a known size through the HIP runtime
+    (a plain copy kernel), so the profiler's FETCH_SIZE / WRITE_SIZE per
+    dispatch can be scaled against bytes that are known exactly."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mbytes << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,7 +135,13 @@ def main():
     ap.add_argument("--config", default="ramp")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--variant", type=int, default=0, help="0 default, 1 cell-, 2 slot-parallel")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run device-to-device copies of MB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(instances or mesh partitions would shard across GPUs, see DESIGN.md)",
@@ -124,7 +149,7 @@ def main():
         sys.exit(2)
     name, grid, case = workload(args.config)
     nc, nf = grid.num_cells, grid.num_faces
-    s = Solver(grid, case)
+    s = Solver(grid, case, kernel_variant=args.variant)
     s.init_freestream()
     s.step(args.warmup)
     t0 = time.perf_counter()
@@ -151,7 +176,7 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic",
            "config": {"workload": name, "cells": nc, "faces": nf, "precision": "FP64 plain",
-                      "state_finite": finite},
+                      "kernel_variant": args.variant, "state_finite": finite},
            "roofline": roof}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(grid, case, nc)

@@ -128,6 +128,7 @@ bool read_inputs(const std::string& path, Inputs* in, std::string* log, std::str
         if (key == "renumber") in->renumber = std::stoi(val);
         if (key == "block_size") in->block_size = std::stoi(val);
         if (key == "xcd_remap") in->xcd_remap = std::stoi(val);
+        if (key == "kernel_variant") in->kernel_variant = std::stoi(val);
       }
     }
   }
@@ -250,6 +251,7 @@ fvk_options as_options(const Inputs& in) {
   o.renumber = in.renumber;
   o.block_size = in.block_size;
   o.xcd_remap = in.xcd_remap;
+  o.kernel_variant = in.kernel_variant;
   return o;
 }
 

@@ -43,7 +43,9 @@ struct StageArgs {
   int32_t sentinel;            // lagged face states are the -101 initial values
   int32_t fs_prims;            // stage input's primitives are the freestream ones
   int32_t store_res;           // write the residual (stage 3 of a reported step)
-  int32_t xcd_blocks;          // grid blocks per XCD for the remap (0 = off)
+  int32_t xcd_remap;           // give each XCD a contiguous range of blocks
+  int32_t xcd_blocks;          // blocks per XCD range (set by the launcher)
+  int32_t variant;             // kVariantCell or kVariantSlot
   const double* qc;            // stage input state (4 blocks)
   const double* qp;            // previous stage's input state (lagged limiter)
   const double* gp;            // previous stage's limited gradient (8 blocks)
@@ -56,7 +58,8 @@ struct StageArgs {
   double* res;                 // residual (4 blocks)
 };
 
-enum PassAKind { kPassA_Plain = 0, kPassA_Squeeze = 1 };
+// launch shapes: one thread per cell, or one thread per (cell, face slot)
+enum { kVariantCell = 1, kVariantSlot = 2 };
 
 hipError_t launch_pass_a(const StageArgs& a, int squeeze, int block, hipStream_t s);
 hipError_t launch_pass_b(const StageArgs& a, int order, int ns, int block, hipStream_t s);

@@ -33,7 +33,7 @@ struct fvk_ctx {
   unsigned long long* smax = nullptr;           // [2]
   double* partial = nullptr;
   int block = 256;
-  int xcd_blocks = 0;
+  int variant = fvk::kVariantSlot;
   int res_blocks = 0;
   int smax_cur = 0;
   bool have_state = false;
@@ -94,6 +94,7 @@ int check_options(const fvk_options& o, std::string* msg) {
   if (o.time_step_type != 1 && o.time_step_type != 2) { *msg = "time_step_type must be GLOBAL or LOCAL"; return FVK_ERR_OPTION; }
   if (o.bc1 < -4 || o.bc1 > -1 || o.bc2 < -4 || o.bc2 > -1) { *msg = "BC1/BC2 must be FREESTREAM, SLIP_WALL, EXTRAPOLATION or NO_SLIP_WALL"; return FVK_ERR_OPTION; }
   if (o.precision != FVK_FP64) { *msg = "precision must be FVK_FP64"; return FVK_ERR_OPTION; }
+  if (o.kernel_variant < 0 || o.kernel_variant > 2) { *msg = "kernel_variant must be 0, 1 or 2"; return FVK_ERR_OPTION; }
   if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64)) {
     *msg = "block_size must be 0 or a multiple of 64 in [64, 256]";
     return FVK_ERR_OPTION;
@@ -107,7 +108,8 @@ fvk::StageArgs base_args(fvk_ctx* c) {
   a.m = c->dm;
   a.fl = c->fl;
   a.dt_global = c->opt.time_step_type == 1;
-  a.xcd_blocks = c->xcd_blocks;
+  a.xcd_remap = c->opt.xcd_remap;
+  a.variant = c->variant;
   a.sigma = c->sigma;
   a.res = c->res;
   a.smax_cur = c->smax + c->smax_cur;
@@ -187,6 +189,7 @@ void fvk_default_options(fvk_options* o) {
   o->renumber = 1;
   o->block_size = 0;
   o->xcd_remap = 1;
+  o->kernel_variant = 0;
 }
 
 const char* fvk_last_error(const fvk_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
@@ -211,8 +214,7 @@ int fvk_create(const fvk_mesh* mesh, const fvk_options* opt, fvk_ctx** out) {
   if (!c->hm.warnings.empty()) fputs(c->hm.warnings.c_str(), stderr);
   const fvk::HostMesh& h = c->hm;
   c->block = opt->block_size ? opt->block_size : 256;
-  const int nblk = (h.nc + c->block - 1) / c->block;
-  c->xcd_blocks = opt->xcd_remap ? (nblk + 7) / 8 : 0;
+  c->variant = opt->kernel_variant ? opt->kernel_variant : fvk::kVariantSlot;
   c->res_blocks = (h.nc + 255) / 256;
   hipError_t e = hipStreamCreate(&c->stream);
   if (e != hipSuccess) {

@@ -51,7 +51,8 @@ class FvkOptions(ctypes.Structure):
                 ("mu", ctypes.c_double), ("Pr", ctypes.c_double), ("P", ctypes.c_double),
                 ("rho", ctypes.c_double), ("u", ctypes.c_double), ("v", ctypes.c_double),
                 ("precision", ctypes.c_int32), ("renumber", ctypes.c_int32),
-                ("block_size", ctypes.c_int32), ("xcd_remap", ctypes.c_int32)]
+                ("block_size", ctypes.c_int32), ("xcd_remap", ctypes.c_int32),
+                ("kernel_variant", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
 
 
 _lib = None

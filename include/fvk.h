@@ -74,8 +74,10 @@ typedef struct fvk_options {
   double P, rho, u, v;     /* [init] freestream */
   int32_t precision;       /* FVK_FP64 */
   int32_t renumber;        /* 1: reverse Cuthill-McKee cell order (default), 0: file order */
-  int32_t block_size;      /* 0: default (256); else 64, 128, 192 or 256 */
+  int32_t block_size;      /* cell-per-thread shape: 0 = 256; else 64, 128, 192 or 256 */
   int32_t xcd_remap;       /* 1: contiguous cell ranges per XCD (default), 0: off */
+  int32_t kernel_variant;  /* 0: default; 1: one thread per cell; 2: one thread per face slot */
+  int32_t reserved_;
 } fvk_options;
 
 /* Fill *opt with the defaults above for the given BC kinds. */

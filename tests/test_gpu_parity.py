@@ -76,11 +76,12 @@ def _bench_case(kind):
     raise ValueError(kind)
 
 
+@pytest.mark.parametrize("variant", [1, 2])
 @pytest.mark.parametrize("kind,steps", [("bump", 100), ("ramp", 4), ("ogrid", 3)])
-def test_full_size_against_oracle(kind, steps):
+def test_full_size_against_oracle(kind, steps, variant):
     grid, case = _bench_case(kind)
-    print(f"[gpu] full size {kind} cells={grid.num_cells} steps={steps}", flush=True)
-    s = Solver(grid, case)
+    print(f"[gpu] full size {kind} cells={grid.num_cells} steps={steps} variant={variant}", flush=True)
+    s = Solver(grid, case, kernel_variant=variant)
     s.init_freestream()
     orc = O.Oracle(grid, case)
     s.step(steps)
@@ -91,8 +92,9 @@ def test_full_size_against_oracle(kind, steps):
     assert same(s.residual(), orc.residual()), mismatch(s.residual(), orc.residual())
 
 
-@pytest.mark.parametrize("over", [dict(renumber=0), dict(block_size=64), dict(block_size=128),
-                                  dict(xcd_remap=0)])
+@pytest.mark.parametrize("over", [dict(renumber=0), dict(kernel_variant=1),
+                                  dict(kernel_variant=1, block_size=64),
+                                  dict(kernel_variant=1, block_size=128), dict(xcd_remap=0)])
 def test_layout_invariance(over):
     g = Golden("ogrid_ns_squeeze_global")
     a = Solver(g.grid, g.case)

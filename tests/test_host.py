@@ -38,7 +38,7 @@ def test_struct_layout_matches_header():
 #include "fvk.h"
 int main(void) {
   printf("%zu %zu %zu %zu\n", sizeof(fvk_mesh), offsetof(fvk_mesh, face_cell2),
-         sizeof(fvk_options), offsetof(fvk_options, xcd_remap));
+         sizeof(fvk_options), offsetof(fvk_options, kernel_variant));
   return 0;
 }
 """
@@ -50,7 +50,7 @@ int main(void) {
                            os.path.join(d, "abi"), os.path.join(d, "abi.c")])
     got = subprocess.run([os.path.join(d, "abi")], capture_output=True, text=True).stdout.split()
     assert [int(v) for v in got] == [ctypes.sizeof(F.FvkMesh), F.FvkMesh.face_cell2.offset,
-                                     ctypes.sizeof(F.FvkOptions), F.FvkOptions.xcd_remap.offset]
+                                     ctypes.sizeof(F.FvkOptions), F.FvkOptions.kernel_variant.offset]
 
 
 GEOM_FACE = ["face_area", "face_nx", "face_ny", "face_centerx", "face_centery"]
