@@ -127,6 +127,53 @@ def copy_calibration(mbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
 
 
+def _pmc_mean_kib(counter: str, child_args, match: str, timeout: int = 180):
+    """Mean value (KiB) of `counter` per dispatch of the kernels whose name
+    contains `match`, from one `rocprofv3 --pmc` run of this script."""
+    import csv
+    import shutil
+    if not shutil.which("rocprofv3"):
+        return None
+    d = tempfile.mkdtemp(dir="/tmp")
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__)] + child_args
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout,
+                       check=True)
+    except (subprocess.SubprocessError, OSError):
+        return None
+    vals = []
+    for root, _, files in os.walk(d):
+        for fn in files:
+            if fn.endswith("counter_collection.csv"):
+                for r in csv.DictReader(open(os.path.join(root, fn))):
+                    if match in r["Kernel_Name"] and r["Counter_Name"] == counter:
+                        vals.append(float(r["Counter_Value"]))
+    return sum(vals) / len(vals) if vals else None
+
+
+def measure_traffic(kernel: str, child_args):
+    """HBM-side bytes per launch of `kernel` (rocprofv3 FETCH_SIZE and
+    WRITE_SIZE), each scaled by what the same counter reads for a
+    device-to-device copy of known size."""
+    copy_mb = 256
+    match = "k_" + kernel + "_"
+    fetch = _pmc_mean_kib("FETCH_SIZE", child_args, match)
+    write = _pmc_mean_kib("WRITE_SIZE", child_args, match)
+    cf = _pmc_mean_kib("FETCH_SIZE", ["--copy-calibration", str(copy_mb)], "copyBuffer")
+    cw = _pmc_mean_kib("WRITE_SIZE", ["--copy-calibration", str(copy_mb)], "copyBuffer")
+    if None in (fetch, write, cf, cw) or cf <= 0 or cw <= 0:
+        return None, None
+    sf = (copy_mb << 20) / (cf * 1024.0)
+    sw = (copy_mb << 20) / (cw * 1024.0)
+    traffic = fetch * 1024.0 * sf + write * 1024.0 * sw
+    detail = {"fetch_kib": round(fetch, 1), "write_kib": round(write, 1),
+              "fetch_scale": round(sf, 4), "write_scale": round(sw, 4),
+              "calibration": f"{copy_mb} MiB device-to-device copy"}
+    return int(traffic), detail
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,8 +181,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", default="ramp")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=200)
-    ap.add_argument("--variant", type=int, default=0, help="0 default, 1 cell-, 2 slot-parallel")
+    ap.add_argument("--kernel-reps", type=int, default=200,
+                    help="back-to-back launches per kernel for the per-launch timing")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
+    ap.add_argument("--variant", type=int, default=0,
+                    help="0 default, 1 cell-parallel, 2 slot-parallel, 3 LDS-staged blocks")
+    ap.add_argument("--block-size", type=int, default=0,
+                    help="cells per block (LDS-staged) or threads per block (cell-parallel)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run device-to-device copies of MB MiB (counter calibration)")
     args = ap.parse_args()
@@ -149,7 +202,7 @@ def main():
         sys.exit(2)
     name, grid, case = workload(args.config)
     nc, nf = grid.num_cells, grid.num_faces
-    s = Solver(grid, case, kernel_variant=args.variant)
+    s = Solver(grid, case, kernel_variant=args.variant, block_size=args.block_size)
     s.init_freestream()
     s.step(args.warmup)
     t0 = time.perf_counter()
@@ -157,26 +210,33 @@ def main():
     wall = time.perf_counter() - t0
     value = nc * 3 * args.steps / wall
 
-    # per-kernel HIP-event timing on the library's stream
-    ms, cnt = s.time_steps(args.profile_steps)
-    kinds = {1: "pass_a", 2: "pass_b"}
-    per_launch = {kinds[k]: ms[k] / cnt[k] for k in (1, 2) if cnt[k]}
-    totals = {kinds[k]: ms[k] for k in (1, 2) if cnt[k]}
+    # per-launch kernel time: back-to-back launches between HIP events on the
+    # library's stream (pass A and pass B run once per stage each)
+    per_launch = {k: v for k, v in s.time_kernels(args.kernel_reps).items() if v > 0}
     algo = algorithmic_bytes(case, nc, nf)
-    dom = max(totals, key=totals.get)
-    achieved = algo[dom] / (per_launch[dom] * 1e-3) / 1e9
+    dom = max(per_launch, key=per_launch.get)
+    achieved = algo[dom] / (per_launch[dom] * 1e-6) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
             "algorithmic_bytes_per_launch": int(algo[dom]),
-            "avg_launch_us": round(per_launch[dom] * 1e3, 2),
-            "all_kernels_avg_us": {k: round(v * 1e3, 2) for k, v in per_launch.items()}}
+            "avg_launch_us": round(per_launch[dom], 2),
+            "all_kernels_avg_us": {k: round(v, 2) for k, v in per_launch.items()}}
+    if not args.no_traffic:
+        child = ["--no-cpu", "--no-traffic", "--steps", "5", "--warmup", "2", "--kernel-reps", "5",
+                 "--config", args.config, "--variant", str(args.variant),
+                 "--block-size", str(args.block_size)]
+        traffic, detail = measure_traffic(dom, child)
+        roof["traffic"] = traffic
+        if detail:
+            roof["traffic_detail"] = detail
     finite = bool(np.isfinite(s.state()).all())
     out = {"metric": METRIC, "value": value, "unit": "cell-stage/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic",
            "config": {"workload": name, "cells": nc, "faces": nf, "precision": "FP64 plain",
-                      "kernel_variant": args.variant, "state_finite": finite},
+                      "kernel_variant": args.variant, "block_size": args.block_size,
+                      "state_finite": finite},
            "roofline": roof}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(grid, case, nc)

@@ -8,10 +8,16 @@ namespace fvk {
 // Read-only mesh data on the device (new numbering; see mesh_prep.hpp).
 struct DevMesh {
   int32_t nc;                  // real cells
-  int32_t ncp;                 // padded cells (multiple of 64)
+  int32_t ncp;                 // padded cells = nblocks * blk
+  int32_t blk;                 // cells per block
+  int32_t nblocks;
+  int32_t max_halo;            // LDS row length is blk + max_halo
+  int32_t max_bfaces;          // LDS face-result row length
   const int32_t* chunk_off;    // slot offset of each 64-cell chunk
-  const uint8_t* deg;          // faces per cell
-  const int32_t* code;         // per slot: neighbour/owner or boundary kind
+  const uint8_t* deg;          // faces per cell (0 = padding)
+  const int32_t* code;         // per slot: global neighbour/owner or boundary code
+  const int32_t* lcode;        // per slot: block-local neighbour/owner or boundary code
+  const int32_t* sface;        // per slot: (block face index << 1) | owner
   const double* s_nx;          // per slot: face unit normal (out of the owner)
   const double* s_ny;
   const double* s_area;        // per slot: face length
@@ -23,6 +29,16 @@ struct DevMesh {
   const double* ixx;           // per cell: LSQ inverse (readmesh.cpp:316-320)
   const double* iyy;
   const double* ixy;
+  const int32_t* halo_ptr;     // per block: halo cells
+  const int32_t* halo_idx;
+  const int32_t* bf_ptr;       // per block: face list
+  const int32_t* bf_a;         // local owner index
+  const int32_t* bf_b;         // local neighbour index or boundary code
+  const double* bf_nx;
+  const double* bf_ny;
+  const double* bf_area;
+  const double* bf_fcx;
+  const double* bf_fcy;
 };
 
 // Gas and freestream constants (class_flow, class_Q members).
@@ -45,7 +61,7 @@ struct StageArgs {
   int32_t store_res;           // write the residual (stage 3 of a reported step)
   int32_t xcd_remap;           // give each XCD a contiguous range of blocks
   int32_t xcd_blocks;          // blocks per XCD range (set by the launcher)
-  int32_t variant;             // kVariantCell or kVariantSlot
+  int32_t variant;             // kVariantCell, kVariantSlot or kVariantLds
   const double* qc;            // stage input state (4 blocks)
   const double* qp;            // previous stage's input state (lagged limiter)
   const double* gp;            // previous stage's limited gradient (8 blocks)
@@ -58,15 +74,20 @@ struct StageArgs {
   double* res;                 // residual (4 blocks)
 };
 
-// launch shapes: one thread per cell, or one thread per (cell, face slot)
-enum { kVariantCell = 1, kVariantSlot = 2 };
+// launch shapes: one thread per cell; one thread per (cell, face slot); or
+// LDS-staged blocks (block + halo in LDS, one thread per face for the flux)
+enum { kVariantCell = 1, kVariantSlot = 2, kVariantLds = 3 };
+
+// LDS bytes the LDS-staged kernels need for this mesh
+size_t lds_bytes_pass_a(const DevMesh& m);
+size_t lds_bytes_pass_b(const DevMesh& m, int order, int stage);
 
 hipError_t launch_pass_a(const StageArgs& a, int squeeze, int block, hipStream_t s);
 hipError_t launch_pass_b(const StageArgs& a, int order, int ns, int block, hipStream_t s);
 hipError_t launch_init_sigma(const StageArgs& a, int block, hipStream_t s);
-// per-block partial sums of R_k^2: out[k*nblocks + b]
-hipError_t launch_res_sq(const double* res, int32_t nc, int32_t ncp, double* partial,
-                         int nblocks, int block, hipStream_t s);
+// per-block partial sums of R_k^2 over all ncp cells: out[k*nblocks + b]
+hipError_t launch_res_sq(const double* res, int32_t ncp, double* partial, int nblocks, int block,
+                         hipStream_t s);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 
 }  // namespace fvk

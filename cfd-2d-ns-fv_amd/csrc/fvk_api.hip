@@ -2,6 +2,7 @@
 // driver (ssp_rk3.cpp:18-77 as a sequence of kernel launches on one stream).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -33,7 +34,7 @@ struct fvk_ctx {
   unsigned long long* smax = nullptr;           // [2]
   double* partial = nullptr;
   int block = 256;
-  int variant = fvk::kVariantSlot;
+  int variant = fvk::kVariantLds;
   int res_blocks = 0;
   int smax_cur = 0;
   bool have_state = false;
@@ -94,7 +95,8 @@ int check_options(const fvk_options& o, std::string* msg) {
   if (o.time_step_type != 1 && o.time_step_type != 2) { *msg = "time_step_type must be GLOBAL or LOCAL"; return FVK_ERR_OPTION; }
   if (o.bc1 < -4 || o.bc1 > -1 || o.bc2 < -4 || o.bc2 > -1) { *msg = "BC1/BC2 must be FREESTREAM, SLIP_WALL, EXTRAPOLATION or NO_SLIP_WALL"; return FVK_ERR_OPTION; }
   if (o.precision != FVK_FP64) { *msg = "precision must be FVK_FP64"; return FVK_ERR_OPTION; }
-  if (o.kernel_variant < 0 || o.kernel_variant > 2) { *msg = "kernel_variant must be 0, 1 or 2"; return FVK_ERR_OPTION; }
+  if (o.kernel_variant < 0 || o.kernel_variant > 3) { *msg = "kernel_variant must be 0, 1, 2 or 3"; return FVK_ERR_OPTION; }
+  if (o.renumber < 0 || o.renumber > 2) { *msg = "renumber must be 0, 1 or 2"; return FVK_ERR_OPTION; }
   if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64)) {
     *msg = "block_size must be 0 or a multiple of 64 in [64, 256]";
     return FVK_ERR_OPTION;
@@ -214,8 +216,8 @@ int fvk_create(const fvk_mesh* mesh, const fvk_options* opt, fvk_ctx** out) {
   if (!c->hm.warnings.empty()) fputs(c->hm.warnings.c_str(), stderr);
   const fvk::HostMesh& h = c->hm;
   c->block = opt->block_size ? opt->block_size : 256;
-  c->variant = opt->kernel_variant ? opt->kernel_variant : fvk::kVariantSlot;
-  c->res_blocks = (h.nc + 255) / 256;
+  c->variant = opt->kernel_variant ? opt->kernel_variant : fvk::kVariantLds;
+  c->res_blocks = (h.ncp + 255) / 256;
   hipError_t e = hipStreamCreate(&c->stream);
   if (e != hipSuccess) {
     hip_err(nullptr, e, "hipStreamCreate");
@@ -230,9 +232,25 @@ int fvk_create(const fvk_mesh* mesh, const fvk_options* opt, fvk_ctx** out) {
   fvk::DevMesh& d = c->dm;
   d.nc = h.nc;
   d.ncp = h.ncp;
+  d.blk = h.blk;
+  d.nblocks = h.nblocks;
+  d.max_halo = h.max_halo;
+  d.max_bfaces = h.max_bfaces;
   if ((rc = upload(c, h.chunk_off, &d.chunk_off))) return fail(rc);
   if ((rc = upload(c, h.deg, &d.deg))) return fail(rc);
   if ((rc = upload(c, h.code, &d.code))) return fail(rc);
+  if ((rc = upload(c, h.lcode, &d.lcode))) return fail(rc);
+  if ((rc = upload(c, h.sface, &d.sface))) return fail(rc);
+  if ((rc = upload(c, h.halo_ptr, &d.halo_ptr))) return fail(rc);
+  if ((rc = upload(c, h.halo_idx, &d.halo_idx))) return fail(rc);
+  if ((rc = upload(c, h.bf_ptr, &d.bf_ptr))) return fail(rc);
+  if ((rc = upload(c, h.bf_a, &d.bf_a))) return fail(rc);
+  if ((rc = upload(c, h.bf_b, &d.bf_b))) return fail(rc);
+  if ((rc = upload(c, h.bf_nx, &d.bf_nx))) return fail(rc);
+  if ((rc = upload(c, h.bf_ny, &d.bf_ny))) return fail(rc);
+  if ((rc = upload(c, h.bf_area, &d.bf_area))) return fail(rc);
+  if ((rc = upload(c, h.bf_fcx, &d.bf_fcx))) return fail(rc);
+  if ((rc = upload(c, h.bf_fcy, &d.bf_fcy))) return fail(rc);
   if ((rc = upload(c, h.s_nx, &d.s_nx))) return fail(rc);
   if ((rc = upload(c, h.s_ny, &d.s_ny))) return fail(rc);
   if ((rc = upload(c, h.s_area, &d.s_area))) return fail(rc);
@@ -244,6 +262,24 @@ int fvk_create(const fvk_mesh* mesh, const fvk_options* opt, fvk_ctx** out) {
   if ((rc = upload(c, h.c_ixx, &d.ixx))) return fail(rc);
   if ((rc = upload(c, h.c_iyy, &d.iyy))) return fail(rc);
   if ((rc = upload(c, h.c_ixy, &d.ixy))) return fail(rc);
+  if (c->variant == fvk::kVariantLds) {
+    // the LDS-staged kernels need the block plus its largest halo in LDS
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if ((e = hipGetDevice(&dev)) != hipSuccess || (e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) {
+      hip_err(c, e, "hipGetDeviceProperties");
+      return fail(FVK_ERR_HIP);
+    }
+    const size_t need = std::max(fvk::lds_bytes_pass_a(d), fvk::lds_bytes_pass_b(d, opt->order, 3));
+    if (need > prop.sharedMemPerBlock) {
+      if (opt->kernel_variant == fvk::kVariantLds) {
+        set_err(c, FVK_ERR_MESH, "mesh blocks need " + std::to_string(need) +
+                                     " bytes of LDS, more than the device allows");
+        return fail(FVK_ERR_MESH);
+      }
+      c->variant = fvk::kVariantCell;
+    }
+  }
   const size_t ncp = h.ncp;
   for (int i = 0; i < 3; ++i) {
     if ((rc = dev_alloc(c, &c->q[i], 4 * ncp))) return fail(rc);
@@ -311,7 +347,8 @@ int fvk_set_state(fvk_ctx* c, const double* q) {
   const fvk::HostMesh& h = c->hm;
   std::vector<double> buf(4 * (size_t)h.ncp, 1.0);
   for (int k = 0; k < 4; ++k)
-    for (int32_t i = 0; i < h.nc; ++i) buf[k * (size_t)h.ncp + i] = q[k * (size_t)h.nc + h.perm[i]];
+    for (int32_t i = 0; i < h.ncp; ++i)
+      if (h.perm[i] >= 0) buf[k * (size_t)h.ncp + i] = q[k * (size_t)h.nc + h.perm[i]];
   FVK_HIP(c, hipMemcpyAsync(c->q[0], buf.data(), buf.size() * 8, hipMemcpyHostToDevice, c->stream));
   int rc = arm_start(c);
   c->fs_prims = false;
@@ -343,13 +380,15 @@ int fvk_get_state(fvk_ctx* c, double* q, double* grad) {
     std::vector<double> buf(4 * ncp);
     FVK_HIP(c, hipMemcpy(buf.data(), c->q[0], buf.size() * 8, hipMemcpyDeviceToHost));
     for (int k = 0; k < 4; ++k)
-      for (int32_t i = 0; i < h.nc; ++i) q[k * (size_t)h.nc + h.perm[i]] = buf[k * ncp + i];
+      for (int32_t i = 0; i < h.ncp; ++i)
+        if (h.perm[i] >= 0) q[k * (size_t)h.nc + h.perm[i]] = buf[k * ncp + i];
   }
   if (grad) {
     std::vector<double> buf(8 * ncp);
     FVK_HIP(c, hipMemcpy(buf.data(), c->g[0], buf.size() * 8, hipMemcpyDeviceToHost));
     for (int k = 0; k < 8; ++k)
-      for (int32_t i = 0; i < h.nc; ++i) grad[k * (size_t)h.nc + h.perm[i]] = buf[k * ncp + i];
+      for (int32_t i = 0; i < h.ncp; ++i)
+        if (h.perm[i] >= 0) grad[k * (size_t)h.nc + h.perm[i]] = buf[k * ncp + i];
   }
   return FVK_OK;
 }
@@ -361,14 +400,15 @@ int fvk_get_residual(fvk_ctx* c, double* r) {
   std::vector<double> buf(4 * (size_t)h.ncp);
   FVK_HIP(c, hipMemcpy(buf.data(), c->res, buf.size() * 8, hipMemcpyDeviceToHost));
   for (int k = 0; k < 4; ++k)
-    for (int32_t i = 0; i < h.nc; ++i) r[k * (size_t)h.nc + h.perm[i]] = buf[k * (size_t)h.ncp + i];
+    for (int32_t i = 0; i < h.ncp; ++i)
+      if (h.perm[i] >= 0) r[k * (size_t)h.nc + h.perm[i]] = buf[k * (size_t)h.ncp + i];
   return FVK_OK;
 }
 
 int fvk_residual_l2(fvk_ctx* c, double out[4]) {
   if (!c || !out) return FVK_ERR_ARG;
   const fvk::HostMesh& h = c->hm;
-  FVK_HIP(c, fvk::launch_res_sq(c->res, h.nc, h.ncp, c->partial, c->res_blocks, 256, c->stream));
+  FVK_HIP(c, fvk::launch_res_sq(c->res, h.ncp, c->partial, c->res_blocks, 256, c->stream));
   std::vector<double> part(4 * (size_t)c->res_blocks);
   FVK_HIP(c, hipMemcpyAsync(part.data(), c->partial, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
   FVK_HIP(c, hipStreamSynchronize(c->stream));
@@ -470,6 +510,36 @@ int fvk_time_steps(fvk_ctx* c, int32_t nsteps, double out_ms[4], int64_t counts[
     }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
+int fvk_time_kernels(fvk_ctx* c, int32_t reps, double avg_us[2]) {
+  if (!c || !avg_us || reps <= 0) return FVK_ERR_ARG;
+  if (!c->have_state) return set_err(c, FVK_ERR_ARG, "fvk_time_kernels: no state");
+  const fvk_options& o = c->opt;
+  fvk::StageArgs a = base_args(c);
+  a.stage = 2;
+  a.qn = c->q[0]; a.qc = c->q[1]; a.qp = c->q[0]; a.gp = c->g[0]; a.g = c->g[1]; a.qout = c->q[2];
+  hipEvent_t e0, e1;
+  FVK_HIP(c, hipEventCreate(&e0));
+  FVK_HIP(c, hipEventCreate(&e1));
+  avg_us[0] = avg_us[1] = 0.0;
+  int rc = FVK_OK;
+  for (int kind = 0; kind < 2 && !rc; ++kind) {
+    if (kind == 0 && o.order != 2) continue;
+    hipError_t e = hipEventRecord(e0, c->stream);
+    for (int32_t r = 0; r < reps && e == hipSuccess; ++r)
+      e = kind == 0 ? fvk::launch_pass_a(a, o.limiter == 2, c->block, c->stream)
+                    : fvk::launch_pass_b(a, o.order, o.eqn == 2, c->block, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) rc = hip_err(c, e, "fvk_time_kernels");
+    else avg_us[kind] = ms * 1e3 / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return rc;
 }
 

@@ -193,6 +193,35 @@ std::vector<int32_t> rcm_order(const HostMesh& h) {
   return order;
 }
 
+// Recursive coordinate bisection: split ids[0..n) into k parts of at most
+// cap cells, cutting each range across the longer extent of its centroids.
+// Appends the parts' cells, part after part, to out and the part sizes to
+// *sizes.
+void rcb(int32_t* ids, int64_t n, int64_t k, int64_t cap, const double* cx, const double* cy,
+         std::vector<int32_t>& out, std::vector<int32_t>* sizes) {
+  if (k <= 1 || n <= 1) {
+    out.insert(out.end(), ids, ids + n);
+    if (sizes) sizes->push_back((int32_t)n);
+    for (int64_t i = 1; sizes && i < k; ++i) sizes->push_back(0);
+    return;
+  }
+  const int64_t kl = k / 2, kr = k - kl;
+  int64_t nl = (n * kl + k / 2) / k;
+  nl = std::max(nl, n - cap * kr);
+  nl = std::min(nl, cap * kl);
+  double x0 = cx[ids[0]], x1 = x0, y0 = cy[ids[0]], y1 = y0;
+  for (int64_t i = 1; i < n; ++i) {
+    x0 = std::min(x0, cx[ids[i]]); x1 = std::max(x1, cx[ids[i]]);
+    y0 = std::min(y0, cy[ids[i]]); y1 = std::max(y1, cy[ids[i]]);
+  }
+  const double* key = (x1 - x0) >= (y1 - y0) ? cx : cy;
+  std::nth_element(ids, ids + nl, ids + n, [key](int32_t a, int32_t b) {
+    return key[a] < key[b] || (key[a] == key[b] && a < b);
+  });
+  rcb(ids, nl, kl, cap, cx, cy, out, sizes);
+  rcb(ids + nl, n - nl, kr, cap, cx, cy, out, sizes);
+}
+
 }  // namespace
 
 int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::string* err) {
@@ -298,19 +327,110 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
       break;
     }
 
-  // renumbering
-  if (o.renumber) h.perm = rcm_order(h);
-  else { h.perm.resize(h.nc); std::iota(h.perm.begin(), h.perm.end(), 0); }
+  // renumbering: blocks of blk cells, numbered block-major
+  h.blk = o.block_size > 0 ? o.block_size : 256;
+  h.nblocks = (h.nc + h.blk - 1) / h.blk;
+  h.ncp = h.nblocks * h.blk;
+  std::vector<int32_t> order, sizes;
+  if (o.renumber == 1) {
+    std::vector<int32_t> ids(h.nc);
+    std::iota(ids.begin(), ids.end(), 0);
+    rcb(ids.data(), h.nc, h.nblocks, h.blk, h.ccx.data(), h.ccy.data(), order, &sizes);
+    // order each block's cells by a further bisection into 64-cell groups
+    std::vector<int32_t> sub;
+    size_t pos = 0;
+    for (int32_t b = 0; b < h.nblocks; ++b) {
+      sub.clear();
+      const int32_t n = sizes[b];
+      rcb(order.data() + pos, n, (n + kChunk - 1) / kChunk, kChunk, h.ccx.data(), h.ccy.data(),
+          sub, nullptr);
+      std::copy(sub.begin(), sub.end(), order.begin() + pos);
+      pos += n;
+    }
+  } else {
+    if (o.renumber == 2) order = rcm_order(h);
+    else { order.resize(h.nc); std::iota(order.begin(), order.end(), 0); }
+    for (int32_t b = 0; b < h.nblocks; ++b) sizes.push_back(std::min(h.blk, h.nc - b * h.blk));
+  }
+  h.perm.assign(h.ncp, -1);
+  {
+    size_t pos = 0;
+    for (int32_t b = 0; b < h.nblocks; ++b)
+      for (int32_t i = 0; i < sizes[b]; ++i) h.perm[(size_t)b * h.blk + i] = order[pos++];
+  }
   h.iperm.assign(h.nc, 0);
-  for (int32_t i = 0; i < h.nc; ++i) h.iperm[h.perm[i]] = i;
+  for (int32_t i = 0; i < h.ncp; ++i)
+    if (h.perm[i] >= 0) h.iperm[h.perm[i]] = i;
+
+  // halos and local indices
+  h.halo_ptr.assign(h.nblocks + 1, 0);
+  std::vector<std::vector<int32_t>> halos(h.nblocks);
+  for (int32_t b = 0; b < h.nblocks; ++b) {
+    std::vector<int32_t>& hl = halos[b];
+    for (int32_t i = 0; i < h.blk; ++i) {
+      const int32_t c = h.perm[(size_t)b * h.blk + i];
+      if (c < 0) continue;
+      for (int32_t j = h.cf_ptr[c]; j < h.cf_ptr[c + 1]; ++j) {
+        const int32_t f = h.cf_idx[j];
+        if (h.fc2[f] < 0) continue;
+        const int32_t other = h.iperm[(h.fc1[f] == c) ? h.fc2[f] : h.fc1[f]];
+        if (other / h.blk != b) hl.push_back(other);
+      }
+    }
+    std::sort(hl.begin(), hl.end());
+    hl.erase(std::unique(hl.begin(), hl.end()), hl.end());
+    h.halo_ptr[b + 1] = h.halo_ptr[b] + (int32_t)hl.size();
+    h.max_halo = std::max<int32_t>(h.max_halo, (int32_t)hl.size());
+  }
+  h.halo_idx.reserve(h.halo_ptr[h.nblocks]);
+  for (auto& hl : halos) h.halo_idx.insert(h.halo_idx.end(), hl.begin(), hl.end());
+  auto local_of = [&](int32_t b, int32_t g) -> int32_t {
+    if (g / h.blk == b) return g % h.blk;
+    const std::vector<int32_t>& hl = halos[b];
+    return h.blk + (int32_t)(std::lower_bound(hl.begin(), hl.end(), g) - hl.begin());
+  };
+
+  // block face lists, in increasing original face index
+  h.bf_ptr.assign(h.nblocks + 1, 0);
+  std::vector<std::vector<int32_t>> bfaces(h.nblocks);
+  for (int32_t b = 0; b < h.nblocks; ++b) {
+    std::vector<int32_t>& fl = bfaces[b];
+    for (int32_t i = 0; i < h.blk; ++i) {
+      const int32_t c = h.perm[(size_t)b * h.blk + i];
+      if (c < 0) continue;
+      for (int32_t j = h.cf_ptr[c]; j < h.cf_ptr[c + 1]; ++j) fl.push_back(h.cf_idx[j]);
+    }
+    std::sort(fl.begin(), fl.end());
+    fl.erase(std::unique(fl.begin(), fl.end()), fl.end());
+    h.bf_ptr[b + 1] = h.bf_ptr[b] + (int32_t)fl.size();
+    h.max_bfaces = std::max<int32_t>(h.max_bfaces, (int32_t)fl.size());
+  }
+  const size_t nbf = h.bf_ptr[h.nblocks];
+  h.bf_a.resize(nbf); h.bf_b.resize(nbf);
+  h.bf_nx.resize(nbf); h.bf_ny.resize(nbf); h.bf_area.resize(nbf);
+  h.bf_fcx.resize(nbf); h.bf_fcy.resize(nbf);
+  auto boundary_code = [&](int32_t f) {
+    const int32_t c2 = h.fc2[f];
+    const int32_t src = (c2 == FVK_BC_NO_SLIP_WALL && f < h.nc) ? h.iperm[f] : 0;
+    return slot_code_boundary(src, -c2 - 1);
+  };
+  for (int32_t b = 0; b < h.nblocks; ++b) {
+    size_t e = h.bf_ptr[b];
+    for (int32_t f : bfaces[b]) {
+      h.bf_a[e] = local_of(b, h.iperm[h.fc1[f]]);
+      h.bf_b[e] = (h.fc2[f] >= 0) ? local_of(b, h.iperm[h.fc2[f]]) : boundary_code(f);
+      h.bf_nx[e] = h.nx[f]; h.bf_ny[e] = h.ny[f]; h.bf_area[e] = h.area[f];
+      h.bf_fcx[e] = h.fcx[f]; h.bf_fcy[e] = h.fcy[f];
+      ++e;
+    }
+  }
 
   // sliced-ELL slot table
-  h.ncp = (h.nc + kChunk - 1) / kChunk * kChunk;
   const int32_t nchunks = h.ncp / kChunk;
   h.deg.assign(h.ncp, 0);
-  for (int32_t i = 0; i < h.nc; ++i) {
+  for (int32_t i = 0; i < h.ncp; ++i) {
     const int32_t c = h.perm[i];
-    h.deg[i] = (uint8_t)(h.cf_ptr[c + 1] - h.cf_ptr[c]);
+    if (c >= 0) h.deg[i] = (uint8_t)(h.cf_ptr[c + 1] - h.cf_ptr[c]);
   }
   h.chunk_off.assign(nchunks + 1, 0);
   for (int32_t k = 0; k < nchunks; ++k) {
@@ -319,31 +439,39 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
     h.chunk_off[k + 1] = h.chunk_off[k] + mx * kChunk;
   }
   const int64_t nslots = h.chunk_off[nchunks];
-  h.code.assign(nslots, 0);
+  h.code.assign(nslots, 0); h.lcode.assign(nslots, 0); h.sface.assign(nslots, 0);
   h.s_nx.assign(nslots, 0.0); h.s_ny.assign(nslots, 0.0); h.s_area.assign(nslots, 0.0);
   h.s_fcx.assign(nslots, 0.0); h.s_fcy.assign(nslots, 0.0);
-  for (int32_t i = 0; i < h.nc; ++i) {
+  for (int32_t i = 0; i < h.ncp; ++i) {
     const int32_t c = h.perm[i];
+    if (c < 0) continue;
+    const int32_t b = i / h.blk;
     const int32_t base = h.chunk_off[i / kChunk] + (i % kChunk);
+    const std::vector<int32_t>& fl = bfaces[b];
     for (int32_t j = h.cf_ptr[c]; j < h.cf_ptr[c + 1]; ++j) {
       const int32_t f = h.cf_idx[j];
       const int64_t s = base + (int64_t)(j - h.cf_ptr[c]) * kChunk;
       const int32_t c2 = h.fc2[f];
+      bool owner = true;
       if (c2 >= 0) {
-        const bool owner = (h.fc1[f] == c);
-        h.code[s] = slot_code_interior(h.iperm[owner ? c2 : h.fc1[f]], owner);
+        owner = (h.fc1[f] == c);
+        const int32_t other = h.iperm[owner ? c2 : h.fc1[f]];
+        h.code[s] = slot_code_interior(other, owner);
+        h.lcode[s] = slot_code_interior(local_of(b, other), owner);
       } else {
-        const int kind = -c2 - 1;
-        const int32_t src = (c2 == FVK_BC_NO_SLIP_WALL && f < h.nc) ? h.iperm[f] : 0;
-        h.code[s] = slot_code_boundary(src, kind);
+        h.code[s] = boundary_code(f);
+        h.lcode[s] = h.code[s];
       }
+      const int32_t fi = (int32_t)(std::lower_bound(fl.begin(), fl.end(), f) - fl.begin());
+      h.sface[s] = (fi << 1) | (owner ? 1 : 0);
       h.s_nx[s] = h.nx[f]; h.s_ny[s] = h.ny[f]; h.s_area[s] = h.area[f];
       h.s_fcx[s] = h.fcx[f]; h.s_fcy[s] = h.fcy[f];
     }
   }
   auto permute = [&](const std::vector<double>& src, std::vector<double>& dst, double pad) {
     dst.assign(h.ncp, pad);
-    for (int32_t i = 0; i < h.nc; ++i) dst[i] = src[h.perm[i]];
+    for (int32_t i = 0; i < h.ncp; ++i)
+      if (h.perm[i] >= 0) dst[i] = src[h.perm[i]];
   };
   permute(h.ccx, h.c_ccx, 0.0);
   permute(h.ccy, h.c_ccy, 0.0);

@@ -3,13 +3,24 @@
 // 1. Geometry of the reference reader (src/base/readmesh.cpp:113-320), computed
 //    in the original numbering with the reference's expression order, so every
 //    value is bit-identical to what readmesh stores in class_mesh.
-// 2. Device layout: cells renumbered by reverse Cuthill-McKee so neighbour
-//    gathers stay within a narrow index band, and a sliced-ELL "face slot"
-//    table: chunk of 64 consecutive cells (one wavefront), slot j of lane l at
-//    chunk_off[chunk] + 64*j + l.  A cell's slots list its faces in increasing
-//    ORIGINAL face index, which is the order the reference's face loops add
-//    into each cell (res.cpp:74-102, cell_gradient.cpp:93-118,
-//    ssp_rk3.cpp:28-36), so per-cell gathers reproduce its sums bit for bit.
+// 2. Device layout.
+//    * Cells are grouped into blocks of `blk` cells (a multiple of 64) and
+//      numbered block-major, a block's cells first and its padding after.
+//      Blocks come from recursive coordinate bisection of the centroids (so a
+//      block is a compact patch whose neighbours are mostly inside it), or
+//      from consecutive ranges of a reverse Cuthill-McKee or file order.
+//    * Every block lists its halo: the cells outside it that share a face
+//      with one of its cells.  Inside a block, cell i has local index i and
+//      halo cell h has local index blk + h; kernels stage the block and its
+//      halo in LDS and address neighbours by local index.
+//    * Every block lists its faces (each face touching one of its cells,
+//      once), with the owner and neighbour as local indices.
+//    * A sliced-ELL "face slot" table: chunk of 64 consecutive cells (one
+//      wavefront), slot j of lane l at chunk_off[chunk] + 64*j + l.  A cell's
+//      slots list its faces in increasing ORIGINAL face index, which is the
+//      order the reference's face loops add into each cell (res.cpp:74-102,
+//      cell_gradient.cpp:93-118, ssp_rk3.cpp:28-36), so per-cell sums taken in
+//      slot order reproduce its sums bit for bit.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -21,16 +32,25 @@ namespace fvk {
 
 constexpr int kChunk = 64;
 
-// slot code: interior  = (neighbour_new_index << 1) | (this cell owns the face)
-//            boundary  = ~((ghost_source_cell << 2) | kind), kind 0..3 =
-//                        FREESTREAM, SLIP_WALL, EXTRAPOLATION, NO_SLIP_WALL
+// slot / face codes:
+//   interior  = (neighbour_index << 1) | (this cell owns the face)
+//               (global index in `code`, block-local index in `lcode`)
+//   boundary  = ~((ghost_source_cell << 2) | kind), kind 0..3 = FREESTREAM,
+//               SLIP_WALL, EXTRAPOLATION, NO_SLIP_WALL; the source cell (global
+//               new index) is the cell whose original index equals the face's
+//               original index, which the no-slip gradient ghost reads
+//               (cell_gradient.cpp:63-66)
 inline int32_t slot_code_interior(int32_t nb, bool owner) { return (nb << 1) | (owner ? 1 : 0); }
 inline int32_t slot_code_boundary(int32_t src, int kind) { return ~((src << 2) | kind); }
 
 struct HostMesh {
   int32_t np = 0, nf = 0, nc = 0, nb = 0;
-  int32_t ncp = 0;          // cells padded to a multiple of kChunk
+  int32_t blk = 256;        // cells per block
+  int32_t nblocks = 0;
+  int32_t ncp = 0;          // nblocks * blk
   int32_t max_deg = 0;
+  int32_t max_halo = 0;     // largest halo of a block
+  int32_t max_bfaces = 0;   // largest face list of a block
   int32_t num_bc1 = 0, num_bc2 = 0, num_wall = 0;
   // original numbering (class_mesh fields)
   std::vector<int32_t> fc1, fc2;      // fc2 after BC-kind substitution
@@ -38,14 +58,22 @@ struct HostMesh {
   std::vector<double> area, nx, ny, fcx, fcy;
   std::vector<double> vol, ccx, ccy, ixx, iyy, ixy;
   // renumbering
-  std::vector<int32_t> perm;   // new -> old
-  std::vector<int32_t> iperm;  // old -> new
-  // device layout (new numbering, padded)
+  std::vector<int32_t> perm;   // ncp: new -> old, -1 for padding
+  std::vector<int32_t> iperm;  // nc: old -> new
+  // slot table (new numbering, padded)
   std::vector<int32_t> chunk_off;     // nchunks + 1, in slots
-  std::vector<uint8_t> deg;           // ncp
-  std::vector<int32_t> code;          // per slot
+  std::vector<uint8_t> deg;           // ncp (0 for padding)
+  std::vector<int32_t> code;          // per slot, global neighbour index
+  std::vector<int32_t> lcode;         // per slot, block-local neighbour index
+  std::vector<int32_t> sface;         // per slot, (index in the block's face list << 1) | owner
   std::vector<double> s_nx, s_ny, s_area, s_fcx, s_fcy;   // per slot
-  std::vector<double> c_ccx, c_ccy, c_vol, c_ixx, c_iyy, c_ixy;  // ncp
+  // per cell (new numbering, padded)
+  std::vector<double> c_ccx, c_ccy, c_vol, c_ixx, c_iyy, c_ixy;
+  // per block
+  std::vector<int32_t> halo_ptr, halo_idx;            // halo cells (global new index)
+  std::vector<int32_t> bf_ptr;                        // block face lists
+  std::vector<int32_t> bf_a, bf_b;                    // local owner; local neighbour or boundary code
+  std::vector<double> bf_nx, bf_ny, bf_area, bf_fcx, bf_fcy;
   std::string warnings;
 };
 

@@ -24,7 +24,7 @@ BC_CODES = {"FREESTREAM": -1, "SLIP_WALL": -2, "EXTRAPOLATION": -3, "NO_SLIP_WAL
 EXPORTS = ["fvk_default_options", "fvk_create", "fvk_destroy", "fvk_last_error",
            "fvk_init_freestream", "fvk_set_state", "fvk_step", "fvk_get_state",
            "fvk_get_residual", "fvk_residual_l2", "fvk_get_geometry", "fvk_geometry",
-           "fvk_get_info", "fvk_time_steps"]
+           "fvk_get_info", "fvk_time_steps", "fvk_time_kernels"]
 
 
 class FvkError(RuntimeError):
@@ -84,6 +84,7 @@ def lib():
         L.fvk_geometry.argtypes = [P(FvkMesh), P(FvkOptions), ctypes.c_char_p, dp]
         L.fvk_get_info.argtypes = [vp, P(ctypes.c_int64)]
         L.fvk_time_steps.argtypes = [vp, ctypes.c_int32, dp, P(ctypes.c_int64)]
+        L.fvk_time_kernels.argtypes = [vp, ctypes.c_int32, dp]
         _lib = L
     return _lib
 
@@ -215,6 +216,12 @@ class Solver:
         keys = ["num_cells", "num_faces", "num_boundary_faces", "num_slots", "max_faces",
                 "num_wall"]
         return dict(zip(keys, list(a)))
+
+    def time_kernels(self, reps: int):
+        """Average microseconds per launch of pass A and pass B (stage 2)."""
+        out = np.zeros(2)
+        self._check(lib().fvk_time_kernels(self.h, reps, _dp(out)))
+        return {"pass_a": float(out[0]), "pass_b": float(out[1])}
 
     def time_steps(self, n: int):
         ms = np.zeros(4)

@@ -133,6 +133,14 @@ int fvk_get_info(fvk_ctx* ctx, int64_t info[6]);
  * [2] pass B (flux+residual+update), [3] other.  counts[0..3] launches. */
 int fvk_time_steps(fvk_ctx* ctx, int32_t nsteps, double out_ms[4], int64_t counts[4]);
 
+/* Per-kernel timing: `reps` back-to-back launches of the stage-2 pass A
+ * kernel (gradient + limiter) and of the stage-2 pass B kernel (flux +
+ * residual + update), each between one HIP event pair on the context's
+ * stream.  Both read Q1/Qn/G1 and write G2/Q2 only, so repeating them leaves
+ * the solver state as it was.  avg_us[0] = pass A (0 at first order),
+ * avg_us[1] = pass B, in microseconds per launch. */
+int fvk_time_kernels(fvk_ctx* ctx, int32_t reps, double avg_us[2]);
+
 #ifdef __cplusplus
 }
 #endif

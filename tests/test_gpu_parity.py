@@ -76,7 +76,7 @@ def _bench_case(kind):
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 3])
 @pytest.mark.parametrize("kind,steps", [("bump", 100), ("ramp", 4), ("ogrid", 3)])
 def test_full_size_against_oracle(kind, steps, variant):
     grid, case = _bench_case(kind)
@@ -92,13 +92,15 @@ def test_full_size_against_oracle(kind, steps, variant):
     assert same(s.residual(), orc.residual()), mismatch(s.residual(), orc.residual())
 
 
-@pytest.mark.parametrize("over", [dict(renumber=0), dict(kernel_variant=1),
-                                  dict(kernel_variant=1, block_size=64),
-                                  dict(kernel_variant=1, block_size=128), dict(xcd_remap=0)])
+@pytest.mark.parametrize("over", [dict(renumber=0), dict(renumber=2), dict(kernel_variant=1),
+                                  dict(kernel_variant=2), dict(kernel_variant=1, block_size=64),
+                                  dict(kernel_variant=3, block_size=128),
+                                  dict(kernel_variant=3, block_size=64, renumber=2),
+                                  dict(xcd_remap=0)])
 def test_layout_invariance(over):
     g = Golden("ogrid_ns_squeeze_global")
     a = Solver(g.grid, g.case)
-    b = Solver(g.grid, g.case, **over)
+    b = Solver(g.grid, g.case, **{"kernel_variant": 1, **over})
     a.init_freestream()
     b.init_freestream()
     a.step(7)
