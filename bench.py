@@ -1,4 +1,5 @@
-"""Benchmark: cell-RK-stage updates/s of the FP64 device path on one MI355X.
+"""Benchmark: cell-RK-stage updates/s of the device path on one MI355X, in
+the FP32-mixed precision form with the FP64 path timed beside it.
 
 Workload (default): the 15-degree compression ramp, 512x256 quads with a
 seed-chosen 50% split into triangles (196,608 cells, 328,448 faces), Euler,
@@ -7,14 +8,18 @@ is one SSP-RK3 iteration (3 residual evaluations + stage updates) over the
 whole mesh with all inputs resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ramp|bump|ogrid]
+                    [--precision both|fp32|fp64]
 
 Prints one JSON line.  `value` = cells * 3 * K / wall seconds of the K timed
-steps (the library call returns after its stream has drained).  The
-`roofline` object prices the dominant kernel's algorithmic bytes per launch
-against its average launch time measured with HIP events on the library's
-stream; `cpu_baseline` is the reference solver built from its own sources
-(or, where that build is absent, the C oracle) timed on one host core over a
-bounded sample of the same workload.
+steps of the FP32-mixed path (the library call returns after its stream has
+drained); `fp64` holds the same for the FP64 path (bit-identical to the
+reference), `speedup_vs_fp64` their ratio, and `error_vs_fp64` the relative
+L2/Linf density and pressure errors of the FP32-mixed state against the FP64
+state after the same steps.  The `roofline` object prices the dominant
+kernel's algorithmic bytes per launch against its average launch time
+measured with HIP events on the library's stream; `cpu_baseline` is the
+reference solver built from its own sources (or, where that build is absent,
+the C oracle) timed on one host core over a bounded sample of the workload.
 """
 from __future__ import annotations
 
@@ -33,10 +38,11 @@ import numpy as np  # noqa: E402
 
 from fv2d import gridgen as gg  # noqa: E402
 from fv2d import inputs  # noqa: E402
-from fv2d.lib import Solver  # noqa: E402
+from fv2d.lib import FP32_MIXED, FP64, Solver  # noqa: E402
 
 METRIC = "cell-RK-stage updates/s, FP32-mixed vs FP64 at stated error; % HBM peak"
 HBM_PEAK_GBS = 8000.0
+PRECISIONS = {"fp64": FP64, "fp32": FP32_MIXED}
 
 
 def workload(name: str):
@@ -127,9 +133,10 @@ def copy_calibration(mbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
 
 
-def _pmc_mean_kib(counter: str, child_args, match: str, timeout: int = 180):
+def _pmc_mean_kib(counter: str, child_args, match, timeout: int = 180):
     """Mean value (KiB) of `counter` per dispatch of the kernels whose name
-    contains `match`, from one `rocprofv3 --pmc` run of this script."""
+    contains `match` (or satisfies it, if callable), from one
+    `rocprofv3 --pmc` run of this script."""
     import csv
     import shutil
     if not shutil.which("rocprofv3"):
@@ -148,17 +155,23 @@ def _pmc_mean_kib(counter: str, child_args, match: str, timeout: int = 180):
         for fn in files:
             if fn.endswith("counter_collection.csv"):
                 for r in csv.DictReader(open(os.path.join(root, fn))):
-                    if match in r["Kernel_Name"] and r["Counter_Name"] == counter:
+                    name = r["Kernel_Name"]
+                    hit = match(name) if callable(match) else match in name
+                    if hit and r["Counter_Name"] == counter:
                         vals.append(float(r["Counter_Value"]))
     return sum(vals) / len(vals) if vals else None
 
 
-def measure_traffic(kernel: str, child_args):
+def measure_traffic(kernel: str, child_args, tchar: str):
     """HBM-side bytes per launch of `kernel` (rocprofv3 FETCH_SIZE and
     WRITE_SIZE), each scaled by what the same counter reads for a
-    device-to-device copy of known size."""
+    device-to-device copy of known size.  `tchar` is the mangled state type
+    ("d" double, "f" float) that tells the two instantiations apart."""
     copy_mb = 256
-    match = "k_" + kernel + "_"
+    tname = {"f": "float", "d": "double"}[tchar]
+
+    def match(name):   # demangled (k_pass_b_lds<float, 2, false>) or mangled (IfLi2...)
+        return ("k_" + kernel + "_") in name and (f"<{tname}" in name or f"I{tchar}L" in name)
     fetch = _pmc_mean_kib("FETCH_SIZE", child_args, match)
     write = _pmc_mean_kib("WRITE_SIZE", child_args, match)
     cf = _pmc_mean_kib("FETCH_SIZE", ["--copy-calibration", str(copy_mb)], "copyBuffer")
@@ -174,12 +187,45 @@ def measure_traffic(kernel: str, child_args):
     return int(traffic), detail
 
 
+def run_precision(prec: str, args, grid, case):
+    """Warm up, time args.steps steps, then time each kernel kind per launch."""
+    nc = grid.num_cells
+    s = Solver(grid, case, kernel_variant=args.variant, block_size=args.block_size,
+               precision=PRECISIONS[prec])
+    s.init_freestream()
+    s.step(args.warmup)
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    wall = time.perf_counter() - t0
+    per_launch = {k: v for k, v in s.time_kernels(args.kernel_reps).items() if v > 0}
+    return {"solver": s, "wall": wall, "value": nc * 3 * args.steps / wall,
+            "per_launch": per_launch}
+
+
+def errors_vs_fp64(q32, q64, gamma):
+    """Relative L2 and Linf errors of density and pressure."""
+    def p(q):
+        return (q[3] - 0.5 * (q[1] ** 2 + q[2] ** 2) / q[0]) * (gamma - 1)
+
+    def l2(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    def linf(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    return {"rho_l2_rel": l2(q32[0], q64[0]), "rho_linf_rel": linf(q32[0], q64[0]),
+            "p_l2_rel": l2(p(q32), p(q64)), "p_linf_rel": linf(p(q32), p(q64))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--config", default="ramp")
+    ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
+                    help="both: FP32-mixed is the measured value, FP64 is timed beside it and "
+                         "is the oracle of the stated error")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=200,
                     help="back-to-back launches per kernel for the per-launch timing")
@@ -202,42 +248,52 @@ def main():
         sys.exit(2)
     name, grid, case = workload(args.config)
     nc, nf = grid.num_cells, grid.num_faces
-    s = Solver(grid, case, kernel_variant=args.variant, block_size=args.block_size)
-    s.init_freestream()
-    s.step(args.warmup)
-    t0 = time.perf_counter()
-    s.step(args.steps)
-    wall = time.perf_counter() - t0
-    value = nc * 3 * args.steps / wall
+    runs = {}
+    for prec in (["fp32", "fp64"] if args.precision == "both" else [args.precision]):
+        runs[prec] = run_precision(prec, args, grid, case)
+        print(f"bench: {prec} {runs[prec]['value']:.4g} cell-stage/s", file=sys.stderr, flush=True)
+    head = "fp32" if "fp32" in runs else "fp64"
+    r = runs[head]
+    state_bytes = 4 if head == "fp32" else 8
 
-    # per-launch kernel time: back-to-back launches between HIP events on the
-    # library's stream (pass A and pass B run once per stage each)
-    per_launch = {k: v for k, v in s.time_kernels(args.kernel_reps).items() if v > 0}
-    algo = algorithmic_bytes(case, nc, nf)
+    # roofline of the dominant kernel of the measured path: algorithmic bytes
+    # per launch over the per-launch time between HIP events on the library's
+    # stream (pass A and pass B each run once per stage)
+    per_launch = r["per_launch"]
+    algo = algorithmic_bytes(case, nc, nf, s=state_bytes)
     dom = max(per_launch, key=per_launch.get)
     achieved = algo[dom] / (per_launch[dom] * 1e-6) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-            "algorithmic_bytes_per_launch": int(algo[dom]),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"{dom} ({head})", "algorithmic_bytes_per_launch": int(algo[dom]),
             "avg_launch_us": round(per_launch[dom], 2),
             "all_kernels_avg_us": {k: round(v, 2) for k, v in per_launch.items()}}
     if not args.no_traffic:
         child = ["--no-cpu", "--no-traffic", "--steps", "5", "--warmup", "2", "--kernel-reps", "5",
                  "--config", args.config, "--variant", str(args.variant),
-                 "--block-size", str(args.block_size)]
-        traffic, detail = measure_traffic(dom, child)
+                 "--block-size", str(args.block_size), "--precision", head]
+        traffic, detail = measure_traffic(dom, child, "f" if head == "fp32" else "d")
         roof["traffic"] = traffic
         if detail:
             roof["traffic_detail"] = detail
-    finite = bool(np.isfinite(s.state()).all())
-    out = {"metric": METRIC, "value": value, "unit": "cell-stage/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-           "data": "synthetic",
-           "config": {"workload": name, "cells": nc, "faces": nf, "precision": "FP64 plain",
-                      "kernel_variant": args.variant, "block_size": args.block_size,
-                      "state_finite": finite},
-           "roofline": roof}
+    q = r["solver"].state()
+    cfg = {"workload": name, "cells": nc, "faces": nf,
+           "precision": "FP32-mixed (FP32 perturbation state)" if head == "fp32" else "FP64 plain",
+           "kernel_variant": args.variant, "block_size": args.block_size,
+           "state_finite": bool(np.isfinite(q).all())}
+    out = {"metric": METRIC, "value": r["value"], "unit": "cell-stage/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": r["wall"] * 1e3 / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32" if head == "fp32" else "f64",
+           "data": "synthetic", "config": cfg, "roofline": roof}
+    if "fp64" in runs and head == "fp32":
+        r64 = runs["fp64"]
+        out["fp64"] = {"value": r64["value"], "ms_per_step": r64["wall"] * 1e3 / args.steps,
+                       "avg_launch_us": {k: round(v, 2) for k, v in r64["per_launch"].items()}}
+        out["speedup_vs_fp64"] = round(r["value"] / r64["value"], 4)
+        err = errors_vs_fp64(q, r64["solver"].state(), case.gamma)
+        out["error_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in err.items()}
+        out["error_vs_fp64"]["after_steps"] = args.warmup + args.steps
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(grid, case, nc)
     print(json.dumps(out))

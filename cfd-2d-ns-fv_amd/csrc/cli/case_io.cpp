@@ -129,6 +129,7 @@ bool read_inputs(const std::string& path, Inputs* in, std::string* log, std::str
         if (key == "block_size") in->block_size = std::stoi(val);
         if (key == "xcd_remap") in->xcd_remap = std::stoi(val);
         if (key == "kernel_variant") in->kernel_variant = std::stoi(val);
+        if (key == "precision") in->precision = (val == "FP32_MIXED") ? FVK_FP32_MIXED : (val == "FP64" ? FVK_FP64 : -1);
       }
     }
   }
@@ -252,6 +253,7 @@ fvk_options as_options(const Inputs& in) {
   o.block_size = in.block_size;
   o.xcd_remap = in.xcd_remap;
   o.kernel_variant = in.kernel_variant;
+  o.precision = in.precision;
   return o;
 }
 

@@ -1,4 +1,13 @@
-// fv_kernels.hpp — argument blocks and launchers of the FP64 FV kernels.
+// fv_kernels.hpp — argument blocks and launchers of the FV kernels.
+//
+// The kernels are templates on the arithmetic type T of the state:
+//   double  the FP64 path: state = conserved variables, arithmetic in FP64,
+//           bit-identical to the reference;
+//   float   the FP32-mixed path: state = FP32 perturbation from freestream
+//           (dQ = Q - Q_inf), reconstruction, limiter, flux and update in
+//           FP32; geometry and LSQ inverses stay FP64 (centroid and face-centre
+//           offsets are formed in FP64 before rounding) and the LSQ solve
+//           multiplies in FP64.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -41,19 +50,22 @@ struct DevMesh {
   const double* bf_fcy;
 };
 
-// Gas and freestream constants (class_flow, class_Q members).
-struct Flow {
-  double gamma, gm1, Rgas, mu, k;
-  double cfl2;                 // 2*CFL
-  double qinf[4];              // freestream conserved state (class_flow::updateQ)
-  double rho, u, v, P;         // freestream primitives
-  double vn0, c0;              // initial face |Vn_avg| and c_avg (class_q.hpp:61-62)
+// Gas and freestream constants in the arithmetic type (class_flow, class_Q).
+template <typename T>
+struct FlowT {
+  T gamma, gm1, Rgas, mu, k;
+  T cfl2;                      // 2*CFL
+  T qinf[4];                   // freestream conserved state (class_flow::updateQ)
+  T rho, u, v, P;              // freestream primitives
+  T vn0, c0;                   // initial face |Vn_avg| and c_avg (class_q.hpp:61-62)
+  T sent[4];                   // the -101 initial face state, in stored-state units
 };
 
-// One RK stage.  State arrays are SoA blocks of ncp doubles: q[k*ncp + cell].
-struct StageArgs {
+// One RK stage.  State arrays are SoA blocks of ncp values: q[k*ncp + cell].
+template <typename T>
+struct StageArgsT {
   DevMesh m;
-  Flow fl;
+  FlowT<T> fl;
   int32_t stage;               // 1, 2, 3
   int32_t dt_global;           // time_step_type GLOBAL
   int32_t sentinel;            // lagged face states are the -101 initial values
@@ -62,32 +74,37 @@ struct StageArgs {
   int32_t xcd_remap;           // give each XCD a contiguous range of blocks
   int32_t xcd_blocks;          // blocks per XCD range (set by the launcher)
   int32_t variant;             // kVariantCell, kVariantSlot or kVariantLds
-  const double* qc;            // stage input state (4 blocks)
-  const double* qp;            // previous stage's input state (lagged limiter)
-  const double* gp;            // previous stage's limited gradient (8 blocks)
-  double* g;                   // this stage's limited gradient (8 blocks)
-  const double* qn;            // state at the start of the step
-  double* qout;                // stage output state
-  double* sigma;               // per-cell spectral sum of the last stage 3
-  unsigned long long* smax_cur;   // max spectral sum used by this step (GLOBAL)
+  const T* qc;                 // stage input state (4 blocks)
+  const T* qp;                 // previous stage's input state (lagged limiter)
+  const T* gp;                 // previous stage's limited gradient (8 blocks)
+  T* g;                        // this stage's limited gradient (8 blocks)
+  const T* qn;                 // state at the start of the step
+  T* qout;                     // stage output state
+  T* sigma;                    // per-cell spectral sum of the last stage 3
+  unsigned long long* smax_cur;   // max spectral sum used by this step (GLOBAL), FP64 bits
   unsigned long long* smax_next;  // max spectral sum written by stage 3
-  double* res;                 // residual (4 blocks)
+  T* res;                      // residual (4 blocks)
 };
 
 // launch shapes: one thread per cell; one thread per (cell, face slot); or
 // LDS-staged blocks (block + halo in LDS, one thread per face for the flux)
 enum { kVariantCell = 1, kVariantSlot = 2, kVariantLds = 3 };
 
-// LDS bytes the LDS-staged kernels need for this mesh
-size_t lds_bytes_pass_a(const DevMesh& m);
-size_t lds_bytes_pass_b(const DevMesh& m, int order, int stage);
+// LDS bytes the LDS-staged kernels need for this mesh and state type size
+size_t lds_bytes_pass_a(const DevMesh& m, size_t tsize);
+size_t lds_bytes_pass_b(const DevMesh& m, int order, int stage, size_t tsize);
 
-hipError_t launch_pass_a(const StageArgs& a, int squeeze, int block, hipStream_t s);
-hipError_t launch_pass_b(const StageArgs& a, int order, int ns, int block, hipStream_t s);
-hipError_t launch_init_sigma(const StageArgs& a, int block, hipStream_t s);
+template <typename T>
+hipError_t launch_pass_a(const StageArgsT<T>& a, int squeeze, int block, hipStream_t s);
+template <typename T>
+hipError_t launch_pass_b(const StageArgsT<T>& a, int order, int ns, int block, hipStream_t s);
+template <typename T>
+hipError_t launch_init_sigma(const StageArgsT<T>& a, int block, hipStream_t s);
 // per-block partial sums of R_k^2 over all ncp cells: out[k*nblocks + b]
-hipError_t launch_res_sq(const double* res, int32_t ncp, double* partial, int nblocks, int block,
+template <typename T>
+hipError_t launch_res_sq(const T* res, int32_t ncp, double* partial, int nblocks, int block,
                          hipStream_t s);
-hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
+template <typename T>
+hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s);
 
 }  // namespace fvk

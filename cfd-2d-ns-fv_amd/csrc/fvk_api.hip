@@ -26,12 +26,17 @@ struct fvk_ctx {
   hipStream_t stream = nullptr;
   std::vector<void*> bufs;   // device allocations, freed by fvk_destroy
   fvk::DevMesh dm{};
-  fvk::Flow fl{};
-  double* q[3] = {nullptr, nullptr, nullptr};   // Qn, Q1, Q2 (4*ncp each)
-  double* g[3] = {nullptr, nullptr, nullptr};   // stage 1..3 limited gradients (8*ncp each)
-  double* sigma = nullptr;
-  double* res = nullptr;
-  unsigned long long* smax = nullptr;           // [2]
+  fvk::FlowT<double> fl64{};
+  fvk::FlowT<float> fl32{};
+  double qinf[4] = {0, 0, 0, 0};                // freestream conserved state (FP64)
+  bool fp32 = false;                            // FP32-mixed: state = float(Q - Q_inf)
+  size_t tsize = 8;                             // bytes per state value
+  // state arrays of the arithmetic type (double, or float perturbations)
+  void* q[3] = {nullptr, nullptr, nullptr};     // Qn, Q1, Q2 (4*ncp each)
+  void* g[3] = {nullptr, nullptr, nullptr};     // stage 1..3 limited gradients (8*ncp each)
+  void* sigma = nullptr;                        // ncp
+  void* res = nullptr;                          // 4*ncp
+  unsigned long long* smax = nullptr;           // [2], FP64 bits
   double* partial = nullptr;
   int block = 256;
   int variant = fvk::kVariantLds;
@@ -94,7 +99,7 @@ int check_options(const fvk_options& o, std::string* msg) {
   if (o.time_scheme != 1 && o.time_scheme != 2) { *msg = "time_scheme must be SSP_RK2 or SSP_RK3"; return FVK_ERR_OPTION; }
   if (o.time_step_type != 1 && o.time_step_type != 2) { *msg = "time_step_type must be GLOBAL or LOCAL"; return FVK_ERR_OPTION; }
   if (o.bc1 < -4 || o.bc1 > -1 || o.bc2 < -4 || o.bc2 > -1) { *msg = "BC1/BC2 must be FREESTREAM, SLIP_WALL, EXTRAPOLATION or NO_SLIP_WALL"; return FVK_ERR_OPTION; }
-  if (o.precision != FVK_FP64) { *msg = "precision must be FVK_FP64"; return FVK_ERR_OPTION; }
+  if (o.precision != FVK_FP64 && o.precision != FVK_FP32_MIXED) { *msg = "precision must be FVK_FP64 or FVK_FP32_MIXED"; return FVK_ERR_OPTION; }
   if (o.kernel_variant < 0 || o.kernel_variant > 3) { *msg = "kernel_variant must be 0, 1, 2 or 3"; return FVK_ERR_OPTION; }
   if (o.renumber < 0 || o.renumber > 2) { *msg = "renumber must be 0, 1 or 2"; return FVK_ERR_OPTION; }
   if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64)) {
@@ -105,18 +110,36 @@ int check_options(const fvk_options& o, std::string* msg) {
   return FVK_OK;
 }
 
-fvk::StageArgs base_args(fvk_ctx* c) {
-  fvk::StageArgs a{};
+template <typename T> const fvk::FlowT<T>& flow(const fvk_ctx* c);
+template <> const fvk::FlowT<double>& flow<double>(const fvk_ctx* c) { return c->fl64; }
+template <> const fvk::FlowT<float>& flow<float>(const fvk_ctx* c) { return c->fl32; }
+
+template <typename T>
+fvk::StageArgsT<T> base_args(fvk_ctx* c) {
+  fvk::StageArgsT<T> a{};
   a.m = c->dm;
-  a.fl = c->fl;
+  a.fl = flow<T>(c);
   a.dt_global = c->opt.time_step_type == 1;
   a.xcd_remap = c->opt.xcd_remap;
   a.variant = c->variant;
-  a.sigma = c->sigma;
-  a.res = c->res;
+  a.sigma = static_cast<T*>(c->sigma);
+  a.res = static_cast<T*>(c->res);
   a.smax_cur = c->smax + c->smax_cur;
   a.smax_next = c->smax + (1 - c->smax_cur);
   return a;
+}
+
+// stage roles of the three state and gradient buffers (ssp_rk3.cpp:38-76):
+// stage s reads Q_{s-1} and the previous stage's Q and limited gradient (the
+// lagged limiter), writes G_s and Q_s; stage 3 writes Q_n in place
+template <typename T>
+void set_roles(fvk_ctx* c, int stage, fvk::StageArgsT<T>& a) {
+  T* const* q = reinterpret_cast<T* const*>(c->q);
+  T* const* g = reinterpret_cast<T* const*>(c->g);
+  a.qn = q[0];
+  if (stage == 1) { a.qc = q[0]; a.qp = q[2]; a.gp = g[2]; a.g = g[0]; a.qout = q[1]; }
+  if (stage == 2) { a.qc = q[1]; a.qp = q[0]; a.gp = g[0]; a.g = g[1]; a.qout = q[2]; }
+  if (stage == 3) { a.qc = q[2]; a.qp = q[1]; a.gp = g[1]; a.g = g[2]; a.qout = q[0]; }
 }
 
 // Spectral sums |V_inf| + c_inf on every face and their max: the lagged face
@@ -124,8 +147,8 @@ fvk::StageArgs base_args(fvk_ctx* c) {
 int arm_start(fvk_ctx* c) {
   c->smax_cur = 0;
   FVK_HIP(c, fvk::launch_fill(reinterpret_cast<double*>(c->smax), 2, 0.0, c->stream));
-  fvk::StageArgs a = base_args(c);
-  FVK_HIP(c, fvk::launch_init_sigma(a, c->block, c->stream));
+  if (c->fp32) FVK_HIP(c, fvk::launch_init_sigma(base_args<float>(c), c->block, c->stream));
+  else FVK_HIP(c, fvk::launch_init_sigma(base_args<double>(c), c->block, c->stream));
   c->sentinel = true;
   c->res_valid = false;
   c->have_state = true;
@@ -135,20 +158,17 @@ int arm_start(fvk_ctx* c) {
 
 enum { kKindA = 1, kKindB = 2 };
 
-// One ssp_rk3 call.  timer(kind, launch) lets fvk_time_steps bracket launches.
-template <typename Launch>
-int run_step(fvk_ctx* c, bool store_res, Launch&& launch) {
+// One ssp_rk3 call.  launch(kind, fn) lets fvk_time_steps bracket launches.
+template <typename T, typename Launch>
+int run_step_t(fvk_ctx* c, bool store_res, Launch&& launch) {
   const fvk_options& o = c->opt;
   for (int stage = 1; stage <= 3; ++stage) {
-    fvk::StageArgs a = base_args(c);
+    fvk::StageArgsT<T> a = base_args<T>(c);
     a.stage = stage;
     a.sentinel = (stage == 1 && c->sentinel) ? 1 : 0;
     a.fs_prims = (stage == 1 && c->fs_prims) ? 1 : 0;
     a.store_res = (stage == 3 && store_res) ? 1 : 0;
-    a.qn = c->q[0];
-    if (stage == 1) { a.qc = c->q[0]; a.qp = c->q[2]; a.gp = c->g[2]; a.g = c->g[0]; a.qout = c->q[1]; }
-    if (stage == 2) { a.qc = c->q[1]; a.qp = c->q[0]; a.gp = c->g[0]; a.g = c->g[1]; a.qout = c->q[2]; }
-    if (stage == 3) { a.qc = c->q[2]; a.qp = c->q[1]; a.gp = c->g[1]; a.g = c->g[2]; a.qout = c->q[0]; }
+    set_roles(c, stage, a);
     if (o.order == 2) {
       int rc = launch(kKindA, [&]() { return fvk::launch_pass_a(a, o.limiter == 2, c->block, c->stream); });
       if (rc) return rc;
@@ -161,6 +181,91 @@ int run_step(fvk_ctx* c, bool store_res, Launch&& launch) {
   c->smax_cur = 1 - c->smax_cur;
   if (store_res) c->res_valid = true;
   return FVK_OK;
+}
+
+template <typename Launch>
+int run_step(fvk_ctx* c, bool store_res, Launch&& launch) {
+  return c->fp32 ? run_step_t<float>(c, store_res, launch) : run_step_t<double>(c, store_res, launch);
+}
+
+// device array of n values of the arithmetic type -> doubles
+int download(fvk_ctx* c, const void* src, size_t n, std::vector<double>* out) {
+  out->resize(n);
+  if (!c->fp32) {
+    FVK_HIP(c, hipMemcpy(out->data(), src, n * 8, hipMemcpyDeviceToHost));
+    return FVK_OK;
+  }
+  std::vector<float> f(n);
+  FVK_HIP(c, hipMemcpy(f.data(), src, n * 4, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) (*out)[i] = f[i];
+  return FVK_OK;
+}
+
+template <typename T>
+int fill_t(fvk_ctx* c, void* p, int64_t n, double v) {
+  hipError_t e = fvk::launch_fill(static_cast<T*>(p), n, T(v), c->stream);
+  return e == hipSuccess ? FVK_OK : hip_err(c, e, "fill");
+}
+
+int fill(fvk_ctx* c, void* p, int64_t n, double v) {
+  return c->fp32 ? fill_t<float>(c, p, n, v) : fill_t<double>(c, p, n, v);
+}
+
+// class_flow (class_flow.hpp:28-46) and class_Q constants in type T
+template <typename T>
+void make_flow(const fvk_options& o, fvk::FlowT<T>* fl, bool delta) {
+  fl->gamma = T(o.gamma);
+  fl->gm1 = T(o.gamma - 1);
+  fl->Rgas = T(o.R);
+  fl->mu = T(o.mu);
+  const double Cp = o.R * o.gamma / (o.gamma - 1);
+  fl->k = T((o.mu * Cp) / o.Pr);
+  fl->cfl2 = T(2 * o.cfl);
+  const double qinf[4] = {o.rho, o.rho * o.u, o.rho * o.v,
+                          (o.P / (o.gamma - 1)) + 0.5 * o.rho * (o.u * o.u + o.v * o.v)};
+  for (int k = 0; k < 4; ++k) {
+    fl->qinf[k] = T(qinf[k]);
+    // the reference's -101 initial face states (class_q.hpp:59-60)
+    fl->sent[k] = T(delta ? -101.0 - qinf[k] : -101.0);
+  }
+  fl->rho = T(o.rho);
+  fl->u = T(o.u);
+  fl->v = T(o.v);
+  fl->P = T(o.P);
+  fl->vn0 = T(std::sqrt(o.v * o.v + o.u * o.u));
+  fl->c0 = T(std::sqrt(o.gamma * (o.P / o.rho)));
+}
+
+// `reps` back-to-back launches of the stage-2 pass A and pass B kernels, each
+// kind between one event pair.  Stage 2 reads Q1, Qn, G1 and writes G2, Q2, so
+// the repeats leave the state unchanged.
+template <typename T>
+int time_kernels_t(fvk_ctx* c, int32_t reps, double avg_us[2]) {
+  const fvk_options& o = c->opt;
+  fvk::StageArgsT<T> a = base_args<T>(c);
+  a.stage = 2;
+  set_roles(c, 2, a);
+  hipEvent_t e0, e1;
+  FVK_HIP(c, hipEventCreate(&e0));
+  FVK_HIP(c, hipEventCreate(&e1));
+  avg_us[0] = avg_us[1] = 0.0;
+  int rc = FVK_OK;
+  for (int kind = 0; kind < 2 && !rc; ++kind) {
+    if (kind == 0 && o.order != 2) continue;
+    hipError_t e = hipEventRecord(e0, c->stream);
+    for (int32_t r = 0; r < reps && e == hipSuccess; ++r)
+      e = kind == 0 ? fvk::launch_pass_a(a, o.limiter == 2, c->block, c->stream)
+                    : fvk::launch_pass_b(a, o.order, o.eqn == 2, c->block, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) rc = hip_err(c, e, "fvk_time_kernels");
+    else avg_us[kind] = ms * 1e3 / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 }  // namespace
@@ -218,6 +323,8 @@ int fvk_create(const fvk_mesh* mesh, const fvk_options* opt, fvk_ctx** out) {
   c->block = opt->block_size ? opt->block_size : 256;
   c->variant = opt->kernel_variant ? opt->kernel_variant : fvk::kVariantLds;
   c->res_blocks = (h.ncp + 255) / 256;
+  c->fp32 = opt->precision == FVK_FP32_MIXED;
+  c->tsize = c->fp32 ? sizeof(float) : sizeof(double);
   hipError_t e = hipStreamCreate(&c->stream);
   if (e != hipSuccess) {
     hip_err(nullptr, e, "hipStreamCreate");
@@ -270,7 +377,8 @@ int fvk_create(const fvk_mesh* mesh, const fvk_options* opt, fvk_ctx** out) {
       hip_err(c, e, "hipGetDeviceProperties");
       return fail(FVK_ERR_HIP);
     }
-    const size_t need = std::max(fvk::lds_bytes_pass_a(d), fvk::lds_bytes_pass_b(d, opt->order, 3));
+    const size_t need = std::max(fvk::lds_bytes_pass_a(d, c->tsize),
+                                 fvk::lds_bytes_pass_b(d, opt->order, 3, c->tsize));
     if (need > prop.sharedMemPerBlock) {
       if (opt->kernel_variant == fvk::kVariantLds) {
         set_err(c, FVK_ERR_MESH, "mesh blocks need " + std::to_string(need) +
@@ -281,42 +389,35 @@ int fvk_create(const fvk_mesh* mesh, const fvk_options* opt, fvk_ctx** out) {
     }
   }
   const size_t ncp = h.ncp;
+  auto alloc = [&](void** p, size_t n) {
+    unsigned char* d = nullptr;
+    const int r = dev_alloc(c, &d, n * c->tsize);
+    *p = d;
+    return r;
+  };
   for (int i = 0; i < 3; ++i) {
-    if ((rc = dev_alloc(c, &c->q[i], 4 * ncp))) return fail(rc);
-    if ((rc = dev_alloc(c, &c->g[i], 8 * ncp))) return fail(rc);
+    if ((rc = alloc(&c->q[i], 4 * ncp))) return fail(rc);
+    if ((rc = alloc(&c->g[i], 8 * ncp))) return fail(rc);
   }
-  if ((rc = dev_alloc(c, &c->sigma, ncp))) return fail(rc);
-  if ((rc = dev_alloc(c, &c->res, 4 * ncp))) return fail(rc);
+  if ((rc = alloc(&c->sigma, ncp))) return fail(rc);
+  if ((rc = alloc(&c->res, 4 * ncp))) return fail(rc);
   if ((rc = dev_alloc(c, &c->smax, 2))) return fail(rc);
   if ((rc = dev_alloc(c, &c->partial, 4 * (size_t)c->res_blocks))) return fail(rc);
   for (int i = 0; i < 3; ++i) {
-    if ((e = fvk::launch_fill(c->q[i], 4 * (int64_t)ncp, 0.0, c->stream)) != hipSuccess) { hip_err(c, e, "fill"); return fail(FVK_ERR_HIP); }
-    if ((e = fvk::launch_fill(c->g[i], 8 * (int64_t)ncp, 0.0, c->stream)) != hipSuccess) { hip_err(c, e, "fill"); return fail(FVK_ERR_HIP); }
+    if ((rc = fill(c, c->q[i], 4 * (int64_t)ncp, 0.0))) return fail(rc);
+    if ((rc = fill(c, c->g[i], 8 * (int64_t)ncp, 0.0))) return fail(rc);
   }
-  if ((e = fvk::launch_fill(c->res, 4 * (int64_t)ncp, 0.0, c->stream)) != hipSuccess) { hip_err(c, e, "fill"); return fail(FVK_ERR_HIP); }
-  if ((e = fvk::launch_fill(c->sigma, (int64_t)ncp, 0.0, c->stream)) != hipSuccess) { hip_err(c, e, "fill"); return fail(FVK_ERR_HIP); }
+  if ((rc = fill(c, c->res, 4 * (int64_t)ncp, 0.0))) return fail(rc);
+  if ((rc = fill(c, c->sigma, (int64_t)ncp, 0.0))) return fail(rc);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) { hip_err(c, e, "sync"); return fail(FVK_ERR_HIP); }
 
-  // class_flow (class_flow.hpp:28-46) and class_Q constants
   const fvk_options& o = *opt;
-  fvk::Flow& fl = c->fl;
-  fl.gamma = o.gamma;
-  fl.gm1 = o.gamma - 1;
-  fl.Rgas = o.R;
-  fl.mu = o.mu;
-  const double Cp = o.R * o.gamma / (o.gamma - 1);
-  fl.k = (o.mu * Cp) / o.Pr;
-  fl.cfl2 = 2 * o.cfl;
-  fl.qinf[0] = o.rho;
-  fl.qinf[1] = o.rho * o.u;
-  fl.qinf[2] = o.rho * o.v;
-  fl.qinf[3] = (o.P / (o.gamma - 1)) + 0.5 * o.rho * (o.u * o.u + o.v * o.v);
-  fl.rho = o.rho;
-  fl.u = o.u;
-  fl.v = o.v;
-  fl.P = o.P;
-  fl.vn0 = std::sqrt(o.v * o.v + o.u * o.u);
-  fl.c0 = std::sqrt(o.gamma * (o.P / o.rho));
+  make_flow(o, &c->fl64, false);
+  make_flow(o, &c->fl32, true);
+  c->qinf[0] = o.rho;
+  c->qinf[1] = o.rho * o.u;
+  c->qinf[2] = o.rho * o.v;
+  c->qinf[3] = (o.P / (o.gamma - 1)) + 0.5 * o.rho * (o.u * o.u + o.v * o.v);
   *out = c;
   return FVK_OK;
 }
@@ -334,8 +435,12 @@ int fvk_destroy(fvk_ctx* c) {
 int fvk_init_freestream(fvk_ctx* c) {
   if (!c) return FVK_ERR_ARG;
   const size_t ncp = c->hm.ncp;
-  for (int k = 0; k < 4; ++k)
-    FVK_HIP(c, fvk::launch_fill(c->q[0] + k * ncp, (int64_t)ncp, c->fl.qinf[k], c->stream));
+  for (int k = 0; k < 4; ++k) {
+    // FP64: Q_inf; FP32-mixed: the perturbation 0
+    void* blk = static_cast<unsigned char*>(c->q[0]) + k * ncp * c->tsize;
+    const int rc = fill(c, blk, (int64_t)ncp, c->fp32 ? 0.0 : c->qinf[k]);
+    if (rc) return rc;
+  }
   int rc = arm_start(c);
   c->fs_prims = true;
   return rc;
@@ -345,11 +450,21 @@ int fvk_set_state(fvk_ctx* c, const double* q) {
   if (!c) return FVK_ERR_ARG;
   if (!q) return set_err(c, FVK_ERR_ARG, "fvk_set_state: NULL state");
   const fvk::HostMesh& h = c->hm;
-  std::vector<double> buf(4 * (size_t)h.ncp, 1.0);
-  for (int k = 0; k < 4; ++k)
-    for (int32_t i = 0; i < h.ncp; ++i)
-      if (h.perm[i] >= 0) buf[k * (size_t)h.ncp + i] = q[k * (size_t)h.nc + h.perm[i]];
-  FVK_HIP(c, hipMemcpyAsync(c->q[0], buf.data(), buf.size() * 8, hipMemcpyHostToDevice, c->stream));
+  const size_t n = 4 * (size_t)h.ncp;
+  if (c->fp32) {
+    std::vector<float> buf(n, 0.0f);
+    for (int k = 0; k < 4; ++k)
+      for (int32_t i = 0; i < h.ncp; ++i)
+        if (h.perm[i] >= 0)
+          buf[k * (size_t)h.ncp + i] = float(q[k * (size_t)h.nc + h.perm[i]] - c->qinf[k]);
+    FVK_HIP(c, hipMemcpy(c->q[0], buf.data(), n * 4, hipMemcpyHostToDevice));
+  } else {
+    std::vector<double> buf(n, 1.0);
+    for (int k = 0; k < 4; ++k)
+      for (int32_t i = 0; i < h.ncp; ++i)
+        if (h.perm[i] >= 0) buf[k * (size_t)h.ncp + i] = q[k * (size_t)h.nc + h.perm[i]];
+    FVK_HIP(c, hipMemcpy(c->q[0], buf.data(), n * 8, hipMemcpyHostToDevice));
+  }
   int rc = arm_start(c);
   c->fs_prims = false;
   return rc;
@@ -377,15 +492,20 @@ int fvk_get_state(fvk_ctx* c, double* q, double* grad) {
   const fvk::HostMesh& h = c->hm;
   const size_t ncp = h.ncp;
   if (q) {
-    std::vector<double> buf(4 * ncp);
-    FVK_HIP(c, hipMemcpy(buf.data(), c->q[0], buf.size() * 8, hipMemcpyDeviceToHost));
-    for (int k = 0; k < 4; ++k)
+    std::vector<double> buf;
+    int rc = download(c, c->q[0], 4 * ncp, &buf);
+    if (rc) return rc;
+    for (int k = 0; k < 4; ++k) {
+      const double base = c->fp32 ? c->qinf[k] : 0.0;
       for (int32_t i = 0; i < h.ncp; ++i)
-        if (h.perm[i] >= 0) q[k * (size_t)h.nc + h.perm[i]] = buf[k * ncp + i];
+        if (h.perm[i] >= 0)
+          q[k * (size_t)h.nc + h.perm[i]] = c->fp32 ? base + buf[k * ncp + i] : buf[k * ncp + i];
+    }
   }
   if (grad) {
-    std::vector<double> buf(8 * ncp);
-    FVK_HIP(c, hipMemcpy(buf.data(), c->g[0], buf.size() * 8, hipMemcpyDeviceToHost));
+    std::vector<double> buf;
+    int rc = download(c, c->g[0], 8 * ncp, &buf);
+    if (rc) return rc;
     for (int k = 0; k < 8; ++k)
       for (int32_t i = 0; i < h.ncp; ++i)
         if (h.perm[i] >= 0) grad[k * (size_t)h.nc + h.perm[i]] = buf[k * ncp + i];
@@ -397,8 +517,9 @@ int fvk_get_residual(fvk_ctx* c, double* r) {
   if (!c) return FVK_ERR_ARG;
   if (!r) return set_err(c, FVK_ERR_ARG, "fvk_get_residual: NULL output");
   const fvk::HostMesh& h = c->hm;
-  std::vector<double> buf(4 * (size_t)h.ncp);
-  FVK_HIP(c, hipMemcpy(buf.data(), c->res, buf.size() * 8, hipMemcpyDeviceToHost));
+  std::vector<double> buf;
+  int rc = download(c, c->res, 4 * (size_t)h.ncp, &buf);
+  if (rc) return rc;
   for (int k = 0; k < 4; ++k)
     for (int32_t i = 0; i < h.ncp; ++i)
       if (h.perm[i] >= 0) r[k * (size_t)h.nc + h.perm[i]] = buf[k * (size_t)h.ncp + i];
@@ -408,7 +529,10 @@ int fvk_get_residual(fvk_ctx* c, double* r) {
 int fvk_residual_l2(fvk_ctx* c, double out[4]) {
   if (!c || !out) return FVK_ERR_ARG;
   const fvk::HostMesh& h = c->hm;
-  FVK_HIP(c, fvk::launch_res_sq(c->res, h.ncp, c->partial, c->res_blocks, 256, c->stream));
+  if (c->fp32)
+    FVK_HIP(c, fvk::launch_res_sq(static_cast<const float*>(c->res), h.ncp, c->partial, c->res_blocks, 256, c->stream));
+  else
+    FVK_HIP(c, fvk::launch_res_sq(static_cast<const double*>(c->res), h.ncp, c->partial, c->res_blocks, 256, c->stream));
   std::vector<double> part(4 * (size_t)c->res_blocks);
   FVK_HIP(c, hipMemcpyAsync(part.data(), c->partial, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
   FVK_HIP(c, hipStreamSynchronize(c->stream));
@@ -516,31 +640,7 @@ int fvk_time_steps(fvk_ctx* c, int32_t nsteps, double out_ms[4], int64_t counts[
 int fvk_time_kernels(fvk_ctx* c, int32_t reps, double avg_us[2]) {
   if (!c || !avg_us || reps <= 0) return FVK_ERR_ARG;
   if (!c->have_state) return set_err(c, FVK_ERR_ARG, "fvk_time_kernels: no state");
-  const fvk_options& o = c->opt;
-  fvk::StageArgs a = base_args(c);
-  a.stage = 2;
-  a.qn = c->q[0]; a.qc = c->q[1]; a.qp = c->q[0]; a.gp = c->g[0]; a.g = c->g[1]; a.qout = c->q[2];
-  hipEvent_t e0, e1;
-  FVK_HIP(c, hipEventCreate(&e0));
-  FVK_HIP(c, hipEventCreate(&e1));
-  avg_us[0] = avg_us[1] = 0.0;
-  int rc = FVK_OK;
-  for (int kind = 0; kind < 2 && !rc; ++kind) {
-    if (kind == 0 && o.order != 2) continue;
-    hipError_t e = hipEventRecord(e0, c->stream);
-    for (int32_t r = 0; r < reps && e == hipSuccess; ++r)
-      e = kind == 0 ? fvk::launch_pass_a(a, o.limiter == 2, c->block, c->stream)
-                    : fvk::launch_pass_b(a, o.order, o.eqn == 2, c->block, c->stream);
-    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    float ms = 0;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e != hipSuccess) rc = hip_err(c, e, "fvk_time_kernels");
-    else avg_us[kind] = ms * 1e3 / reps;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return rc;
+  return c->fp32 ? time_kernels_t<float>(c, reps, avg_us) : time_kernels_t<double>(c, reps, avg_us);
 }
 
 }  // extern "C"

@@ -19,6 +19,9 @@ FVK_OK = 0
 ERRORS = {-1: "FVK_ERR_ARG", -2: "FVK_ERR_MESH", -3: "FVK_ERR_OPTION", -4: "FVK_ERR_HIP",
           -5: "FVK_ERR_NONFINITE", -6: "FVK_ERR_UNSUPPORTED"}
 BC_CODES = {"FREESTREAM": -1, "SLIP_WALL": -2, "EXTRAPOLATION": -3, "NO_SLIP_WALL": -4}
+# precision forms (FVK_FP64, FVK_FP32_MIXED)
+FP64 = 0
+FP32_MIXED = 1
 
 # every symbol include/fvk.h declares
 EXPORTS = ["fvk_default_options", "fvk_create", "fvk_destroy", "fvk_last_error",

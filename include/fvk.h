@@ -37,6 +37,10 @@ extern "C" {
 
 /* precision forms */
 #define FVK_FP64 0         /* plain FP64: reproduces the reference */
+#define FVK_FP32_MIXED 1   /* state stored as the FP32 perturbation Q - Q_inf; face
+                              states, fluxes and the update in FP32 perturbation
+                              form; geometry, LSQ inverses and the dt maximum FP64.
+                              No counterpart in the reference (it is FP64 only). */
 
 typedef struct fvk_ctx fvk_ctx;
 
@@ -72,11 +76,14 @@ typedef struct fvk_options {
   double cfl;
   double gamma, R, mu, Pr; /* [fluid] */
   double P, rho, u, v;     /* [init] freestream */
-  int32_t precision;       /* FVK_FP64 */
-  int32_t renumber;        /* 1: reverse Cuthill-McKee cell order (default), 0: file order */
-  int32_t block_size;      /* cell-per-thread shape: 0 = 256; else 64, 128, 192 or 256 */
-  int32_t xcd_remap;       /* 1: contiguous cell ranges per XCD (default), 0: off */
-  int32_t kernel_variant;  /* 0: default; 1: one thread per cell; 2: one thread per face slot */
+  int32_t precision;       /* FVK_FP64 (default) or FVK_FP32_MIXED */
+  int32_t renumber;        /* cell blocks: 1 recursive coordinate bisection (default),
+                              2 reverse Cuthill-McKee ranges, 0 file-order ranges */
+  int32_t block_size;      /* cells per block: 0 = 256; else 64, 128, 192 or 256 */
+  int32_t xcd_remap;       /* 1: contiguous block ranges per XCD (default), 0: off */
+  int32_t kernel_variant;  /* 0: default (3 when the blocks fit in LDS, else 1);
+                              1: one thread per cell; 2: one thread per face slot;
+                              3: LDS-staged blocks */
   int32_t reserved_;
 } fvk_options;
 
@@ -106,7 +113,8 @@ int fvk_step(fvk_ctx* ctx, int32_t nsteps);
 
 /* Conserved state q[4*N_c]; grad[8*N_c] (nullable) = the stage-1 limited
  * gradients Qxp1..4, Qyp1..4 that Qbar holds after ssp_rk3 and that
- * save_sol.cpp:34-37,123-142 reads (zeros at first order). */
+ * save_sol.cpp:34-37,123-142 reads (zeros at first order).  FVK_FP32_MIXED
+ * returns Q_inf + (double)dQ and the FP32 gradients widened. */
 int fvk_get_state(fvk_ctx* ctx, double* q, double* grad);
 /* Stage-3 residual r[4*N_c] of the last step (class_residual after ssp_rk3). */
 int fvk_get_residual(fvk_ctx* ctx, double* r);
