@@ -1,25 +1,29 @@
 """Benchmark: cell-RK-stage updates/s of the device path on one MI355X, in
 the FP32-mixed precision form with the FP64 path timed beside it.
 
-Workload (default): the 15-degree compression ramp, 512x256 quads with a
-seed-chosen 50% split into triangles (196,608 cells, 328,448 faces), Euler,
-second order, SQUEEZE limiter, LOCAL time step, CFL 0.8, M=2.5.  One "step"
-is one SSP-RK3 iteration (3 residual evaluations + stage updates) over the
-whole mesh with all inputs resident in HBM.
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--config batch|ramp|channel|cylinder] [--precision both|fp32|fp64]
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ramp|bump|ogrid]
-                    [--precision both|fp32|fp64]
+A "step" is one SSP-RK3 iteration (3 residual evaluations + stage updates)
+over every active cell, with all inputs resident in HBM.  Prints one JSON
+line; `value` = (sum over instances of cells * 3 * steps) / wall seconds of
+the timed run of the FP32-mixed path (the library call returns after its
+stream has drained).
 
-Prints one JSON line.  `value` = cells * 3 * K / wall seconds of the K timed
-steps of the FP32-mixed path (the library call returns after its stream has
-drained); `fp64` holds the same for the FP64 path (bit-identical to the
-reference), `speedup_vs_fp64` their ratio, and `error_vs_fp64` the relative
-L2/Linf density and pressure errors of the FP32-mixed state against the FP64
-state after the same steps.  The `roofline` object prices the dominant
-kernel's algorithmic bytes per launch against its average launch time
-measured with HIP events on the library's stream; `cpu_baseline` is the
-reference solver built from its own sources (or, where that build is absent,
-the C oracle) timed on one host core over a bounded sample of the workload.
+--config batch (default): 256 independent channel/ramp instances (about 11M
+  cells, Euler, second order, SQUEEZE, LOCAL dt) in one batch, run to a 1e-6
+  density-residual drop or K steps, converged instances retired.  Beside the
+  FP32-mixed batch: the FP64 batch, and 256 sequential FP64 runs of the plain
+  single-instance path (each instance timed alone over a few steps, priced at
+  the steps its FP64 run took); per-instance FP32-mixed errors against FP64.
+--config ramp: the single 512x256 ramp mesh (196,608 cells), FP32-mixed and
+  FP64, K steps each.
+
+`roofline` prices the dominant kernel's algorithmic bytes per launch against
+its average launch time measured with HIP events on the library's stream;
+`traffic` is measured with rocprofv3 counters in child runs.  `cpu_baseline`
+is the reference solver built from its own sources (or, where that build is
+absent, the C oracle) timed on one host core over a bounded sample.
 """
 from __future__ import annotations
 
@@ -36,37 +40,25 @@ sys.path.insert(0, os.path.join(ROOT, "cfd-2d-ns-fv_amd"))
 
 import numpy as np  # noqa: E402
 
-from fv2d import gridgen as gg  # noqa: E402
 from fv2d import inputs  # noqa: E402
-from fv2d.lib import FP32_MIXED, FP64, Solver  # noqa: E402
+from fv2d import workloads as W  # noqa: E402
+from fv2d.lib import FP32_MIXED, FP64, Batch, Solver  # noqa: E402
 
 METRIC = "cell-RK-stage updates/s, FP32-mixed vs FP64 at stated error; % HBM peak"
 HBM_PEAK_GBS = 8000.0
 PRECISIONS = {"fp64": FP64, "fp32": FP32_MIXED}
 
 
-def workload(name: str):
-    if name == "ramp":
-        u, v = inputs.mach_velocity(2.5)
-        return ("ramp512x256-split50 Euler o2 SQUEEZE LOCAL CFL0.8 M2.5", gg.ramp(512, 256),
-                inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE", dt_type="LOCAL", CFL=0.8))
-    if name == "bump":
-        u, v = inputs.mach_velocity(0.5)
-        return ("bump64x32 Euler o1 GLOBAL CFL0.5 M0.5", gg.bump(64, 32),
-                inputs.Case(u=u, v=v, poly=0, dt_type="GLOBAL", CFL=0.5))
-    if name == "ogrid":
-        u, v = inputs.mach_velocity(0.2)
-        return ("ogrid1024x512 NS Re100 o2 SQUEEZE GLOBAL CFL0.5 M0.2", gg.ogrid(1024, 512),
-                inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE", eqn="NS", mu=1.225 * u / 100.0,
-                            BC1="NO_SLIP_WALL", dt_type="GLOBAL", CFL=0.5))
-    raise SystemExit(f"unknown config {name}")
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def algorithmic_bytes(case, nc: int, nf: int, s: int = 8, g: int = 8, i: int = 4):
     """Compulsory HBM bytes per launch of each kernel (minimal-traffic model:
     every array read or written once per pass, neighbour re-reads free,
     derived geometry recomputed from centroids, face centres, normals,
-    lengths, volumes and LSQ inverses)."""
+    lengths, volumes and LSQ inverses).  s = state bytes, g = geometry bytes,
+    i = index bytes."""
     if case.poly == 0:
         # one pass: Q (4s) + Q_n (8s/3) + dt (s) + write Q (4s) + sigma (s/3) + V (g)
         b = nc * (12 * s + g) + nf * (2 * i + 2 * g)
@@ -78,40 +70,48 @@ def algorithmic_bytes(case, nc: int, nf: int, s: int = 8, g: int = 8, i: int = 4
     return {"pass_a": a, "pass_b": b}
 
 
-def cpu_baseline(grid, case, nc: int, target_s: float = 15.0):
+def cpu_baseline(samples, target_s: float = 15.0):
     """Time the reference solver (oracle/_ref/ref_harness) or the C oracle on
-    one core over a bounded number of steps of the same workload."""
+    one core over a bounded number of steps of each (grid, case) sample."""
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
-    if os.path.exists(harness):
-        d = tempfile.mkdtemp()
-        grid.write(os.path.join(d, "grid.in"))
-        c = inputs.Case(**{**case.__dict__, "grid_file": "grid.in"})
-        c.write(os.path.join(d, "input.in"))
+    per = target_s / len(samples)
+    work = 0.0
+    secs = 0.0
+    parts = []
+    for grid, case in samples:
+        nc = grid.num_cells
+        if os.path.exists(harness):
+            d = tempfile.mkdtemp()
+            grid.write(os.path.join(d, "grid.in"))
+            c = inputs.Case(**{**case.__dict__, "grid_file": "grid.in"})
+            c.write(os.path.join(d, "input.in"))
 
-        def run(n):
-            r = subprocess.run(["taskset", "-c", "0", harness, "input.in", "out.bin", str(n),
-                                "--dump", "0", "--time"], cwd=d, capture_output=True, text=True,
-                               check=True)
-            return float(r.stdout.strip().splitlines()[-1].split()[1])
+            def run(n):
+                r = subprocess.run(["taskset", "-c", "0", harness, "input.in", "out.bin", str(n),
+                                    "--dump", "0", "--time"], cwd=d, capture_output=True,
+                                   text=True, check=True)
+                return float(r.stdout.strip().splitlines()[-1].split()[1])
+            kind = "reference"
+        else:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            o = O.Oracle(grid, case)
 
+            def run(n):
+                t0 = time.perf_counter()
+                o.step(n)
+                return time.perf_counter() - t0
+            kind = "port"
         t1 = run(2)
-        n = max(2, min(400, int(target_s / max(t1 / 2, 1e-6))))
+        n = max(2, min(400, int(per / max(t1 / 2, 1e-6))))
         t = run(n)
-        kind, what = "reference", f"reference solver built from its own sources, {n} steps"
-    else:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        o = O.Oracle(grid, case)
-        t0 = time.perf_counter()
-        o.step(2)
-        t1 = time.perf_counter() - t0
-        n = max(2, min(400, int(target_s / max(t1 / 2, 1e-6))))
-        t0 = time.perf_counter()
-        o.step(n)
-        t = time.perf_counter() - t0
-        kind, what = "port", f"C oracle (serial restatement), {n} steps"
-    return {"value": nc * 3 * n / t, "unit": "cell-stage/s", "cores": 1, "kind": kind,
-            "sample": f"{what} of the same workload ({t:.2f} s of step loop)"}
+        work += nc * 3 * n
+        secs += t
+        parts.append(f"{nc} cells x {n} steps")
+    what = ("reference solver built from its own sources" if kind == "reference"
+            else "C oracle (serial restatement)")
+    return {"value": work / secs, "unit": "cell-stage/s", "cores": 1, "kind": kind,
+            "sample": f"{what}, {'; '.join(parts)} of the same workload ({secs:.2f} s of step loop)"}
 
 
 def copy_calibration(mbytes: int) -> None:
@@ -133,7 +133,7 @@ def copy_calibration(mbytes: int) -> None:
     print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
 
 
-def _pmc_mean_kib(counter: str, child_args, match, timeout: int = 180):
+def _pmc_mean_kib(counter: str, child_args, match, timeout: int = 300):
     """Mean value (KiB) of `counter` per dispatch of the kernels whose name
     contains `match` (or satisfies it, if callable), from one
     `rocprofv3 --pmc` run of this script."""
@@ -172,6 +172,7 @@ def measure_traffic(kernel: str, child_args, tchar: str):
 
     def match(name):   # demangled (k_pass_b_lds<float, 2, false>) or mangled (IfLi2...)
         return ("k_" + kernel + "_") in name and (f"<{tname}" in name or f"I{tchar}L" in name)
+    log(f"traffic: FETCH_SIZE / WRITE_SIZE of {kernel} ({tname})")
     fetch = _pmc_mean_kib("FETCH_SIZE", child_args, match)
     write = _pmc_mean_kib("WRITE_SIZE", child_args, match)
     cf = _pmc_mean_kib("FETCH_SIZE", ["--copy-calibration", str(copy_mb)], "copyBuffer")
@@ -185,21 +186,6 @@ def measure_traffic(kernel: str, child_args, tchar: str):
               "fetch_scale": round(sf, 4), "write_scale": round(sw, 4),
               "calibration": f"{copy_mb} MiB device-to-device copy"}
     return int(traffic), detail
-
-
-def run_precision(prec: str, args, grid, case):
-    """Warm up, time args.steps steps, then time each kernel kind per launch."""
-    nc = grid.num_cells
-    s = Solver(grid, case, kernel_variant=args.variant, block_size=args.block_size,
-               precision=PRECISIONS[prec])
-    s.init_freestream()
-    s.step(args.warmup)
-    t0 = time.perf_counter()
-    s.step(args.steps)
-    wall = time.perf_counter() - t0
-    per_launch = {k: v for k, v in s.time_kernels(args.kernel_reps).items() if v > 0}
-    return {"solver": s, "wall": wall, "value": nc * 3 * args.steps / wall,
-            "per_launch": per_launch}
 
 
 def errors_vs_fp64(q32, q64, gamma):
@@ -217,50 +203,8 @@ def errors_vs_fp64(q32, q64, gamma):
             "p_l2_rel": l2(p(q32), p(q64)), "p_linf_rel": linf(p(q32), p(q64))}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--config", default="ramp")
-    ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
-                    help="both: FP32-mixed is the measured value, FP64 is timed beside it and "
-                         "is the oracle of the stated error")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel-reps", type=int, default=200,
-                    help="back-to-back launches per kernel for the per-launch timing")
-    ap.add_argument("--no-traffic", action="store_true",
-                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
-    ap.add_argument("--variant", type=int, default=0,
-                    help="0 default, 1 cell-parallel, 2 slot-parallel, 3 LDS-staged blocks")
-    ap.add_argument("--block-size", type=int, default=0,
-                    help="cells per block (LDS-staged) or threads per block (cell-parallel)")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
-                    help="only run device-to-device copies of MB MiB (counter calibration)")
-    args = ap.parse_args()
-    if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
-        return
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(instances or mesh partitions would shard across GPUs, see DESIGN.md)",
-              file=sys.stderr)
-        sys.exit(2)
-    name, grid, case = workload(args.config)
-    nc, nf = grid.num_cells, grid.num_faces
-    runs = {}
-    for prec in (["fp32", "fp64"] if args.precision == "both" else [args.precision]):
-        runs[prec] = run_precision(prec, args, grid, case)
-        print(f"bench: {prec} {runs[prec]['value']:.4g} cell-stage/s", file=sys.stderr, flush=True)
-    head = "fp32" if "fp32" in runs else "fp64"
-    r = runs[head]
-    state_bytes = 4 if head == "fp32" else 8
-
-    # roofline of the dominant kernel of the measured path: algorithmic bytes
-    # per launch over the per-launch time between HIP events on the library's
-    # stream (pass A and pass B each run once per stage)
-    per_launch = r["per_launch"]
-    algo = algorithmic_bytes(case, nc, nf, s=state_bytes)
+def roofline(per_launch, algo, head, args, child):
+    """Roofline object of the dominant kernel (longest average launch)."""
     dom = max(per_launch, key=per_launch.get)
     achieved = algo[dom] / (per_launch[dom] * 1e-6) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -269,13 +213,45 @@ def main():
             "avg_launch_us": round(per_launch[dom], 2),
             "all_kernels_avg_us": {k: round(v, 2) for k, v in per_launch.items()}}
     if not args.no_traffic:
-        child = ["--no-cpu", "--no-traffic", "--steps", "5", "--warmup", "2", "--kernel-reps", "5",
-                 "--config", args.config, "--variant", str(args.variant),
-                 "--block-size", str(args.block_size), "--precision", head]
         traffic, detail = measure_traffic(dom, child, "f" if head == "fp32" else "d")
         roof["traffic"] = traffic
         if detail:
             roof["traffic_detail"] = detail
+    return roof
+
+
+# ---------------------------------------------------------------------------
+# single mesh
+# ---------------------------------------------------------------------------
+def run_single(prec: str, args, grid, case):
+    """Warm up, time args.steps steps, then time each kernel kind per launch."""
+    nc = grid.num_cells
+    s = Solver(grid, case, kernel_variant=args.variant, block_size=args.block_size,
+               precision=PRECISIONS[prec])
+    s.init_freestream()
+    s.step(args.warmup)
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    wall = time.perf_counter() - t0
+    per_launch = {k: v for k, v in s.time_kernels(args.kernel_reps).items() if v > 0}
+    return {"solver": s, "wall": wall, "value": nc * 3 * args.steps / wall,
+            "per_launch": per_launch}
+
+
+def bench_single(args):
+    name, grid, case = W.SINGLE[args.config]()
+    nc, nf = grid.num_cells, grid.num_faces
+    runs = {}
+    for prec in (["fp32", "fp64"] if args.precision == "both" else [args.precision]):
+        runs[prec] = run_single(prec, args, grid, case)
+        log(f"{prec} {runs[prec]['value']:.4g} cell-stage/s")
+    head = "fp32" if "fp32" in runs else "fp64"
+    r = runs[head]
+    algo = algorithmic_bytes(case, nc, nf, s=4 if head == "fp32" else 8)
+    child = ["--no-cpu", "--no-traffic", "--steps", "5", "--warmup", "2", "--kernel-reps", "5",
+             "--config", args.config, "--variant", str(args.variant),
+             "--block-size", str(args.block_size), "--precision", head]
+    roof = roofline(r["per_launch"], algo, head, args, child)
     q = r["solver"].state()
     cfg = {"workload": name, "cells": nc, "faces": nf,
            "precision": "FP32-mixed (FP32 perturbation state)" if head == "fp32" else "FP64 plain",
@@ -295,7 +271,166 @@ def main():
         out["error_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in err.items()}
         out["error_vs_fp64"]["after_steps"] = args.warmup + args.steps
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(grid, case, nc)
+        out["cpu_baseline"] = cpu_baseline([(grid, case)])
+    return out
+
+
+# ---------------------------------------------------------------------------
+# batch
+# ---------------------------------------------------------------------------
+def run_batch(prec: str, args, cases, nc):
+    """Warm up, restart, then time one run to the residual drop (or
+    args.steps steps) with retirement; then time each kernel kind per launch
+    over all instances."""
+    b = Batch(cases, precision=PRECISIONS[prec], block_size=args.block_size)
+    b.init()
+    if args.warmup:
+        b.step(args.warmup)
+        b.init()
+    t0 = time.perf_counter()
+    steps, r0, r1 = b.run_to_drop(args.drop, args.steps, args.check_every)
+    wall = time.perf_counter() - t0
+    work = float((nc * 3 * steps).sum())
+    b.set_active(np.ones(len(cases), dtype=np.uint8))
+    per_launch = {k: v for k, v in b.time_kernels(args.kernel_reps).items() if v > 0}
+    return {"batch": b, "wall": wall, "value": work / wall, "steps": steps, "r0": r0, "r1": r1,
+            "per_launch": per_launch, "work": work}
+
+
+def sequential_fp64(cases, steps, reps: int = 20):
+    """The sequential FP64 runs of the plain single-instance path: each
+    instance's per-step time measured alone (freestream start, 3 untimed
+    steps, then `reps` timed steps), priced at the steps its FP64 run took."""
+    total_t = 0.0
+    total_work = 0.0
+    for (grid, case), n in zip(cases, steps):
+        s = Solver(grid, case)
+        s.init_freestream()
+        s.step(3)
+        t0 = time.perf_counter()
+        s.step(reps)
+        per_step = (time.perf_counter() - t0) / reps
+        s.close()
+        total_t += per_step * int(n)
+        total_work += grid.num_cells * 3 * int(n)
+    return total_work / total_t, total_t
+
+
+def bench_batch(args):
+    name, plan = W.batch(args.instances)
+    cases = [(i.grid, i.case) for i in plan]
+    nc = W.cells_of(plan)
+    nf = np.array([i.grid.num_faces for i in plan], dtype=np.int64)
+    log(f"{name}: {len(cases)} instances, {int(nc.sum())} cells")
+    runs = {}
+    for prec in (["fp32", "fp64"] if args.precision == "both" else [args.precision]):
+        runs[prec] = run_batch(prec, args, cases, nc)
+        r = runs[prec]
+        log(f"{prec} batch {r['value']:.4g} cell-stage/s, {r['wall']:.2f} s, "
+            f"retired by drop {int((r['r1'] <= args.drop * r['r0']).sum())}")
+    head = "fp32" if "fp32" in runs else "fp64"
+    r = runs[head]
+    algo = algorithmic_bytes(cases[0][1], int(nc.sum()), int(nf.sum()),
+                             s=4 if head == "fp32" else 8)
+    child = ["--no-cpu", "--no-traffic", "--no-seq", "--steps", "4", "--warmup", "0",
+             "--kernel-reps", "3", "--config", "batch", "--instances", str(args.instances),
+             "--block-size", str(args.block_size), "--precision", head]
+    roof = roofline(r["per_launch"], algo, head, args, child)
+    steps = r["steps"]
+    cfg = {"workload": name, "instances": len(cases), "cells": int(nc.sum()),
+           "faces": int(nf.sum()),
+           "precision": "FP32-mixed (FP32 perturbation state)" if head == "fp32" else "FP64 plain",
+           "drop": args.drop, "max_steps": args.steps, "check_every": args.check_every,
+           "block_size": args.block_size, "graph": "two-step hipGraph replay",
+           "steps_min_median_max": [int(steps.min()), int(np.median(steps)), int(steps.max())],
+           "retired_by_drop": int((r["r1"] <= args.drop * r["r0"]).sum())}
+    out = {"metric": METRIC, "value": r["value"], "unit": "cell-stage/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": r["wall"] * 1e3 / max(1, int(steps.max())), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32" if head == "fp32" else "f64",
+           "data": "synthetic", "config": cfg, "roofline": roof}
+    if "fp64" in runs:
+        r64 = runs["fp64"]
+        out["fp64_batched"] = {"value": r64["value"], "wall_s": round(r64["wall"], 3),
+                               "avg_launch_us": {k: round(v, 2) for k, v in r64["per_launch"].items()}}
+        if not args.no_seq:
+            log("sequential FP64 runs of the plain path")
+            seq, seq_t = sequential_fp64(cases, r64["steps"])
+            out["fp64_sequential"] = {"value": seq, "modelled_wall_s": round(seq_t, 3),
+                                      "how": "each instance alone through fvk_ctx (20 timed "
+                                             "steps after 3), times its FP64 step count"}
+            if head == "fp32":
+                out["speedup_vs_fp64_sequential"] = round(r["value"] / seq, 4)
+        if head == "fp32":
+            out["speedup_vs_fp64_batched"] = round(r["value"] / r64["value"], 4)
+            errs, bad32, bad64 = [], 0, 0
+            for i, (g, c) in enumerate(cases):
+                q32, q64 = r["batch"].state(i), r64["batch"].state(i)
+                f32, f64 = bool(np.isfinite(q32).all()), bool(np.isfinite(q64).all())
+                bad32 += not f32
+                bad64 += not f64
+                if f32 and f64:
+                    errs.append(errors_vs_fp64(q32, q64, c.gamma))
+            rho = np.array([e["rho_l2_rel"] for e in errs])
+            pl2 = np.array([e["p_l2_rel"] for e in errs])
+            out["error_vs_fp64"] = {
+                "rho_l2_rel_median": float(f"{np.median(rho):.3e}"),
+                "rho_l2_rel_max": float(f"{rho.max():.3e}"),
+                "p_l2_rel_median": float(f"{np.median(pl2):.3e}"),
+                "p_l2_rel_max": float(f"{pl2.max():.3e}"),
+                "instances_rho_l2_le_1e-5": int((rho <= 1e-5).sum()),
+                "instances_compared": len(errs), "nonfinite_fp32": bad32, "nonfinite_fp64": bad64,
+                "compared": "each instance's final FP32-mixed state vs its final FP64 state"}
+    if not args.no_cpu:
+        # a bounded sample: the smallest, the median and the largest instance
+        order = np.argsort(nc)
+        pick = [order[0], order[len(order) // 2], order[-1]]
+        out["cpu_baseline"] = cpu_baseline([cases[i] for i in pick])
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (single mesh; default 2000) or the step cap of the "
+                         "run to the residual drop (batch; default 1000)")
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--config", default="batch", choices=["batch", *W.SINGLE])
+    ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both",
+                    help="both: FP32-mixed is the measured value, FP64 is timed beside it and "
+                         "is the oracle of the stated error")
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--drop", type=float, default=1e-6)
+    ap.add_argument("--check-every", type=int, default=100)
+    ap.add_argument("--no-seq", action="store_true", help="batch: skip the sequential FP64 runs")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=None,
+                    help="back-to-back launches per kernel for the per-launch timing")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
+    ap.add_argument("--variant", type=int, default=0,
+                    help="single mesh: 0 default, 1 cell-parallel, 2 slot-parallel, 3 LDS-staged")
+    ap.add_argument("--block-size", type=int, default=0, help="cells per mesh block")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run device-to-device copies of MB MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(instances or mesh partitions would shard across GPUs, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    batch = args.config == "batch"
+    if args.steps is None:
+        args.steps = 1000 if batch else 2000
+    if args.warmup is None:
+        args.warmup = 10 if batch else 50
+    if args.kernel_reps is None:
+        args.kernel_reps = 20 if batch else 200
+    out = bench_batch(args) if batch else bench_single(args)
     print(json.dumps(out))
 
 

@@ -63,6 +63,22 @@ __device__ __forceinline__ int logical_block(int per_xcd) {
   return per_xcd > 0 ? (b & 7) * per_xcd + (b >> 3) : b;
 }
 
+// Mesh block and instance of logical block lb: through the active-block list
+// of a batch (retired instances' blocks are not in it), else the identity.
+// False for the padding blocks of the launch.
+template <typename T>
+__device__ __forceinline__ bool block_of(const StageArgsT<T>& a, int lb, int& b, int& inst) {
+  if (a.active) {
+    if (lb >= a.nactive) return false;
+    b = a.active[lb];
+    inst = a.blk_inst[b];
+    return true;
+  }
+  b = lb;
+  inst = 0;
+  return lb < a.m.nblocks;
+}
+
 template <typename T>
 struct Prim {
   T rho, u, v, E, P, c;
@@ -197,11 +213,11 @@ __device__ __forceinline__ void lsq_interior(bool own, const T q[4], double xc, 
 // Pass A, boundary face: ghost-neighbour terms (cell_gradient.cpp:37-73,
 // ghost centroid readmesh.cpp:238-243); the face offers only Q_i to min/max.
 template <typename T>
-__device__ __forceinline__ void lsq_boundary(const StageArgsT<T>& a, int code, const T q[4],
+__device__ __forceinline__ void lsq_boundary(const StageArgsT<T>& a, const FlowT<T>& fl, int code,
+                                             const T q[4],
                                              double xc, double yc, double nx, double ny,
                                              double fx, double fy, T px[4], T py[4], T tmin[4],
                                              T tmax[4]) {
-  const FlowT<T>& fl = a.fl;
   const int v = ~code;
   const int kind = v & 3;
   const double rx = fx - xc, ry = fy - yc;
@@ -267,14 +283,15 @@ __device__ __forceinline__ void lsq_boundary(const StageArgsT<T>& a, int code, c
 // face state the previous res() call left on this cell's side
 // (squeeze.cpp:68-121): Q_prev + grad_prev . (x_f - x_i), or -101 at first.
 template <typename T>
-__device__ __forceinline__ void face_phi(const StageArgsT<T>& a, double fx, double fy, double xc,
+__device__ __forceinline__ void face_phi(const StageArgsT<T>& a, const FlowT<T>& fl, double fx,
+                                         double fy, double xc,
                                          double yc, const T q[4], const T qmin[4],
                                          const T qmax[4], const T qp[4], const T gpx[4],
                                          const T gpy[4], T ph[4]) {
   const T dx = T(fx - xc), dy = T(fy - yc);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const T qf = a.sentinel ? a.fl.sent[k] : qp[k] + gpx[k] * dx + gpy[k] * dy;
+    const T qf = a.sentinel ? fl.sent[k] : qp[k] + gpx[k] * dx + gpy[k] * dy;
     T p = T(1.0);
     if (qf < qmin[k]) p = (qmin[k] - q[k]) / (qf - q[k]);
     else if (qf > qmax[k]) p = (qmax[k] - q[k]) / (qf - q[k]);
@@ -328,11 +345,11 @@ __device__ __forceinline__ void side_flux_delta(const FlowT<T>& fl, const T d[4]
 // out adds F(Q_inf) . sum(n A) over a closed cell, zero up to the rounding of
 // the face normals.
 template <int ORDER, bool NS, typename T>
-__device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const CellQG<T>& c1,
+__device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>& fl,
+                                          const CellQG<T>& c1,
                                           const CellQG<T>* c2, int kind, double nx_d,
                                           double ny_d, double A_d, double fx, double fy, T x[4],
                                           T& sterm) {
-  const FlowT<T>& fl = a.fl;
   const T gamma = fl.gamma, gm1 = fl.gm1;
   const T nx = T(nx_d), ny = T(ny_d), A = T(A_d);
   T L[4], R[4];               // face states (conserved, or perturbations)
@@ -492,14 +509,15 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const CellQG<T
 // The combinations' coefficients sum to one, so they apply unchanged to a
 // perturbation state.
 template <typename T>
-__device__ __forceinline__ void cell_update(const StageArgsT<T>& a, int c, const T q[4], const T qn[4],
-                                            T r[4], T sg, double V_d, T sig_old) {
+__device__ __forceinline__ void cell_update(const StageArgsT<T>& a, const FlowT<T>& fl, int inst,
+                                            int c, const T q[4], const T qn[4], T r[4], T sg,
+                                            double V_d, T sig_old) {
   const int ncp = a.m.ncp;
   const T V = T(V_d);
 #pragma unroll
   for (int k = 0; k < 4; ++k) r[k] = r[k] / V;
-  const T denom = a.dt_global ? T(__longlong_as_double((long long)*a.smax_cur)) : sig_old;
-  const T dt = (a.fl.cfl2 * V) / denom;
+  const T denom = a.dt_global ? T(__longlong_as_double((long long)a.smax_cur[inst])) : sig_old;
+  const T dt = (fl.cfl2 * V) / denom;
   T out[4];
   if (a.stage == 1) {
 #pragma unroll
@@ -525,12 +543,12 @@ __device__ __forceinline__ void cell_update(const StageArgsT<T>& a, int c, const
 
 // cell_update with its own loads (cell-parallel and slot-parallel shapes)
 template <typename T>
-__device__ __forceinline__ void cell_update_ld(const StageArgsT<T>& a, int c, const T q[4], T r[4],
-                                               T sg) {
+__device__ __forceinline__ void cell_update_ld(const StageArgsT<T>& a, const FlowT<T>& fl, int c,
+                                               const T q[4], T r[4], T sg) {
   T qn[4] = {0, 0, 0, 0};
   if (a.stage > 1) load_q(a.qn, a.m.ncp, c, qn);
   const T sig_old = a.dt_global ? T(1) : a.sigma[c];
-  cell_update(a, c, q, qn, r, sg, a.m.vol[c], sig_old);
+  cell_update(a, fl, 0, c, q, qn, r, sg, a.m.vol[c], sig_old);
 }
 
 // gradient from the LSQ sums (cell_gradient.cpp:120-130); the FP32-mixed
@@ -601,6 +619,7 @@ __device__ __forceinline__ void accumulate_lsq(T sx[4], T sy[4], T qmin[4], T qm
 // ---------------------------------------------------------------------------
 template <typename T, bool SQUEEZE>
 __global__ __launch_bounds__(kMaxBlock) void k_pass_a_cell(StageArgsT<T> a) {
+  const FlowT<T>& fl = a.flows[0];   // single-instance shape
   const int c = logical_block(a.xcd_blocks) * blockDim.x + threadIdx.x;
   const DevMesh& m = a.m;
   if (c >= m.ncp) return;
@@ -622,7 +641,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_cell(StageArgsT<T> a) {
       load_q(a.qc, m.ncp, nb, qn);
       lsq_interior(code & 1, q, xc, yc, qn, m.ccx[nb], m.ccy[nb], px, py, tmin, tmax);
     } else {
-      lsq_boundary(a, code, q, xc, yc, m.s_nx[s], m.s_ny[s], m.s_fcx[s], m.s_fcy[s], px, py,
+      lsq_boundary(a, fl, code, q, xc, yc, m.s_nx[s], m.s_ny[s], m.s_fcx[s], m.s_fcy[s], px, py,
                    tmin, tmax);
     }
     accumulate_lsq(sx, sy, qmin, qmax, px, py, tmin, tmax);
@@ -635,7 +654,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_cell(StageArgsT<T> a) {
     for (int j = 0; j < deg; ++j) {
       const int s = base + j * 64;
       T ph[4];
-      face_phi(a, m.s_fcx[s], m.s_fcy[s], xc, yc, q, qmin, qmax, qp, gpx, gpy, ph);
+      face_phi(a, fl, m.s_fcx[s], m.s_fcy[s], xc, yc, q, qmin, qmax, qp, gpx, gpy, ph);
 #pragma unroll
       for (int k = 0; k < 4; ++k) phi[k] = min2(ph[k], phi[k]);
     }
@@ -650,6 +669,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_cell(StageArgsT<T> a) {
 
 template <typename T, int ORDER, bool NS>
 __global__ __launch_bounds__(kMaxBlock) void k_pass_b_cell(StageArgsT<T> a) {
+  const FlowT<T>& fl = a.flows[0];   // single-instance shape
   const int c = logical_block(a.xcd_blocks) * blockDim.x + threadIdx.x;
   const DevMesh& m = a.m;
   double sig_new = 0.0;
@@ -668,16 +688,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_cell(StageArgsT<T> a) {
         CellQG<T> nb;
         load_cell(a, code >> 1, ORDER == 2, nb);
         own = (code & 1) != 0;
-        if (own) face_flux<ORDER, NS>(a, me, &nb, -1, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
-        else face_flux<ORDER, NS>(a, nb, &me, -1, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
+        if (own) face_flux<ORDER, NS>(a, fl, me, &nb, -1, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
+        else face_flux<ORDER, NS>(a, fl, nb, &me, -1, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
       } else {
-        face_flux<ORDER, NS>(a, me, (const CellQG<T>*)nullptr, (~code) & 3, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
+        face_flux<ORDER, NS>(a, fl, me, (const CellQG<T>*)nullptr, (~code) & 3, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) r[k] = r[k] + (own ? x[k] : -x[k]);
       sg += st;
     }
-    cell_update_ld(a, c, me.q, r, sg);
+    cell_update_ld(a, fl, c, me.q, r, sg);
     sig_new = double(sg);
   }
   if (a.stage == 3 && a.dt_global) block_max_to(sig_new, a.smax_next);
@@ -689,6 +709,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_cell(StageArgsT<T> a) {
 // ---------------------------------------------------------------------------
 template <typename T, bool SQUEEZE>
 __global__ __launch_bounds__(kMaxBlock) void k_pass_a_slot(StageArgsT<T> a) {
+  const FlowT<T>& fl = a.flows[0];   // single-instance shape
   __shared__ T t1[kRows][16][64];        // per row: px[4] py[4] tmin[4] tmax[4]
   __shared__ T cv[8][64];                // qmin[4] qmax[4] of each cell
   const DevMesh& m = a.m;
@@ -722,7 +743,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_slot(StageArgsT<T> a) {
         load_q(a.qc, m.ncp, nb, qn);
         lsq_interior(code & 1, q, xc, yc, qn, m.ccx[nb], m.ccy[nb], px, py, tmin, tmax);
       } else {
-        lsq_boundary(a, code, q, xc, yc, m.s_nx[s], m.s_ny[s], m.s_fcx[s], m.s_fcy[s], px, py,
+        lsq_boundary(a, fl, code, q, xc, yc, m.s_nx[s], m.s_ny[s], m.s_fcx[s], m.s_fcy[s], px, py,
                      tmin, tmax);
       }
 #pragma unroll
@@ -775,7 +796,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_slot(StageArgsT<T> a) {
       if (j < deg) {
         const int s = base + j * 64;
         T ph[4];
-        face_phi(a, m.s_fcx[s], m.s_fcy[s], xc, yc, q, qmin, qmax, qp, gpx, gpy, ph);
+        face_phi(a, fl, m.s_fcx[s], m.s_fcy[s], xc, yc, q, qmin, qmax, qp, gpx, gpy, ph);
 #pragma unroll
         for (int k = 0; k < 4; ++k) t1[row][k][lane] = ph[k];
       }
@@ -801,6 +822,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_slot(StageArgsT<T> a) {
 
 template <typename T, int ORDER, bool NS>
 __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
+  const FlowT<T>& fl = a.flows[0];   // single-instance shape
   __shared__ T t1[kRows][5][64];         // per row: signed residual terms[4], sigma term
   const DevMesh& m = a.m;
   const int chunk = logical_block(a.xcd_blocks);
@@ -827,10 +849,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
           CellQG<T> nb;
           load_cell(a, code >> 1, ORDER == 2, nb);
           own = (code & 1) != 0;
-          if (own) face_flux<ORDER, NS>(a, me, &nb, -1, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
-          else face_flux<ORDER, NS>(a, nb, &me, -1, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
+          if (own) face_flux<ORDER, NS>(a, fl, me, &nb, -1, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
+          else face_flux<ORDER, NS>(a, fl, nb, &me, -1, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
         } else {
-          face_flux<ORDER, NS>(a, me, (const CellQG<T>*)nullptr, (~code) & 3, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
+          face_flux<ORDER, NS>(a, fl, me, (const CellQG<T>*)nullptr, (~code) & 3, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) t1[row][k][lane] = own ? x[k] : -x[k];
@@ -848,7 +870,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
       __syncthreads();
     }
     if (row == 0 && active) {
-      cell_update_ld(a, c, me.q, r, sg);
+      cell_update_ld(a, fl, c, me.q, r, sg);
       sig_new = double(sg);
     }
   }
@@ -905,8 +927,10 @@ template <typename T, bool SQUEEZE>
 __global__ __launch_bounds__(kMaxBlock) void k_pass_a_lds(StageArgsT<T> a) {
   extern __shared__ unsigned char lds_raw[];
   const DevMesh& m = a.m;
-  const int b = logical_block(a.xcd_blocks);
-  if (b >= m.nblocks) return;             // uniform over the block
+  int b, inst;
+  if (!block_of(a, logical_block(a.xcd_blocks), b, inst)) return;   // uniform over the block
+  const FlowT<T>& fl = a.flows[inst];
+  const bool squeeze = SQUEEZE && fl.squeeze;
   LdsCells<T, 4> L(lds_raw, m.blk + m.max_halo);
   const int st = L.stride;
   const int i = threadIdx.x;
@@ -926,7 +950,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_lds(StageArgsT<T> a) {
     }
   }
   T qp[4], gpx[4], gpy[4];
-  if (SQUEEZE && deg > 0) load_lagged(a, c, qp, gpx, gpy);
+  if (squeeze && deg > 0) load_lagged(a, c, qp, gpx, gpy);
   double ixx = 0, iyy = 0, ixy = 0;
   if (deg > 0) { ixx = m.ixx[c]; iyy = m.iyy[c]; ixy = m.ixy[c]; }
   stage_block(a, b, L);
@@ -948,7 +972,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_lds(StageArgsT<T> a) {
       lsq_interior(code & 1, q, xc, yc, qn, L.xy[nb], L.xy[st + nb], px, py, tmin, tmax);
     } else {
       const int s = base + j * 64;
-      lsq_boundary(a, code, q, xc, yc, m.s_nx[s], m.s_ny[s], fx, fy, px, py, tmin, tmax);
+      lsq_boundary(a, fl, code, q, xc, yc, m.s_nx[s], m.s_ny[s], fx, fy, px, py, tmin, tmax);
     }
     accumulate_lsq(sx, sy, qmin, qmax, px, py, tmin, tmax);
   };
@@ -961,11 +985,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_lds(StageArgsT<T> a) {
   }
   T gx[4], gy[4];
   lsq_solve(ixx, iyy, ixy, sx, sy, gx, gy);
-  if (SQUEEZE) {
+  if (squeeze) {
     T phi[4] = {1, 1, 1, 1};
     auto lim = [&](double fx, double fy) {
       T ph[4];
-      face_phi(a, fx, fy, xc, yc, q, qmin, qmax, qp, gpx, gpy, ph);
+      face_phi(a, fl, fx, fy, xc, yc, q, qmin, qmax, qp, gpx, gpy, ph);
 #pragma unroll
       for (int k = 0; k < 4; ++k) phi[k] = min2(ph[k], phi[k]);
     };
@@ -1019,9 +1043,11 @@ __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell) void k_pass_b_lds
   extern __shared__ unsigned char lds_raw[];
   constexpr int NQ = ORDER == 2 ? 12 : 4;
   const DevMesh& m = a.m;
-  const int b = logical_block(a.xcd_blocks);
+  int b = 0, inst = 0;
+  const bool live = block_of(a, logical_block(a.xcd_blocks), b, inst);   // uniform
+  const FlowT<T>& fl = a.flows[inst];
   double sig_new = 0.0;
-  if (b < m.nblocks) {                    // uniform over the block
+  if (live) {
     LdsCells<T, NQ> L(lds_raw, m.blk + m.max_halo);
     T* lf = reinterpret_cast<T*>(lds_raw + L.bytes());
     const int fstride = m.max_bfaces;
@@ -1061,9 +1087,9 @@ __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell) void k_pass_b_lds
       T x[4], st;
       if (f.ib >= 0) {
         lds_cell<T, ORDER>(L, f.ib, c2);
-        face_flux<ORDER, NS>(a, c1, &c2, -1, f.nx, f.ny, f.A, f.fx, f.fy, x, st);
+        face_flux<ORDER, NS>(a, fl, c1, &c2, -1, f.nx, f.ny, f.A, f.fx, f.fy, x, st);
       } else {
-        face_flux<ORDER, NS>(a, c1, (const CellQG<T>*)nullptr, (~f.ib) & 3, f.nx, f.ny, f.A, f.fx, f.fy, x, st);
+        face_flux<ORDER, NS>(a, fl, c1, (const CellQG<T>*)nullptr, (~f.ib) & 3, f.nx, f.ny, f.A, f.fx, f.fy, x, st);
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) lf[k * fstride + e] = x[k];
@@ -1100,30 +1126,34 @@ __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell) void k_pass_b_lds
       T q[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) q[k] = L.q[k * L.stride + i];
-      cell_update(a, c, q, qn, r, sg, V, sig_old);
+      cell_update(a, fl, inst, c, q, qn, r, sg, V, sig_old);
       sig_new = double(sg);
     }
   }
-  if (a.stage == 3 && a.dt_global) block_max_to(sig_new, a.smax_next);
-  if (a.stage == 1 && blockIdx.x == 0 && threadIdx.x == 0) *a.smax_next = 0ull;
+  if (live && a.stage == 3 && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
+  // the instance's first block clears the maximum stage 3 will build
+  if (live && a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) a.smax_next[inst] = 0ull;
 }
 
 // Spectral sums of the first step: every face carries |V_inf| and c_inf
 // (class_q.hpp:61-62), accumulated as ssp_rk3.cpp:28-36 does.
+// One thread block per mesh block (blockDim = blk).
 template <typename T>
 __global__ __launch_bounds__(kMaxBlock) void k_init_sigma(StageArgsT<T> a) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const DevMesh& m = a.m;
+  const int b = blockIdx.x;
+  const int inst = a.blk_inst ? a.blk_inst[b] : 0;
+  const FlowT<T>& fl = a.flows[inst];
+  const int c = b * m.blk + threadIdx.x;
+  const int deg = m.deg[c];
+  const int base = m.chunk_off[c >> 6] + (c & 63);
   T sg = 0;
-  if (c < m.ncp) {
-    const int deg = m.deg[c];
-    const int base = m.chunk_off[c >> 6] + (c & 63);
-    for (int j = 0; j < deg; ++j) sg += (fabs(a.fl.vn0) + a.fl.c0) * T(m.s_area[base + j * 64]);
-    a.sigma[c] = sg;
-  }
-  block_max_to(double(sg), a.smax_cur);
+  for (int j = 0; j < deg; ++j) sg += (fabs(fl.vn0) + fl.c0) * T(m.s_area[base + j * 64]);
+  a.sigma[c] = sg;
+  block_max_to(double(sg), a.smax_cur + inst);
 }
 
+// One thread block per mesh block (blockDim = blk): partial[k*nblocks + b].
 template <typename T>
 __global__ __launch_bounds__(kMaxBlock) void k_res_sq(const T* __restrict__ res, int ncp,
                                                       double* __restrict__ partial) {
@@ -1132,7 +1162,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_res_sq(const T* __restrict__ res,
   double s[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const double r = (c < ncp) ? double(res[k * ncp + c]) : 0.0;
+    const double r = double(res[k * ncp + c]);
     s[k] = r * r;
   }
 #pragma unroll
@@ -1187,7 +1217,8 @@ hipError_t launch_pass_a(const StageArgsT<T>& a0, int squeeze, int block, hipStr
   StageArgsT<T> a = a0;
   int grid;
   if (a.variant == kVariantLds) {
-    shape(a, a.m.nblocks, &grid, &a.xcd_blocks);
+    shape(a, a.active ? a.nactive : a.m.nblocks, &grid, &a.xcd_blocks);
+    if (grid == 0) return hipSuccess;
     const size_t lds = lds_bytes_pass_a(a.m, sizeof(T));
     if (squeeze) hipLaunchKernelGGL((k_pass_a_lds<T, true>), dim3(grid), dim3(a.m.blk), lds, s, a);
     else hipLaunchKernelGGL((k_pass_a_lds<T, false>), dim3(grid), dim3(a.m.blk), lds, s, a);
@@ -1208,7 +1239,8 @@ hipError_t launch_pass_b(const StageArgsT<T>& a0, int order, int ns, int block, 
   StageArgsT<T> a = a0;
   int grid;
   if (a.variant == kVariantLds) {
-    shape(a, a.m.nblocks, &grid, &a.xcd_blocks);
+    shape(a, a.active ? a.nactive : a.m.nblocks, &grid, &a.xcd_blocks);
+    if (grid == 0) return hipSuccess;
     const size_t lds = lds_bytes_pass_b(a.m, order, a.stage, sizeof(T));
     const dim3 blk(a.m.blk * kPassBThreadsPerCell);
     if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false>), dim3(grid), blk, lds, s, a);
@@ -1230,15 +1262,14 @@ hipError_t launch_pass_b(const StageArgsT<T>& a0, int order, int ns, int block, 
 }
 
 template <typename T>
-hipError_t launch_init_sigma(const StageArgsT<T>& a, int block, hipStream_t s) {
-  hipLaunchKernelGGL(k_init_sigma<T>, dim3((a.m.ncp + block - 1) / block), dim3(block), 0, s, a);
+hipError_t launch_init_sigma(const StageArgsT<T>& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_init_sigma<T>, dim3(a.m.nblocks), dim3(a.m.blk), 0, s, a);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_res_sq(const T* res, int32_t ncp, double* partial, int nblocks, int block,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_res_sq<T>, dim3(nblocks), dim3(block), 0, s, res, ncp, partial);
+hipError_t launch_res_sq(const T* res, int32_t ncp, int32_t blk, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_res_sq<T>, dim3(ncp / blk), dim3(blk), 0, s, res, ncp, partial);
   return hipGetLastError();
 }
 
@@ -1254,8 +1285,8 @@ hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s) {
 #define FVK_INSTANTIATE(T)                                                                      \
   template hipError_t launch_pass_a<T>(const StageArgsT<T>&, int, int, hipStream_t);            \
   template hipError_t launch_pass_b<T>(const StageArgsT<T>&, int, int, int, hipStream_t);       \
-  template hipError_t launch_init_sigma<T>(const StageArgsT<T>&, int, hipStream_t);             \
-  template hipError_t launch_res_sq<T>(const T*, int32_t, double*, int, int, hipStream_t);      \
+  template hipError_t launch_init_sigma<T>(const StageArgsT<T>&, hipStream_t);                  \
+  template hipError_t launch_res_sq<T>(const T*, int32_t, int32_t, double*, hipStream_t);       \
   template hipError_t launch_fill<T>(T*, int64_t, T, hipStream_t);
 FVK_INSTANTIATE(double)
 FVK_INSTANTIATE(float)

@@ -50,7 +50,9 @@ struct DevMesh {
   const double* bf_fcy;
 };
 
-// Gas and freestream constants in the arithmetic type (class_flow, class_Q).
+// Per-instance constants: gas and freestream in the arithmetic type
+// (class_flow, class_Q), the limiter switch and the instance's first block.
+// A single solve is a batch of one instance.
 template <typename T>
 struct FlowT {
   T gamma, gm1, Rgas, mu, k;
@@ -59,13 +61,21 @@ struct FlowT {
   T rho, u, v, P;              // freestream primitives
   T vn0, c0;                   // initial face |Vn_avg| and c_avg (class_q.hpp:61-62)
   T sent[4];                   // the -101 initial face state, in stored-state units
+  int32_t squeeze;             // limiter SQUEEZE (1) or none (0)
+  int32_t first_block;         // first mesh block of the instance
 };
 
 // One RK stage.  State arrays are SoA blocks of ncp values: q[k*ncp + cell].
+// A batch concatenates its instances' padded cell ranges and block lists;
+// per-block instance ids select the instance's constants and dt maximum.
 template <typename T>
 struct StageArgsT {
   DevMesh m;
-  FlowT<T> fl;
+  const FlowT<T>* flows;       // per instance
+  const int32_t* blk_inst;     // per mesh block: instance (batch only, else null)
+  const int32_t* active;       // logical block -> mesh block (batch only, else null)
+  int32_t nactive;             // active blocks (batch only)
+  int32_t pad_;
   int32_t stage;               // 1, 2, 3
   int32_t dt_global;           // time_step_type GLOBAL
   int32_t sentinel;            // lagged face states are the -101 initial values
@@ -81,8 +91,9 @@ struct StageArgsT {
   const T* qn;                 // state at the start of the step
   T* qout;                     // stage output state
   T* sigma;                    // per-cell spectral sum of the last stage 3
-  unsigned long long* smax_cur;   // max spectral sum used by this step (GLOBAL), FP64 bits
-  unsigned long long* smax_next;  // max spectral sum written by stage 3
+  unsigned long long* smax_cur;   // per instance: max spectral sum used by this step
+                                  // (GLOBAL dt), FP64 bits
+  unsigned long long* smax_next;  // per instance: max spectral sum written by stage 3
   T* res;                      // residual (4 blocks)
 };
 
@@ -98,12 +109,12 @@ template <typename T>
 hipError_t launch_pass_a(const StageArgsT<T>& a, int squeeze, int block, hipStream_t s);
 template <typename T>
 hipError_t launch_pass_b(const StageArgsT<T>& a, int order, int ns, int block, hipStream_t s);
+// first-step spectral sums and their per-instance maxima
 template <typename T>
-hipError_t launch_init_sigma(const StageArgsT<T>& a, int block, hipStream_t s);
-// per-block partial sums of R_k^2 over all ncp cells: out[k*nblocks + b]
+hipError_t launch_init_sigma(const StageArgsT<T>& a, hipStream_t s);
+// per-mesh-block partial sums of R_k^2: partial[k*nblocks + b]
 template <typename T>
-hipError_t launch_res_sq(const T* res, int32_t ncp, double* partial, int nblocks, int block,
-                         hipStream_t s);
+hipError_t launch_res_sq(const T* res, int32_t ncp, int32_t blk, double* partial, hipStream_t s);
 template <typename T>
 hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s);
 

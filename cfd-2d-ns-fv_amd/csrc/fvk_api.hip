@@ -1,12 +1,25 @@
-// fvk_api.hip — the C ABI (include/fvk.h): context, device memory, the SSP-RK3
-// driver (ssp_rk3.cpp:18-77 as a sequence of kernel launches on one stream).
+// fvk_api.hip — the C ABI (include/fvk.h): contexts, device memory, and the
+// SSP-RK3 driver (ssp_rk3.cpp:18-77 as a sequence of kernel launches on one
+// stream).
+//
+// One engine serves both entry-point families.  A single solve (fvk_ctx) is an
+// engine with one instance; a batch (fvk_batch) concatenates its instances'
+// device layouts: each instance's padded cell range, block list, slot table,
+// halo lists and block face lists are laid end to end, with the global cell
+// indices in them (neighbour codes, boundary ghost sources, halo cells)
+// offset by the instance's first cell.  Block-local data need no change.  The
+// kernels find a block's instance in a per-block table and take that
+// instance's constants (freestream, CFL, limiter switch) and dt maximum from
+// per-instance arrays, so one launch advances every instance by one stage.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fv_kernels.hpp"
@@ -17,58 +30,83 @@ namespace {
 
 thread_local std::string g_create_error;
 
-}  // namespace
-
-struct fvk_ctx {
-  std::string err;
+struct Instance {
+  fvk::HostMesh hm;             // the instance's own layout (cells numbered from 0)
   fvk_options opt{};
-  fvk::HostMesh hm;
+  double qinf[4] = {0, 0, 0, 0};
+  int32_t cell_off = 0;         // first padded cell in the engine's arrays
+  int32_t block_off = 0;        // first mesh block
+  int32_t slot_off = 0;         // first face slot
+  int32_t halo_off = 0;         // first halo-list entry
+  int32_t bface_off = 0;        // first block-face entry
+  bool active = true;
+  int64_t steps = 0;            // steps run since the last (re)start
+};
+
+struct Engine {
+  std::string err;
+  fvk_options opt{};            // instance 0's options (the kernel-shaping ones are common)
+  std::vector<Instance> in;
+  bool batched = false;
   hipStream_t stream = nullptr;
-  std::vector<void*> bufs;   // device allocations, freed by fvk_destroy
+  std::vector<void*> bufs;      // device allocations, freed by engine_free
   fvk::DevMesh dm{};
-  fvk::FlowT<double> fl64{};
-  fvk::FlowT<float> fl32{};
-  double qinf[4] = {0, 0, 0, 0};                // freestream conserved state (FP64)
-  bool fp32 = false;                            // FP32-mixed: state = float(Q - Q_inf)
-  size_t tsize = 8;                             // bytes per state value
+  bool fp32 = false;            // FP32-mixed: state = float(Q - Q_inf)
+  size_t tsize = 8;             // bytes per state value
+  void* flows = nullptr;        // FlowT<T>[n]
+  int32_t* blk_inst = nullptr;  // batch: per-block instance
+  int32_t* active = nullptr;    // batch: active block list
+  int32_t nactive = 0;
   // state arrays of the arithmetic type (double, or float perturbations)
-  void* q[3] = {nullptr, nullptr, nullptr};     // Qn, Q1, Q2 (4*ncp each)
-  void* g[3] = {nullptr, nullptr, nullptr};     // stage 1..3 limited gradients (8*ncp each)
-  void* sigma = nullptr;                        // ncp
-  void* res = nullptr;                          // 4*ncp
-  unsigned long long* smax = nullptr;           // [2], FP64 bits
-  double* partial = nullptr;
+  void* q[3] = {nullptr, nullptr, nullptr};   // Qn, Q1, Q2 (4*ncp each)
+  void* g[3] = {nullptr, nullptr, nullptr};   // stage 1..3 limited gradients (8*ncp each)
+  void* sigma = nullptr;                      // ncp
+  void* res = nullptr;                        // 4*ncp
+  unsigned long long* smax = nullptr;         // [2][n], FP64 bits
+  double* partial = nullptr;                  // [4][nblocks]
   int block = 256;
   int variant = fvk::kVariantLds;
-  int res_blocks = 0;
   int smax_cur = 0;
   bool have_state = false;
   bool sentinel = true;
   bool fs_prims = false;
   bool res_valid = false;
+  bool use_graph = true;
+  hipGraphExec_t graph = nullptr;             // two steps, captured for `graph_nactive`
+  int32_t graph_nactive = -1;
+};
+
+}  // namespace
+
+struct fvk_ctx {
+  Engine e;
+};
+
+struct fvk_batch {
+  Engine e;
 };
 
 namespace {
 
-int set_err(fvk_ctx* c, int code, const std::string& msg) {
+int set_err(Engine* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   else g_create_error = msg;
   return code;
 }
 
-int hip_err(fvk_ctx* c, hipError_t e, const char* what) {
+int hip_err(Engine* c, hipError_t e, const char* what) {
   std::string m = std::string(what) + ": " + hipGetErrorString(e);
   return set_err(c, FVK_ERR_HIP, m);
 }
 
-#define FVK_HIP(ctx, call)                         \
-  do {                                             \
-    hipError_t e_ = (call);                        \
+#define FVK_HIP(ctx, call)                                \
+  do {                                                    \
+    hipError_t e_ = (call);                               \
     if (e_ != hipSuccess) return hip_err(ctx, e_, #call); \
   } while (0)
 
 template <typename T>
-int dev_alloc(fvk_ctx* c, T** out, size_t count) {
+int dev_alloc(Engine* c, T** out, size_t count) {
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, count * sizeof(T) + 256);
   if (e != hipSuccess) return hip_err(c, e, "hipMalloc");
@@ -77,18 +115,47 @@ int dev_alloc(fvk_ctx* c, T** out, size_t count) {
   return FVK_OK;
 }
 
-template <typename T>
-int upload(fvk_ctx* c, const std::vector<T>& v, const T** out) {
+// Concatenate one HostMesh field over the instances into a new device array.
+// `adjust(instance, x)` rewrites a value (global indices get the instance's
+// offsets); prefix arrays (`prefix`) drop every instance's last entry except
+// the last instance's.
+template <typename T, typename Adjust>
+int upload_cat(Engine* c, std::vector<T> fvk::HostMesh::*field, const T** out, Adjust adjust,
+               bool prefix = false) {
+  const size_t n = c->in.size();
+  auto count = [&](size_t i) {
+    const size_t sz = (c->in[i].hm.*field).size();
+    return (prefix && i + 1 < n) ? sz - 1 : sz;
+  };
+  size_t total = 0;
+  for (size_t i = 0; i < n; ++i) total += count(i);
   T* d = nullptr;
-  int rc = dev_alloc(c, &d, v.size());
+  int rc = dev_alloc(c, &d, total);
   if (rc) return rc;
-  if (!v.empty()) {
-    hipError_t e = hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return hip_err(c, e, "hipMemcpy H2D");
+  size_t off = 0;
+  std::vector<T> tmp;
+  for (size_t i = 0; i < n; ++i) {
+    const std::vector<T>& v = c->in[i].hm.*field;
+    const size_t k = count(i);
+    tmp.assign(v.begin(), v.begin() + k);
+    for (T& x : tmp) x = adjust(c->in[i], x);
+    if (k) {
+      hipError_t e = hipMemcpy(d + off, tmp.data(), k * sizeof(T), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return hip_err(c, e, "hipMemcpy H2D");
+    }
+    off += k;
   }
   *out = d;
   return FVK_OK;
 }
+
+template <typename T>
+int upload_cat(Engine* c, std::vector<T> fvk::HostMesh::*field, const T** out) {
+  return upload_cat(c, field, out, [](const Instance&, T x) { return x; });
+}
+
+// boundary code ~((src << 2) | kind) with the ghost source cell moved by `off`
+inline int32_t shift_boundary(int32_t code, int32_t off) { return ~((~code) + (off << 2)); }
 
 int check_options(const fvk_options& o, std::string* msg) {
   if (o.order != 1 && o.order != 2) { *msg = "order must be 1 or 2 (solution_poly_order 0 or 1)"; return FVK_ERR_OPTION; }
@@ -110,22 +177,82 @@ int check_options(const fvk_options& o, std::string* msg) {
   return FVK_OK;
 }
 
-template <typename T> const fvk::FlowT<T>& flow(const fvk_ctx* c);
-template <> const fvk::FlowT<double>& flow<double>(const fvk_ctx* c) { return c->fl64; }
-template <> const fvk::FlowT<float>& flow<float>(const fvk_ctx* c) { return c->fl32; }
+// The options that select kernels or the launch shape must agree across the
+// instances of a batch; the flow conditions, gas constants, CFL, limiter
+// switch, BC kinds and renumbering may differ.
+int check_common(const fvk_options& a, const fvk_options& b, std::string* msg) {
+  const bool same = a.order == b.order && a.eqn == b.eqn && a.time_step_type == b.time_step_type &&
+                    a.precision == b.precision && a.block_size == b.block_size &&
+                    a.kernel_variant == b.kernel_variant && a.xcd_remap == b.xcd_remap;
+  if (!same) {
+    *msg = "batch instances must agree on order, eqn, time_step_type, precision, block_size, "
+           "kernel_variant and xcd_remap";
+    return FVK_ERR_OPTION;
+  }
+  if (a.kernel_variant == fvk::kVariantCell || a.kernel_variant == fvk::kVariantSlot) {
+    *msg = "batches run the LDS-staged kernels only (kernel_variant 0 or 3)";
+    return FVK_ERR_OPTION;
+  }
+  return FVK_OK;
+}
+
+// class_flow (class_flow.hpp:28-46) and class_Q constants in type T
+template <typename T>
+fvk::FlowT<T> make_flow(const fvk_options& o, bool delta, int32_t first_block) {
+  fvk::FlowT<T> fl{};
+  fl.gamma = T(o.gamma);
+  fl.gm1 = T(o.gamma - 1);
+  fl.Rgas = T(o.R);
+  fl.mu = T(o.mu);
+  const double Cp = o.R * o.gamma / (o.gamma - 1);
+  fl.k = T((o.mu * Cp) / o.Pr);
+  fl.cfl2 = T(2 * o.cfl);
+  const double qinf[4] = {o.rho, o.rho * o.u, o.rho * o.v,
+                          (o.P / (o.gamma - 1)) + 0.5 * o.rho * (o.u * o.u + o.v * o.v)};
+  for (int k = 0; k < 4; ++k) {
+    fl.qinf[k] = T(qinf[k]);
+    // the reference's -101 initial face states (class_q.hpp:59-60)
+    fl.sent[k] = T(delta ? -101.0 - qinf[k] : -101.0);
+  }
+  fl.rho = T(o.rho);
+  fl.u = T(o.u);
+  fl.v = T(o.v);
+  fl.P = T(o.P);
+  fl.vn0 = T(std::sqrt(o.v * o.v + o.u * o.u));
+  fl.c0 = T(std::sqrt(o.gamma * (o.P / o.rho)));
+  fl.squeeze = o.limiter == 2;
+  fl.first_block = first_block;
+  return fl;
+}
 
 template <typename T>
-fvk::StageArgsT<T> base_args(fvk_ctx* c) {
+int upload_flows(Engine* c) {
+  std::vector<fvk::FlowT<T>> h;
+  for (const Instance& i : c->in) h.push_back(make_flow<T>(i.opt, c->fp32, i.block_off));
+  fvk::FlowT<T>* d = nullptr;
+  int rc = dev_alloc(c, &d, h.size());
+  if (rc) return rc;
+  FVK_HIP(c, hipMemcpy(d, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
+  c->flows = d;
+  return FVK_OK;
+}
+
+template <typename T>
+fvk::StageArgsT<T> base_args(Engine* c) {
   fvk::StageArgsT<T> a{};
   a.m = c->dm;
-  a.fl = flow<T>(c);
+  a.flows = static_cast<const fvk::FlowT<T>*>(c->flows);
+  a.blk_inst = c->blk_inst;
+  a.active = c->active;
+  a.nactive = c->nactive;
   a.dt_global = c->opt.time_step_type == 1;
   a.xcd_remap = c->opt.xcd_remap;
   a.variant = c->variant;
   a.sigma = static_cast<T*>(c->sigma);
   a.res = static_cast<T*>(c->res);
-  a.smax_cur = c->smax + c->smax_cur;
-  a.smax_next = c->smax + (1 - c->smax_cur);
+  const size_t n = c->in.size();
+  a.smax_cur = c->smax + c->smax_cur * n;
+  a.smax_next = c->smax + (1 - c->smax_cur) * n;
   return a;
 }
 
@@ -133,7 +260,7 @@ fvk::StageArgsT<T> base_args(fvk_ctx* c) {
 // stage s reads Q_{s-1} and the previous stage's Q and limited gradient (the
 // lagged limiter), writes G_s and Q_s; stage 3 writes Q_n in place
 template <typename T>
-void set_roles(fvk_ctx* c, int stage, fvk::StageArgsT<T>& a) {
+void set_roles(Engine* c, int stage, fvk::StageArgsT<T>& a) {
   T* const* q = reinterpret_cast<T* const*>(c->q);
   T* const* g = reinterpret_cast<T* const*>(c->g);
   a.qn = q[0];
@@ -142,16 +269,38 @@ void set_roles(fvk_ctx* c, int stage, fvk::StageArgsT<T>& a) {
   if (stage == 3) { a.qc = q[2]; a.qp = q[1]; a.gp = g[1]; a.g = g[2]; a.qout = q[0]; }
 }
 
+template <typename T>
+int fill_t(Engine* c, void* p, int64_t n, double v) {
+  hipError_t e = fvk::launch_fill(static_cast<T*>(p), n, T(v), c->stream);
+  return e == hipSuccess ? FVK_OK : hip_err(c, e, "fill");
+}
+
+int fill(Engine* c, void* p, int64_t n, double v) {
+  return c->fp32 ? fill_t<float>(c, p, n, v) : fill_t<double>(c, p, n, v);
+}
+
+void* elem(Engine* c, void* base, size_t index) {
+  return static_cast<unsigned char*>(base) + index * c->tsize;
+}
+
+bool any_squeeze(const Engine* c) {
+  for (const Instance& i : c->in)
+    if (i.opt.limiter == 2) return true;
+  return false;
+}
+
 // Spectral sums |V_inf| + c_inf on every face and their max: the lagged face
 // data of a fresh start (class_q.hpp:61-62, main.cpp:117-118).
-int arm_start(fvk_ctx* c) {
+int arm_start(Engine* c) {
   c->smax_cur = 0;
-  FVK_HIP(c, fvk::launch_fill(reinterpret_cast<double*>(c->smax), 2, 0.0, c->stream));
-  if (c->fp32) FVK_HIP(c, fvk::launch_init_sigma(base_args<float>(c), c->block, c->stream));
-  else FVK_HIP(c, fvk::launch_init_sigma(base_args<double>(c), c->block, c->stream));
+  FVK_HIP(c, fvk::launch_fill(reinterpret_cast<double*>(c->smax), 2 * (int64_t)c->in.size(), 0.0,
+                              c->stream));
+  if (c->fp32) FVK_HIP(c, fvk::launch_init_sigma(base_args<float>(c), c->stream));
+  else FVK_HIP(c, fvk::launch_init_sigma(base_args<double>(c), c->stream));
   c->sentinel = true;
   c->res_valid = false;
   c->have_state = true;
+  for (Instance& i : c->in) i.steps = 0;
   FVK_HIP(c, hipStreamSynchronize(c->stream));
   return FVK_OK;
 }
@@ -160,8 +309,10 @@ enum { kKindA = 1, kKindB = 2 };
 
 // One ssp_rk3 call.  launch(kind, fn) lets fvk_time_steps bracket launches.
 template <typename T, typename Launch>
-int run_step_t(fvk_ctx* c, bool store_res, Launch&& launch) {
+int run_step_t(Engine* c, bool store_res, Launch&& launch) {
   const fvk_options& o = c->opt;
+  // the SQUEEZE pass A when any instance limits; per-instance switch inside
+  const bool squeeze = any_squeeze(c);
   for (int stage = 1; stage <= 3; ++stage) {
     fvk::StageArgsT<T> a = base_args<T>(c);
     a.stage = stage;
@@ -170,7 +321,7 @@ int run_step_t(fvk_ctx* c, bool store_res, Launch&& launch) {
     a.store_res = (stage == 3 && store_res) ? 1 : 0;
     set_roles(c, stage, a);
     if (o.order == 2) {
-      int rc = launch(kKindA, [&]() { return fvk::launch_pass_a(a, o.limiter == 2, c->block, c->stream); });
+      int rc = launch(kKindA, [&]() { return fvk::launch_pass_a(a, squeeze, c->block, c->stream); });
       if (rc) return rc;
     }
     int rc = launch(kKindB, [&]() { return fvk::launch_pass_b(a, o.order, o.eqn == 2, c->block, c->stream); });
@@ -184,67 +335,214 @@ int run_step_t(fvk_ctx* c, bool store_res, Launch&& launch) {
 }
 
 template <typename Launch>
-int run_step(fvk_ctx* c, bool store_res, Launch&& launch) {
+int run_step(Engine* c, bool store_res, Launch&& launch) {
   return c->fp32 ? run_step_t<float>(c, store_res, launch) : run_step_t<double>(c, store_res, launch);
 }
 
-// device array of n values of the arithmetic type -> doubles
-int download(fvk_ctx* c, const void* src, size_t n, std::vector<double>* out) {
-  out->resize(n);
-  if (!c->fp32) {
-    FVK_HIP(c, hipMemcpy(out->data(), src, n * 8, hipMemcpyDeviceToHost));
-    return FVK_OK;
-  }
-  std::vector<float> f(n);
-  FVK_HIP(c, hipMemcpy(f.data(), src, n * 4, hipMemcpyDeviceToHost));
-  for (size_t i = 0; i < n; ++i) (*out)[i] = f[i];
+int plain_step(Engine* c, bool store_res) {
+  auto plain = [c](int, auto&& fn) -> int {
+    hipError_t e = fn();
+    return e == hipSuccess ? FVK_OK : hip_err(c, e, "kernel launch");
+  };
+  int rc = run_step(c, store_res, plain);
+  if (rc) return rc;
+  for (Instance& i : c->in)
+    if (i.active) ++i.steps;
   return FVK_OK;
 }
 
-template <typename T>
-int fill_t(fvk_ctx* c, void* p, int64_t n, double v) {
-  hipError_t e = fvk::launch_fill(static_cast<T*>(p), n, T(v), c->stream);
-  return e == hipSuccess ? FVK_OK : hip_err(c, e, "fill");
+void drop_graph(Engine* c) {
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  c->graph = nullptr;
+  c->graph_nactive = -1;
 }
 
-int fill(fvk_ctx* c, void* p, int64_t n, double v) {
-  return c->fp32 ? fill_t<float>(c, p, n, v) : fill_t<double>(c, p, n, v);
-}
-
-// class_flow (class_flow.hpp:28-46) and class_Q constants in type T
-template <typename T>
-void make_flow(const fvk_options& o, fvk::FlowT<T>* fl, bool delta) {
-  fl->gamma = T(o.gamma);
-  fl->gm1 = T(o.gamma - 1);
-  fl->Rgas = T(o.R);
-  fl->mu = T(o.mu);
-  const double Cp = o.R * o.gamma / (o.gamma - 1);
-  fl->k = T((o.mu * Cp) / o.Pr);
-  fl->cfl2 = T(2 * o.cfl);
-  const double qinf[4] = {o.rho, o.rho * o.u, o.rho * o.v,
-                          (o.P / (o.gamma - 1)) + 0.5 * o.rho * (o.u * o.u + o.v * o.v)};
-  for (int k = 0; k < 4; ++k) {
-    fl->qinf[k] = T(qinf[k]);
-    // the reference's -101 initial face states (class_q.hpp:59-60)
-    fl->sent[k] = T(delta ? -101.0 - qinf[k] : -101.0);
+// Capture two steps (both dt-maximum parities) of a batch into a graph.  The
+// launches read everything that changes from device memory, so the graph
+// stays valid until the active block list changes.
+int capture_graph(Engine* c) {
+  drop_graph(c);
+  hipGraph_t g = nullptr;
+  FVK_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  auto plain = [](int, auto&& fn) -> int { return fn() == hipSuccess ? FVK_OK : FVK_ERR_HIP; };
+  int rc = run_step(c, true, plain);
+  if (!rc) rc = run_step(c, true, plain);
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return set_err(c, FVK_ERR_HIP, "kernel launch during graph capture failed");
   }
-  fl->rho = T(o.rho);
-  fl->u = T(o.u);
-  fl->v = T(o.v);
-  fl->P = T(o.P);
-  fl->vn0 = T(std::sqrt(o.v * o.v + o.u * o.u));
-  fl->c0 = T(std::sqrt(o.gamma * (o.P / o.rho)));
+  if (e != hipSuccess) return hip_err(c, e, "hipStreamEndCapture");
+  e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->graph = nullptr;
+    return hip_err(c, e, "hipGraphInstantiate");
+  }
+  c->graph_nactive = c->nactive;
+  return FVK_OK;
+}
+
+// nsteps steps of every active instance.  A batch replays its two-step graph
+// wherever a step is not the first after a start and starts on parity 0.
+int engine_step(Engine* c, int32_t nsteps) {
+  if (!c->have_state) return set_err(c, FVK_ERR_ARG, "step: no state (initialise it first)");
+  if (nsteps < 0) return set_err(c, FVK_ERR_ARG, "step: negative step count");
+  if (c->batched && c->nactive == 0) return FVK_OK;   // every instance retired
+  int32_t left = nsteps;
+  while (left > 0) {
+    const bool graphable = c->batched && c->use_graph && !c->sentinel && !c->fs_prims &&
+                           c->smax_cur == 0 && left >= 2;
+    if (!graphable) {
+      // a batch always keeps the residual (per-instance convergence checks)
+      int rc = plain_step(c, c->batched || left == 1);
+      if (rc) return rc;
+      --left;
+      continue;
+    }
+    if (!c->graph || c->graph_nactive != c->nactive) {
+      int rc = capture_graph(c);
+      if (rc) return rc;
+    }
+    FVK_HIP(c, hipGraphLaunch(c->graph, c->stream));
+    for (Instance& i : c->in)
+      if (i.active) i.steps += 2;
+    c->res_valid = true;
+    left -= 2;
+  }
+  FVK_HIP(c, hipStreamSynchronize(c->stream));
+  return FVK_OK;
+}
+
+// instance i's rows of a [nrows][ncp] device array of the arithmetic type,
+// as doubles [nrows][ncp_i] in the instance's padded numbering
+int download_rows(Engine* c, const void* src, int nrows, size_t i, std::vector<double>* out) {
+  const Instance& in = c->in[i];
+  const size_t ncp = c->dm.ncp, n = in.hm.ncp;
+  out->resize(nrows * n);
+  std::vector<float> f(c->fp32 ? n : 0);
+  for (int k = 0; k < nrows; ++k) {
+    const void* p = static_cast<const unsigned char*>(src) + (k * ncp + in.cell_off) * c->tsize;
+    if (c->fp32) {
+      FVK_HIP(c, hipMemcpy(f.data(), p, n * 4, hipMemcpyDeviceToHost));
+      for (size_t j = 0; j < n; ++j) (*out)[k * n + j] = f[j];
+    } else {
+      FVK_HIP(c, hipMemcpy(out->data() + k * n, p, n * 8, hipMemcpyDeviceToHost));
+    }
+  }
+  return FVK_OK;
+}
+
+// unpermute [nrows][ncp_i] into the caller's [nrows][nc] (grid-file order),
+// adding add[k] to row k when given
+void unpermute(const fvk::HostMesh& h, const std::vector<double>& buf, int nrows, double* out,
+               const double* add = nullptr) {
+  const size_t n = h.ncp;
+  for (int k = 0; k < nrows; ++k)
+    for (int32_t j = 0; j < h.ncp; ++j)
+      if (h.perm[j] >= 0)
+        out[k * (size_t)h.nc + h.perm[j]] = add ? add[k] + buf[k * n + j] : buf[k * n + j];
+}
+
+int get_state(Engine* c, size_t i, double* q, double* grad) {
+  if (!q && !grad) return set_err(c, FVK_ERR_ARG, "get_state: nothing requested");
+  const Instance& in = c->in[i];
+  std::vector<double> buf;
+  if (q) {
+    int rc = download_rows(c, c->q[0], 4, i, &buf);
+    if (rc) return rc;
+    unpermute(in.hm, buf, 4, q, c->fp32 ? in.qinf : nullptr);
+  }
+  if (grad) {
+    int rc = download_rows(c, c->g[0], 8, i, &buf);
+    if (rc) return rc;
+    unpermute(in.hm, buf, 8, grad);
+  }
+  return FVK_OK;
+}
+
+// store instance i's conserved state q[4*nc] (grid-file order), or its
+// freestream when q is NULL
+int put_state(Engine* c, size_t i, const double* q) {
+  const Instance& in = c->in[i];
+  const fvk::HostMesh& h = in.hm;
+  const size_t n = h.ncp, ncp = c->dm.ncp;
+  for (int k = 0; k < 4; ++k) {
+    void* dst = elem(c, c->q[0], k * ncp + in.cell_off);
+    if (!q) {
+      // FP64: Q_inf; FP32-mixed: the perturbation 0
+      int rc = fill(c, dst, (int64_t)n, c->fp32 ? 0.0 : in.qinf[k]);
+      if (rc) return rc;
+      continue;
+    }
+    if (c->fp32) {
+      std::vector<float> buf(n, 0.0f);
+      for (int32_t j = 0; j < h.ncp; ++j)
+        if (h.perm[j] >= 0) buf[j] = float(q[k * (size_t)h.nc + h.perm[j]] - in.qinf[k]);
+      FVK_HIP(c, hipMemcpy(dst, buf.data(), n * 4, hipMemcpyHostToDevice));
+    } else {
+      std::vector<double> buf(n, 1.0);
+      for (int32_t j = 0; j < h.ncp; ++j)
+        if (h.perm[j] >= 0) buf[j] = q[k * (size_t)h.nc + h.perm[j]];
+      FVK_HIP(c, hipMemcpy(dst, buf.data(), n * 8, hipMemcpyHostToDevice));
+    }
+  }
+  return FVK_OK;
+}
+
+// sqrt(sum R_k^2) of every instance's last stored residual: out[4*i + k]
+int residual_l2(Engine* c, double* out) {
+  const int32_t nb = c->dm.nblocks;
+  if (c->fp32)
+    FVK_HIP(c, fvk::launch_res_sq(static_cast<const float*>(c->res), c->dm.ncp, c->dm.blk, c->partial, c->stream));
+  else
+    FVK_HIP(c, fvk::launch_res_sq(static_cast<const double*>(c->res), c->dm.ncp, c->dm.blk, c->partial, c->stream));
+  std::vector<double> part(4 * (size_t)nb);
+  FVK_HIP(c, hipMemcpyAsync(part.data(), c->partial, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  FVK_HIP(c, hipStreamSynchronize(c->stream));
+  bool finite = true;
+  for (size_t i = 0; i < c->in.size(); ++i) {
+    const Instance& in = c->in[i];
+    for (int k = 0; k < 4; ++k) {
+      // Neumaier-compensated sum of the instance's block partials
+      double s = 0.0, comp = 0.0;
+      for (int32_t b = in.block_off; b < in.block_off + in.hm.nblocks; ++b) {
+        const double x = part[k * (size_t)nb + b];
+        const double t = s + x;
+        if (std::fabs(s) >= std::fabs(x)) comp += (s - t) + x;
+        else comp += (x - t) + s;
+        s = t;
+      }
+      out[4 * i + k] = std::sqrt(s + comp);
+      if (!std::isfinite(out[4 * i + k])) finite = false;
+    }
+  }
+  if (!finite) return set_err(c, FVK_ERR_NONFINITE, "residual norm is not finite");
+  return FVK_OK;
+}
+
+// the active block list from the instances' active flags
+int upload_active(Engine* c) {
+  std::vector<int32_t> list;
+  for (const Instance& in : c->in)
+    if (in.active)
+      for (int32_t b = 0; b < in.hm.nblocks; ++b) list.push_back(in.block_off + b);
+  c->nactive = (int32_t)list.size();
+  if (!list.empty())
+    FVK_HIP(c, hipMemcpy(c->active, list.data(), list.size() * 4, hipMemcpyHostToDevice));
+  return FVK_OK;
 }
 
 // `reps` back-to-back launches of the stage-2 pass A and pass B kernels, each
 // kind between one event pair.  Stage 2 reads Q1, Qn, G1 and writes G2, Q2, so
 // the repeats leave the state unchanged.
 template <typename T>
-int time_kernels_t(fvk_ctx* c, int32_t reps, double avg_us[2]) {
+int time_kernels_t(Engine* c, int32_t reps, double avg_us[2]) {
   const fvk_options& o = c->opt;
   fvk::StageArgsT<T> a = base_args<T>(c);
   a.stage = 2;
   set_roles(c, 2, a);
+  const bool squeeze = any_squeeze(c);
   hipEvent_t e0, e1;
   FVK_HIP(c, hipEventCreate(&e0));
   FVK_HIP(c, hipEventCreate(&e1));
@@ -254,18 +552,199 @@ int time_kernels_t(fvk_ctx* c, int32_t reps, double avg_us[2]) {
     if (kind == 0 && o.order != 2) continue;
     hipError_t e = hipEventRecord(e0, c->stream);
     for (int32_t r = 0; r < reps && e == hipSuccess; ++r)
-      e = kind == 0 ? fvk::launch_pass_a(a, o.limiter == 2, c->block, c->stream)
+      e = kind == 0 ? fvk::launch_pass_a(a, squeeze, c->block, c->stream)
                     : fvk::launch_pass_b(a, o.order, o.eqn == 2, c->block, c->stream);
     if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e != hipSuccess) rc = hip_err(c, e, "fvk_time_kernels");
+    if (e != hipSuccess) rc = hip_err(c, e, "time_kernels");
     else avg_us[kind] = ms * 1e3 / reps;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
+}
+
+int time_kernels(Engine* c, int32_t reps, double avg_us[2]) {
+  if (!avg_us || reps <= 0) return set_err(c, FVK_ERR_ARG, "time_kernels: bad arguments");
+  if (!c->have_state) return set_err(c, FVK_ERR_ARG, "time_kernels: no state");
+  return c->fp32 ? time_kernels_t<float>(c, reps, avg_us) : time_kernels_t<double>(c, reps, avg_us);
+}
+
+void engine_free(Engine* c) {
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  for (void* p : c->bufs) (void)hipFree(p);
+  c->bufs.clear();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  c->stream = nullptr;
+}
+
+// Host preparation of every instance (in parallel), then the concatenated
+// device layout, state buffers and per-instance constants.
+int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options* opts, bool batched) {
+  std::string msg;
+  for (int32_t i = 0; i < n; ++i) {
+    int rc = check_options(opts[i], &msg);
+    if (!rc && batched) rc = check_common(opts[0], opts[i], &msg);
+    if (rc) return set_err(nullptr, rc, batched ? "instance " + std::to_string(i) + ": " + msg : msg);
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_err(nullptr, FVK_ERR_HIP, "no HIP device");
+  c->opt = opts[0];
+  c->batched = batched;
+  c->in.resize(n);
+  {
+    std::vector<int> rcs(n, FVK_OK);
+    std::vector<std::string> errs(n);
+    std::atomic<int32_t> next{0};
+    auto work = [&]() {
+      for (int32_t i = next++; i < n; i = next++) {
+        c->in[i].opt = opts[i];
+        rcs[i] = fvk::prepare_mesh(meshes[i], opts[i], &c->in[i].hm, &errs[i]);
+      }
+    };
+    const int hw = (int)std::thread::hardware_concurrency();
+    const int nthreads = std::max(1, std::min<int>(n, std::min(16, hw > 0 ? hw : 1)));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    for (int32_t i = 0; i < n; ++i)
+      if (rcs[i])
+        return set_err(nullptr, rcs[i], batched ? "instance " + std::to_string(i) + ": " + errs[i] : errs[i]);
+  }
+  int64_t ncells = 0;
+  for (Instance& in : c->in) {
+    if (!in.hm.warnings.empty()) fputs(in.hm.warnings.c_str(), stderr);
+    const fvk_options& o = in.opt;
+    in.qinf[0] = o.rho;
+    in.qinf[1] = o.rho * o.u;
+    in.qinf[2] = o.rho * o.v;
+    in.qinf[3] = (o.P / (o.gamma - 1)) + 0.5 * o.rho * (o.u * o.u + o.v * o.v);
+    ncells += in.hm.ncp;
+  }
+  if (ncells >= (int64_t(1) << 29)) return set_err(nullptr, FVK_ERR_MESH, "more than 2^29 cells in total");
+  {
+    int32_t cell = 0, block = 0, slot = 0, halo = 0, bface = 0;
+    for (Instance& in : c->in) {
+      in.cell_off = cell; in.block_off = block; in.slot_off = slot;
+      in.halo_off = halo; in.bface_off = bface;
+      cell += in.hm.ncp; block += in.hm.nblocks; slot += in.hm.chunk_off.back();
+      halo += (int32_t)in.hm.halo_idx.size(); bface += (int32_t)in.hm.bf_a.size();
+    }
+  }
+  const fvk_options& o0 = opts[0];
+  c->fp32 = o0.precision == FVK_FP32_MIXED;
+  c->tsize = c->fp32 ? sizeof(float) : sizeof(double);
+  c->block = o0.block_size ? o0.block_size : 256;
+  c->variant = o0.kernel_variant ? o0.kernel_variant : fvk::kVariantLds;
+  hipError_t e = hipStreamCreate(&c->stream);
+  if (e != hipSuccess) {
+    c->stream = nullptr;
+    return hip_err(nullptr, e, "hipStreamCreate");
+  }
+  fvk::DevMesh& d = c->dm;
+  d.nc = 0;
+  d.ncp = (int32_t)ncells;
+  d.blk = c->in[0].hm.blk;
+  d.nblocks = 0;
+  d.max_halo = 0;
+  d.max_bfaces = 0;
+  for (const Instance& in : c->in) {
+    d.nc += in.hm.nc;
+    d.nblocks += in.hm.nblocks;
+    d.max_halo = std::max(d.max_halo, in.hm.max_halo);
+    d.max_bfaces = std::max(d.max_bfaces, in.hm.max_bfaces);
+  }
+  using HM = fvk::HostMesh;
+  int rc;
+  auto ident = [](const Instance&, int32_t x) { return x; };
+  if ((rc = upload_cat<int32_t>(c, &HM::chunk_off, &d.chunk_off,
+                                [](const Instance& in, int32_t x) { return x + in.slot_off; }, true))) return rc;
+  if ((rc = upload_cat(c, &HM::deg, &d.deg))) return rc;
+  if ((rc = upload_cat<int32_t>(c, &HM::code, &d.code, [](const Instance& in, int32_t x) {
+         return x >= 0 ? x + (in.cell_off << 1) : shift_boundary(x, in.cell_off);
+       }))) return rc;
+  if ((rc = upload_cat<int32_t>(c, &HM::lcode, &d.lcode, [](const Instance& in, int32_t x) {
+         return x >= 0 ? x : shift_boundary(x, in.cell_off);
+       }))) return rc;
+  if ((rc = upload_cat<int32_t>(c, &HM::sface, &d.sface, ident))) return rc;
+  if ((rc = upload_cat<int32_t>(c, &HM::halo_ptr, &d.halo_ptr,
+                                [](const Instance& in, int32_t x) { return x + in.halo_off; }, true))) return rc;
+  if ((rc = upload_cat<int32_t>(c, &HM::halo_idx, &d.halo_idx,
+                                [](const Instance& in, int32_t x) { return x + in.cell_off; }))) return rc;
+  if ((rc = upload_cat<int32_t>(c, &HM::bf_ptr, &d.bf_ptr,
+                                [](const Instance& in, int32_t x) { return x + in.bface_off; }, true))) return rc;
+  if ((rc = upload_cat<int32_t>(c, &HM::bf_a, &d.bf_a, ident))) return rc;
+  if ((rc = upload_cat<int32_t>(c, &HM::bf_b, &d.bf_b, [](const Instance& in, int32_t x) {
+         return x >= 0 ? x : shift_boundary(x, in.cell_off);
+       }))) return rc;
+  if ((rc = upload_cat(c, &HM::bf_nx, &d.bf_nx))) return rc;
+  if ((rc = upload_cat(c, &HM::bf_ny, &d.bf_ny))) return rc;
+  if ((rc = upload_cat(c, &HM::bf_area, &d.bf_area))) return rc;
+  if ((rc = upload_cat(c, &HM::bf_fcx, &d.bf_fcx))) return rc;
+  if ((rc = upload_cat(c, &HM::bf_fcy, &d.bf_fcy))) return rc;
+  if ((rc = upload_cat(c, &HM::s_nx, &d.s_nx))) return rc;
+  if ((rc = upload_cat(c, &HM::s_ny, &d.s_ny))) return rc;
+  if ((rc = upload_cat(c, &HM::s_area, &d.s_area))) return rc;
+  if ((rc = upload_cat(c, &HM::s_fcx, &d.s_fcx))) return rc;
+  if ((rc = upload_cat(c, &HM::s_fcy, &d.s_fcy))) return rc;
+  if ((rc = upload_cat(c, &HM::c_ccx, &d.ccx))) return rc;
+  if ((rc = upload_cat(c, &HM::c_ccy, &d.ccy))) return rc;
+  if ((rc = upload_cat(c, &HM::c_vol, &d.vol))) return rc;
+  if ((rc = upload_cat(c, &HM::c_ixx, &d.ixx))) return rc;
+  if ((rc = upload_cat(c, &HM::c_iyy, &d.iyy))) return rc;
+  if ((rc = upload_cat(c, &HM::c_ixy, &d.ixy))) return rc;
+  if (c->variant == fvk::kVariantLds) {
+    // the LDS-staged kernels need a block plus its halo (the largest) in LDS
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if ((e = hipGetDevice(&dev)) != hipSuccess || (e = hipGetDeviceProperties(&prop, dev)) != hipSuccess)
+      return hip_err(c, e, "hipGetDeviceProperties");
+    const size_t need = std::max(fvk::lds_bytes_pass_a(d, c->tsize),
+                                 fvk::lds_bytes_pass_b(d, o0.order, 3, c->tsize));
+    if (need > prop.sharedMemPerBlock) {
+      if (o0.kernel_variant == fvk::kVariantLds || batched)
+        return set_err(c, FVK_ERR_MESH, "mesh blocks need " + std::to_string(need) +
+                                            " bytes of LDS, more than the device allows");
+      c->variant = fvk::kVariantCell;
+    }
+  }
+  if (batched) {
+    std::vector<int32_t> bi;
+    for (size_t i = 0; i < c->in.size(); ++i) bi.insert(bi.end(), c->in[i].hm.nblocks, (int32_t)i);
+    if ((rc = dev_alloc(c, &c->blk_inst, bi.size()))) return rc;
+    FVK_HIP(c, hipMemcpy(c->blk_inst, bi.data(), bi.size() * 4, hipMemcpyHostToDevice));
+    if ((rc = dev_alloc(c, &c->active, bi.size()))) return rc;
+    if ((rc = upload_active(c))) return rc;
+  }
+  if ((rc = c->fp32 ? upload_flows<float>(c) : upload_flows<double>(c))) return rc;
+  const size_t ncp = d.ncp;
+  auto alloc = [&](void** p, size_t count) {
+    unsigned char* b = nullptr;
+    const int r = dev_alloc(c, &b, count * c->tsize);
+    *p = b;
+    return r;
+  };
+  for (int i = 0; i < 3; ++i) {
+    if ((rc = alloc(&c->q[i], 4 * ncp))) return rc;
+    if ((rc = alloc(&c->g[i], 8 * ncp))) return rc;
+  }
+  if ((rc = alloc(&c->sigma, ncp))) return rc;
+  if ((rc = alloc(&c->res, 4 * ncp))) return rc;
+  if ((rc = dev_alloc(c, &c->smax, 2 * c->in.size()))) return rc;
+  if ((rc = dev_alloc(c, &c->partial, 4 * (size_t)d.nblocks))) return rc;
+  for (int i = 0; i < 3; ++i) {
+    if ((rc = fill(c, c->q[i], 4 * (int64_t)ncp, 0.0))) return rc;
+    if ((rc = fill(c, c->g[i], 8 * (int64_t)ncp, 0.0))) return rc;
+  }
+  if ((rc = fill(c, c->res, 4 * (int64_t)ncp, 0.0))) return rc;
+  if ((rc = fill(c, c->sigma, (int64_t)ncp, 0.0))) return rc;
+  FVK_HIP(c, hipStreamSynchronize(c->stream));
+  return FVK_OK;
 }
 
 }  // namespace
@@ -299,264 +778,76 @@ void fvk_default_options(fvk_options* o) {
   o->kernel_variant = 0;
 }
 
-const char* fvk_last_error(const fvk_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+const char* fvk_last_error(const fvk_ctx* c) { return c ? c->e.err.c_str() : g_create_error.c_str(); }
 
 int fvk_create(const fvk_mesh* mesh, const fvk_options* opt, fvk_ctx** out) {
   g_create_error.clear();
   if (!mesh || !opt || !out) return set_err(nullptr, FVK_ERR_ARG, "fvk_create: NULL argument");
   *out = nullptr;
-  std::string msg;
-  int rc = check_options(*opt, &msg);
-  if (rc) return set_err(nullptr, rc, msg);
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-    return set_err(nullptr, FVK_ERR_HIP, "fvk_create: no HIP device");
   fvk_ctx* c = new fvk_ctx();
-  c->opt = *opt;
-  rc = fvk::prepare_mesh(*mesh, *opt, &c->hm, &msg);
+  int rc = engine_init(&c->e, 1, mesh, opt, false);
   if (rc) {
+    if (!c->e.err.empty()) g_create_error = c->e.err;
+    engine_free(&c->e);
     delete c;
-    return set_err(nullptr, rc, msg);
+    return rc;
   }
-  if (!c->hm.warnings.empty()) fputs(c->hm.warnings.c_str(), stderr);
-  const fvk::HostMesh& h = c->hm;
-  c->block = opt->block_size ? opt->block_size : 256;
-  c->variant = opt->kernel_variant ? opt->kernel_variant : fvk::kVariantLds;
-  c->res_blocks = (h.ncp + 255) / 256;
-  c->fp32 = opt->precision == FVK_FP32_MIXED;
-  c->tsize = c->fp32 ? sizeof(float) : sizeof(double);
-  hipError_t e = hipStreamCreate(&c->stream);
-  if (e != hipSuccess) {
-    hip_err(nullptr, e, "hipStreamCreate");
-    delete c;
-    return FVK_ERR_HIP;
-  }
-  auto fail = [&](int code) {
-    g_create_error = c->err;
-    fvk_destroy(c);
-    return code;
-  };
-  fvk::DevMesh& d = c->dm;
-  d.nc = h.nc;
-  d.ncp = h.ncp;
-  d.blk = h.blk;
-  d.nblocks = h.nblocks;
-  d.max_halo = h.max_halo;
-  d.max_bfaces = h.max_bfaces;
-  if ((rc = upload(c, h.chunk_off, &d.chunk_off))) return fail(rc);
-  if ((rc = upload(c, h.deg, &d.deg))) return fail(rc);
-  if ((rc = upload(c, h.code, &d.code))) return fail(rc);
-  if ((rc = upload(c, h.lcode, &d.lcode))) return fail(rc);
-  if ((rc = upload(c, h.sface, &d.sface))) return fail(rc);
-  if ((rc = upload(c, h.halo_ptr, &d.halo_ptr))) return fail(rc);
-  if ((rc = upload(c, h.halo_idx, &d.halo_idx))) return fail(rc);
-  if ((rc = upload(c, h.bf_ptr, &d.bf_ptr))) return fail(rc);
-  if ((rc = upload(c, h.bf_a, &d.bf_a))) return fail(rc);
-  if ((rc = upload(c, h.bf_b, &d.bf_b))) return fail(rc);
-  if ((rc = upload(c, h.bf_nx, &d.bf_nx))) return fail(rc);
-  if ((rc = upload(c, h.bf_ny, &d.bf_ny))) return fail(rc);
-  if ((rc = upload(c, h.bf_area, &d.bf_area))) return fail(rc);
-  if ((rc = upload(c, h.bf_fcx, &d.bf_fcx))) return fail(rc);
-  if ((rc = upload(c, h.bf_fcy, &d.bf_fcy))) return fail(rc);
-  if ((rc = upload(c, h.s_nx, &d.s_nx))) return fail(rc);
-  if ((rc = upload(c, h.s_ny, &d.s_ny))) return fail(rc);
-  if ((rc = upload(c, h.s_area, &d.s_area))) return fail(rc);
-  if ((rc = upload(c, h.s_fcx, &d.s_fcx))) return fail(rc);
-  if ((rc = upload(c, h.s_fcy, &d.s_fcy))) return fail(rc);
-  if ((rc = upload(c, h.c_ccx, &d.ccx))) return fail(rc);
-  if ((rc = upload(c, h.c_ccy, &d.ccy))) return fail(rc);
-  if ((rc = upload(c, h.c_vol, &d.vol))) return fail(rc);
-  if ((rc = upload(c, h.c_ixx, &d.ixx))) return fail(rc);
-  if ((rc = upload(c, h.c_iyy, &d.iyy))) return fail(rc);
-  if ((rc = upload(c, h.c_ixy, &d.ixy))) return fail(rc);
-  if (c->variant == fvk::kVariantLds) {
-    // the LDS-staged kernels need the block plus its largest halo in LDS
-    int dev = 0;
-    hipDeviceProp_t prop;
-    if ((e = hipGetDevice(&dev)) != hipSuccess || (e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) {
-      hip_err(c, e, "hipGetDeviceProperties");
-      return fail(FVK_ERR_HIP);
-    }
-    const size_t need = std::max(fvk::lds_bytes_pass_a(d, c->tsize),
-                                 fvk::lds_bytes_pass_b(d, opt->order, 3, c->tsize));
-    if (need > prop.sharedMemPerBlock) {
-      if (opt->kernel_variant == fvk::kVariantLds) {
-        set_err(c, FVK_ERR_MESH, "mesh blocks need " + std::to_string(need) +
-                                     " bytes of LDS, more than the device allows");
-        return fail(FVK_ERR_MESH);
-      }
-      c->variant = fvk::kVariantCell;
-    }
-  }
-  const size_t ncp = h.ncp;
-  auto alloc = [&](void** p, size_t n) {
-    unsigned char* d = nullptr;
-    const int r = dev_alloc(c, &d, n * c->tsize);
-    *p = d;
-    return r;
-  };
-  for (int i = 0; i < 3; ++i) {
-    if ((rc = alloc(&c->q[i], 4 * ncp))) return fail(rc);
-    if ((rc = alloc(&c->g[i], 8 * ncp))) return fail(rc);
-  }
-  if ((rc = alloc(&c->sigma, ncp))) return fail(rc);
-  if ((rc = alloc(&c->res, 4 * ncp))) return fail(rc);
-  if ((rc = dev_alloc(c, &c->smax, 2))) return fail(rc);
-  if ((rc = dev_alloc(c, &c->partial, 4 * (size_t)c->res_blocks))) return fail(rc);
-  for (int i = 0; i < 3; ++i) {
-    if ((rc = fill(c, c->q[i], 4 * (int64_t)ncp, 0.0))) return fail(rc);
-    if ((rc = fill(c, c->g[i], 8 * (int64_t)ncp, 0.0))) return fail(rc);
-  }
-  if ((rc = fill(c, c->res, 4 * (int64_t)ncp, 0.0))) return fail(rc);
-  if ((rc = fill(c, c->sigma, (int64_t)ncp, 0.0))) return fail(rc);
-  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) { hip_err(c, e, "sync"); return fail(FVK_ERR_HIP); }
-
-  const fvk_options& o = *opt;
-  make_flow(o, &c->fl64, false);
-  make_flow(o, &c->fl32, true);
-  c->qinf[0] = o.rho;
-  c->qinf[1] = o.rho * o.u;
-  c->qinf[2] = o.rho * o.v;
-  c->qinf[3] = (o.P / (o.gamma - 1)) + 0.5 * o.rho * (o.u * o.u + o.v * o.v);
   *out = c;
   return FVK_OK;
 }
 
 int fvk_destroy(fvk_ctx* c) {
   if (!c) return FVK_OK;
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (void* p : c->bufs) (void)hipFree(p);
-  c->bufs.clear();
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  engine_free(&c->e);
   delete c;
   return FVK_OK;
 }
 
 int fvk_init_freestream(fvk_ctx* c) {
   if (!c) return FVK_ERR_ARG;
-  const size_t ncp = c->hm.ncp;
-  for (int k = 0; k < 4; ++k) {
-    // FP64: Q_inf; FP32-mixed: the perturbation 0
-    void* blk = static_cast<unsigned char*>(c->q[0]) + k * ncp * c->tsize;
-    const int rc = fill(c, blk, (int64_t)ncp, c->fp32 ? 0.0 : c->qinf[k]);
-    if (rc) return rc;
-  }
-  int rc = arm_start(c);
-  c->fs_prims = true;
+  int rc = put_state(&c->e, 0, nullptr);
+  if (!rc) rc = arm_start(&c->e);
+  c->e.fs_prims = true;
   return rc;
 }
 
 int fvk_set_state(fvk_ctx* c, const double* q) {
   if (!c) return FVK_ERR_ARG;
-  if (!q) return set_err(c, FVK_ERR_ARG, "fvk_set_state: NULL state");
-  const fvk::HostMesh& h = c->hm;
-  const size_t n = 4 * (size_t)h.ncp;
-  if (c->fp32) {
-    std::vector<float> buf(n, 0.0f);
-    for (int k = 0; k < 4; ++k)
-      for (int32_t i = 0; i < h.ncp; ++i)
-        if (h.perm[i] >= 0)
-          buf[k * (size_t)h.ncp + i] = float(q[k * (size_t)h.nc + h.perm[i]] - c->qinf[k]);
-    FVK_HIP(c, hipMemcpy(c->q[0], buf.data(), n * 4, hipMemcpyHostToDevice));
-  } else {
-    std::vector<double> buf(n, 1.0);
-    for (int k = 0; k < 4; ++k)
-      for (int32_t i = 0; i < h.ncp; ++i)
-        if (h.perm[i] >= 0) buf[k * (size_t)h.ncp + i] = q[k * (size_t)h.nc + h.perm[i]];
-    FVK_HIP(c, hipMemcpy(c->q[0], buf.data(), n * 8, hipMemcpyHostToDevice));
-  }
-  int rc = arm_start(c);
-  c->fs_prims = false;
+  if (!q) return set_err(&c->e, FVK_ERR_ARG, "fvk_set_state: NULL state");
+  int rc = put_state(&c->e, 0, q);
+  if (!rc) rc = arm_start(&c->e);
+  c->e.fs_prims = false;
   return rc;
 }
 
 int fvk_step(fvk_ctx* c, int32_t nsteps) {
   if (!c) return FVK_ERR_ARG;
-  if (!c->have_state) return set_err(c, FVK_ERR_ARG, "fvk_step: no state (call fvk_init_freestream or fvk_set_state)");
-  if (nsteps < 0) return set_err(c, FVK_ERR_ARG, "fvk_step: negative step count");
-  auto plain = [c](int, auto&& fn) -> int {
-    hipError_t e = fn();
-    return e == hipSuccess ? FVK_OK : hip_err(c, e, "kernel launch");
-  };
-  for (int32_t n = 0; n < nsteps; ++n) {
-    int rc = run_step(c, n == nsteps - 1, plain);
-    if (rc) return rc;
-  }
-  FVK_HIP(c, hipStreamSynchronize(c->stream));
-  return FVK_OK;
+  return engine_step(&c->e, nsteps);
 }
 
 int fvk_get_state(fvk_ctx* c, double* q, double* grad) {
   if (!c) return FVK_ERR_ARG;
-  if (!q && !grad) return set_err(c, FVK_ERR_ARG, "fvk_get_state: nothing requested");
-  const fvk::HostMesh& h = c->hm;
-  const size_t ncp = h.ncp;
-  if (q) {
-    std::vector<double> buf;
-    int rc = download(c, c->q[0], 4 * ncp, &buf);
-    if (rc) return rc;
-    for (int k = 0; k < 4; ++k) {
-      const double base = c->fp32 ? c->qinf[k] : 0.0;
-      for (int32_t i = 0; i < h.ncp; ++i)
-        if (h.perm[i] >= 0)
-          q[k * (size_t)h.nc + h.perm[i]] = c->fp32 ? base + buf[k * ncp + i] : buf[k * ncp + i];
-    }
-  }
-  if (grad) {
-    std::vector<double> buf;
-    int rc = download(c, c->g[0], 8 * ncp, &buf);
-    if (rc) return rc;
-    for (int k = 0; k < 8; ++k)
-      for (int32_t i = 0; i < h.ncp; ++i)
-        if (h.perm[i] >= 0) grad[k * (size_t)h.nc + h.perm[i]] = buf[k * ncp + i];
-  }
-  return FVK_OK;
+  return get_state(&c->e, 0, q, grad);
 }
 
 int fvk_get_residual(fvk_ctx* c, double* r) {
   if (!c) return FVK_ERR_ARG;
-  if (!r) return set_err(c, FVK_ERR_ARG, "fvk_get_residual: NULL output");
-  const fvk::HostMesh& h = c->hm;
+  if (!r) return set_err(&c->e, FVK_ERR_ARG, "fvk_get_residual: NULL output");
   std::vector<double> buf;
-  int rc = download(c, c->res, 4 * (size_t)h.ncp, &buf);
+  int rc = download_rows(&c->e, c->e.res, 4, 0, &buf);
   if (rc) return rc;
-  for (int k = 0; k < 4; ++k)
-    for (int32_t i = 0; i < h.ncp; ++i)
-      if (h.perm[i] >= 0) r[k * (size_t)h.nc + h.perm[i]] = buf[k * (size_t)h.ncp + i];
+  unpermute(c->e.in[0].hm, buf, 4, r);
   return FVK_OK;
 }
 
 int fvk_residual_l2(fvk_ctx* c, double out[4]) {
   if (!c || !out) return FVK_ERR_ARG;
-  const fvk::HostMesh& h = c->hm;
-  if (c->fp32)
-    FVK_HIP(c, fvk::launch_res_sq(static_cast<const float*>(c->res), h.ncp, c->partial, c->res_blocks, 256, c->stream));
-  else
-    FVK_HIP(c, fvk::launch_res_sq(static_cast<const double*>(c->res), h.ncp, c->partial, c->res_blocks, 256, c->stream));
-  std::vector<double> part(4 * (size_t)c->res_blocks);
-  FVK_HIP(c, hipMemcpyAsync(part.data(), c->partial, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
-  FVK_HIP(c, hipStreamSynchronize(c->stream));
-  bool finite = true;
-  for (int k = 0; k < 4; ++k) {
-    // Neumaier-compensated sum of the block partials
-    double s = 0.0, comp = 0.0;
-    for (int b = 0; b < c->res_blocks; ++b) {
-      const double x = part[k * (size_t)c->res_blocks + b];
-      const double t = s + x;
-      if (std::fabs(s) >= std::fabs(x)) comp += (s - t) + x;
-      else comp += (x - t) + s;
-      s = t;
-    }
-    out[k] = std::sqrt(s + comp);
-    if (!std::isfinite(out[k])) finite = false;
-  }
-  if (!finite) return set_err(c, FVK_ERR_NONFINITE, "residual norm is not finite");
-  return FVK_OK;
+  return residual_l2(&c->e, out);
 }
 
 int fvk_get_geometry(fvk_ctx* c, const char* name, double* out) {
   if (!c || !name || !out) return FVK_ERR_ARG;
-  const fvk::HostMesh& h = c->hm;
+  const fvk::HostMesh& h = c->e.in[0].hm;
   const std::vector<double>* v = nullptr;
   const std::string n = name;
   if (n == "face_area") v = &h.area;
@@ -570,7 +861,7 @@ int fvk_get_geometry(fvk_ctx* c, const char* name, double* out) {
   else if (n == "Ixx") v = &h.ixx;
   else if (n == "Iyy") v = &h.iyy;
   else if (n == "Ixy") v = &h.ixy;
-  if (!v) return set_err(c, FVK_ERR_ARG, "fvk_get_geometry: unknown array " + n);
+  if (!v) return set_err(&c->e, FVK_ERR_ARG, "fvk_get_geometry: unknown array " + n);
   std::memcpy(out, v->data(), v->size() * sizeof(double));
   return FVK_OK;
 }
@@ -579,17 +870,18 @@ int fvk_geometry(const fvk_mesh* mesh, const fvk_options* opt, const char* name,
   g_create_error.clear();
   if (!mesh || !opt || !name || !out) return set_err(nullptr, FVK_ERR_ARG, "fvk_geometry: NULL argument");
   fvk_ctx tmp;
+  tmp.e.in.resize(1);
   std::string msg;
-  int rc = fvk::prepare_mesh(*mesh, *opt, &tmp.hm, &msg);
+  int rc = fvk::prepare_mesh(*mesh, *opt, &tmp.e.in[0].hm, &msg);
   if (rc) return set_err(nullptr, rc, msg);
   rc = fvk_get_geometry(&tmp, name, out);
-  if (rc) g_create_error = tmp.err;
+  if (rc) g_create_error = tmp.e.err;
   return rc;
 }
 
 int fvk_get_info(fvk_ctx* c, int64_t info[6]) {
   if (!c || !info) return FVK_ERR_ARG;
-  const fvk::HostMesh& h = c->hm;
+  const fvk::HostMesh& h = c->e.in[0].hm;
   info[0] = h.nc;
   info[1] = h.nf;
   info[2] = h.nb;
@@ -599,8 +891,9 @@ int fvk_get_info(fvk_ctx* c, int64_t info[6]) {
   return FVK_OK;
 }
 
-int fvk_time_steps(fvk_ctx* c, int32_t nsteps, double out_ms[4], int64_t counts[4]) {
-  if (!c || !out_ms || !counts) return FVK_ERR_ARG;
+int fvk_time_steps(fvk_ctx* cx, int32_t nsteps, double out_ms[4], int64_t counts[4]) {
+  if (!cx || !out_ms || !counts) return FVK_ERR_ARG;
+  Engine* c = &cx->e;
   if (!c->have_state) return set_err(c, FVK_ERR_ARG, "fvk_time_steps: no state");
   for (int i = 0; i < 4; ++i) { out_ms[i] = 0; counts[i] = 0; }
   const int nev = 2 * 6 * (nsteps > 0 ? nsteps : 1) + 2;
@@ -638,9 +931,154 @@ int fvk_time_steps(fvk_ctx* c, int32_t nsteps, double out_ms[4], int64_t counts[
 }
 
 int fvk_time_kernels(fvk_ctx* c, int32_t reps, double avg_us[2]) {
-  if (!c || !avg_us || reps <= 0) return FVK_ERR_ARG;
-  if (!c->have_state) return set_err(c, FVK_ERR_ARG, "fvk_time_kernels: no state");
-  return c->fp32 ? time_kernels_t<float>(c, reps, avg_us) : time_kernels_t<double>(c, reps, avg_us);
+  if (!c) return FVK_ERR_ARG;
+  return time_kernels(&c->e, reps, avg_us);
+}
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+
+const char* fvk_batch_last_error(const fvk_batch* b) {
+  return b ? b->e.err.c_str() : g_create_error.c_str();
+}
+
+int fvk_batch_create(int32_t n, const fvk_mesh* meshes, const fvk_options* opts, fvk_batch** out) {
+  g_create_error.clear();
+  if (n <= 0 || !meshes || !opts || !out) return set_err(nullptr, FVK_ERR_ARG, "fvk_batch_create: bad arguments");
+  *out = nullptr;
+  fvk_batch* b = new fvk_batch();
+  int rc = engine_init(&b->e, n, meshes, opts, true);
+  if (rc) {
+    if (!b->e.err.empty()) g_create_error = b->e.err;
+    engine_free(&b->e);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return FVK_OK;
+}
+
+int fvk_batch_destroy(fvk_batch* b) {
+  if (!b) return FVK_OK;
+  engine_free(&b->e);
+  delete b;
+  return FVK_OK;
+}
+
+int fvk_batch_init(fvk_batch* b, const double* const* q) {
+  if (!b) return FVK_ERR_ARG;
+  Engine* c = &b->e;
+  size_t given = 0;
+  for (size_t i = 0; q && i < c->in.size(); ++i) given += q[i] != nullptr;
+  if (given != 0 && given != c->in.size())
+    return set_err(c, FVK_ERR_ARG, "fvk_batch_init: give every instance a state, or none");
+  for (size_t i = 0; i < c->in.size(); ++i) {
+    int rc = put_state(c, i, given ? q[i] : nullptr);
+    if (rc) return rc;
+    c->in[i].active = true;
+  }
+  int rc = upload_active(c);
+  if (!rc) rc = arm_start(c);
+  c->fs_prims = given == 0;
+  drop_graph(c);
+  return rc;
+}
+
+int fvk_batch_step(fvk_batch* b, int32_t nsteps) {
+  if (!b) return FVK_ERR_ARG;
+  return engine_step(&b->e, nsteps);
+}
+
+int fvk_batch_set_active(fvk_batch* b, const uint8_t* active) {
+  if (!b || !active) return FVK_ERR_ARG;
+  Engine* c = &b->e;
+  for (size_t i = 0; i < c->in.size(); ++i) c->in[i].active = active[i] != 0;
+  return upload_active(c);
+}
+
+int fvk_batch_residual_l2(fvk_batch* b, double* out) {
+  if (!b || !out) return FVK_ERR_ARG;
+  if (!b->e.res_valid) return set_err(&b->e, FVK_ERR_ARG, "fvk_batch_residual_l2: no step run yet");
+  return residual_l2(&b->e, out);
+}
+
+int fvk_batch_run_to_drop(fvk_batch* b, double drop, int32_t max_steps, int32_t check_every,
+                          int32_t* steps, double* res0, double* res_end) {
+  if (!b || !steps || check_every <= 0 || max_steps < 0) return FVK_ERR_ARG;
+  Engine* c = &b->e;
+  if (!c->have_state) return set_err(c, FVK_ERR_ARG, "fvk_batch_run_to_drop: no state");
+  const size_t n = c->in.size();
+  std::vector<double> r(4 * n), r0(n, -1.0);
+  std::vector<int64_t> start(n);
+  for (size_t i = 0; i < n; ++i) start[i] = c->in[i].steps;
+  int32_t done = 0;
+  bool any = c->nactive > 0;
+  while (any && done < max_steps) {
+    // the first check follows the first step: it fixes each instance's reference norm
+    const int32_t k = done == 0 ? 1 : std::min(check_every, max_steps - done);
+    int rc = engine_step(c, k);
+    if (rc) return rc;
+    done += k;
+    rc = residual_l2(c, r.data());
+    if (rc && rc != FVK_ERR_NONFINITE) return rc;
+    bool changed = false;
+    any = false;
+    for (size_t i = 0; i < n; ++i) {
+      Instance& in = c->in[i];
+      if (!in.active) continue;
+      const double rho_res = r[4 * i];
+      if (r0[i] < 0) r0[i] = rho_res;
+      if (res_end) res_end[i] = rho_res;
+      // retire on convergence, and on a non-finite residual (state frozen)
+      if (!(rho_res > drop * r0[i]) || !std::isfinite(rho_res)) {
+        in.active = false;
+        changed = true;
+      } else {
+        any = true;
+      }
+    }
+    if (changed && (rc = upload_active(c))) return rc;
+  }
+  for (size_t i = 0; i < n; ++i) {
+    steps[i] = (int32_t)(c->in[i].steps - start[i]);
+    if (res0) res0[i] = r0[i];
+  }
+  return FVK_OK;
+}
+
+int fvk_batch_get_state(fvk_batch* b, int32_t i, double* q, double* grad) {
+  if (!b || i < 0 || i >= (int32_t)b->e.in.size()) return FVK_ERR_ARG;
+  return get_state(&b->e, i, q, grad);
+}
+
+int fvk_batch_get_info(fvk_batch* b, int64_t info[6]) {
+  if (!b || !info) return FVK_ERR_ARG;
+  const Engine& c = b->e;
+  int64_t nf = 0, active = 0;
+  for (const Instance& in : c.in) {
+    nf += in.hm.nf;
+    active += in.active;
+  }
+  info[0] = (int64_t)c.in.size();
+  info[1] = c.dm.nc;
+  info[2] = nf;
+  info[3] = c.dm.nblocks;
+  info[4] = c.nactive;
+  info[5] = active;
+  return FVK_OK;
+}
+
+int fvk_batch_use_graph(fvk_batch* b, int32_t on) {
+  if (!b) return FVK_ERR_ARG;
+  b->e.use_graph = on != 0;
+  if (!on) drop_graph(&b->e);
+  return FVK_OK;
+}
+
+int fvk_batch_time_kernels(fvk_batch* b, int32_t reps, double avg_us[2]) {
+  if (!b) return FVK_ERR_ARG;
+  return time_kernels(&b->e, reps, avg_us);
 }
 
 }  // extern "C"

@@ -27,7 +27,11 @@ FP32_MIXED = 1
 EXPORTS = ["fvk_default_options", "fvk_create", "fvk_destroy", "fvk_last_error",
            "fvk_init_freestream", "fvk_set_state", "fvk_step", "fvk_get_state",
            "fvk_get_residual", "fvk_residual_l2", "fvk_get_geometry", "fvk_geometry",
-           "fvk_get_info", "fvk_time_steps", "fvk_time_kernels"]
+           "fvk_get_info", "fvk_time_steps", "fvk_time_kernels",
+           "fvk_batch_create", "fvk_batch_destroy", "fvk_batch_last_error", "fvk_batch_init",
+           "fvk_batch_step", "fvk_batch_set_active", "fvk_batch_residual_l2",
+           "fvk_batch_run_to_drop", "fvk_batch_get_state", "fvk_batch_get_info",
+           "fvk_batch_use_graph", "fvk_batch_time_kernels"]
 
 
 class FvkError(RuntimeError):
@@ -88,6 +92,21 @@ def lib():
         L.fvk_get_info.argtypes = [vp, P(ctypes.c_int64)]
         L.fvk_time_steps.argtypes = [vp, ctypes.c_int32, dp, P(ctypes.c_int64)]
         L.fvk_time_kernels.argtypes = [vp, ctypes.c_int32, dp]
+        ip = P(ctypes.c_int32)
+        L.fvk_batch_create.argtypes = [ctypes.c_int32, P(FvkMesh), P(FvkOptions), P(vp)]
+        L.fvk_batch_destroy.argtypes = [vp]
+        L.fvk_batch_last_error.argtypes = [vp]
+        L.fvk_batch_last_error.restype = ctypes.c_char_p
+        L.fvk_batch_init.argtypes = [vp, P(dp)]
+        L.fvk_batch_step.argtypes = [vp, ctypes.c_int32]
+        L.fvk_batch_set_active.argtypes = [vp, P(ctypes.c_uint8)]
+        L.fvk_batch_residual_l2.argtypes = [vp, dp]
+        L.fvk_batch_run_to_drop.argtypes = [vp, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                            ip, dp, dp]
+        L.fvk_batch_get_state.argtypes = [vp, ctypes.c_int32, dp, dp]
+        L.fvk_batch_get_info.argtypes = [vp, P(ctypes.c_int64)]
+        L.fvk_batch_use_graph.argtypes = [vp, ctypes.c_int32]
+        L.fvk_batch_time_kernels.argtypes = [vp, ctypes.c_int32, dp]
         _lib = L
     return _lib
 
@@ -231,3 +250,94 @@ class Solver:
         cnt = (ctypes.c_int64 * 4)()
         self._check(lib().fvk_time_steps(self.h, n, _dp(ms), cnt))
         return ms, list(cnt)
+
+
+class Batch:
+    """Many independent instances advanced together (fvk_batch).
+
+    `cases` is a list of (grid, case) pairs; `overrides` apply to every
+    instance's options (the kernel-shaping ones must agree)."""
+
+    def __init__(self, cases, **overrides):
+        L = lib()
+        self._ma = [_MeshArrays(g) for g, _ in cases]
+        self.nc = [g.num_cells for g, _ in cases]
+        n = len(cases)
+        meshes = (FvkMesh * n)(*[m.m for m in self._ma])
+        opts = (FvkOptions * n)(*[make_options(c, **overrides) for _, c in cases])
+        self.opts = opts
+        h = ctypes.c_void_p()
+        rc = L.fvk_batch_create(n, meshes, opts, ctypes.byref(h))
+        if rc != FVK_OK:
+            raise FvkError(rc, L.fvk_batch_last_error(None).decode())
+        self.h = h
+        self.n = n
+
+    def _check(self, rc):
+        if rc != FVK_OK:
+            raise FvkError(rc, lib().fvk_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fvk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def init(self, states=None):
+        """Freestream start (states None) or one conserved state per instance."""
+        if states is None:
+            self._check(lib().fvk_batch_init(self.h, None))
+            return
+        keep = [np.ascontiguousarray(q, dtype=np.float64).reshape(-1) for q in states]
+        arr = (ctypes.POINTER(ctypes.c_double) * self.n)(*[_dp(q) for q in keep])
+        self._check(lib().fvk_batch_init(self.h, arr))
+
+    def step(self, n: int = 1):
+        self._check(lib().fvk_batch_step(self.h, n))
+
+    def set_active(self, flags):
+        a = np.ascontiguousarray(np.asarray(flags, dtype=np.uint8))
+        self._check(lib().fvk_batch_set_active(
+            self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+
+    def residual_l2(self) -> np.ndarray:
+        out = np.empty(4 * self.n)
+        rc = lib().fvk_batch_residual_l2(self.h, _dp(out))
+        if rc not in (FVK_OK, -5):
+            self._check(rc)
+        return out.reshape(self.n, 4)
+
+    def run_to_drop(self, drop: float, max_steps: int, check_every: int):
+        steps = np.zeros(self.n, dtype=np.int32)
+        r0 = np.zeros(self.n)
+        r1 = np.zeros(self.n)
+        self._check(lib().fvk_batch_run_to_drop(self.h, drop, max_steps, check_every, _ip(steps),
+                                                _dp(r0), _dp(r1)))
+        return steps, r0, r1
+
+    def state(self, i: int) -> np.ndarray:
+        q = np.empty(4 * self.nc[i])
+        self._check(lib().fvk_batch_get_state(self.h, i, _dp(q), None))
+        return q.reshape(4, self.nc[i])
+
+    def state_and_grad(self, i: int):
+        q = np.empty(4 * self.nc[i])
+        g = np.empty(8 * self.nc[i])
+        self._check(lib().fvk_batch_get_state(self.h, i, _dp(q), _dp(g)))
+        return q.reshape(4, self.nc[i]), g.reshape(8, self.nc[i])
+
+    def info(self) -> dict:
+        a = (ctypes.c_int64 * 6)()
+        self._check(lib().fvk_batch_get_info(self.h, a))
+        keys = ["instances", "cells", "faces", "blocks", "active_blocks", "active_instances"]
+        return dict(zip(keys, list(a)))
+
+    def use_graph(self, on: bool):
+        self._check(lib().fvk_batch_use_graph(self.h, 1 if on else 0))
+
+    def time_kernels(self, reps: int):
+        out = np.zeros(2)
+        self._check(lib().fvk_batch_time_kernels(self.h, reps, _dp(out)))
+        return {"pass_a": float(out[0]), "pass_b": float(out[1])}

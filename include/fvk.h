@@ -149,6 +149,51 @@ int fvk_time_steps(fvk_ctx* ctx, int32_t nsteps, double out_ms[4], int64_t count
  * avg_us[1] = pass B, in microseconds per launch. */
 int fvk_time_kernels(fvk_ctx* ctx, int32_t reps, double avg_us[2]);
 
+/* ------------------------------------------------------------------------
+ * Batches: many independent instances (own mesh, freestream, CFL, BC kinds,
+ * limiter switch) advanced together, one launch per kernel and stage for the
+ * whole batch.  No counterpart in the reference, which solves one case per
+ * process; each instance's results are those of a single fvk_ctx run of the
+ * same mesh and options, bit for bit.  Instances must agree on order, eqn,
+ * time_step_type, precision, block_size, kernel_variant (0 or 3) and
+ * xcd_remap.  Steps after the first two of a start replay a captured
+ * two-step HIP graph (fvk_batch_use_graph).  Every call is synchronous.
+ * ------------------------------------------------------------------------ */
+typedef struct fvk_batch fvk_batch;
+
+int fvk_batch_create(int32_t n, const fvk_mesh* meshes, const fvk_options* opts, fvk_batch** out);
+int fvk_batch_destroy(fvk_batch* b);
+/* message of the last failing call on b (or of the last fvk_batch_create
+ * when b is NULL) */
+const char* fvk_batch_last_error(const fvk_batch* b);
+/* Start every instance: q == NULL, or every q[i] NULL, for the freestream
+ * start of fvk_init_freestream; else q[i] = instance i's conserved state as
+ * for fvk_set_state.  All instances become active. */
+int fvk_batch_init(fvk_batch* b, const double* const* q);
+/* nsteps steps of every active instance */
+int fvk_batch_step(fvk_batch* b, int32_t nsteps);
+/* active[i] != 0: instance i takes part in the following steps; a retired
+ * instance keeps its state and residual */
+int fvk_batch_set_active(fvk_batch* b, const uint8_t* active);
+/* out[4*i + k] = sqrt(sum R_k^2) of instance i's last stage-3 residual */
+int fvk_batch_residual_l2(fvk_batch* b, double* out);
+/* Step the active instances, checking every check_every steps (and after
+ * the first step, whose density residual norm is instance i's reference
+ * res0[i]), and retire each instance whose density residual norm has fallen
+ * to drop * res0[i] or is not finite; stop when none is active or after
+ * max_steps steps.  steps[i] = steps instance i ran; res_end[i] = its last
+ * checked density residual norm (res0, res_end nullable). */
+int fvk_batch_run_to_drop(fvk_batch* b, double drop, int32_t max_steps, int32_t check_every,
+                          int32_t* steps, double* res0, double* res_end);
+/* instance i's state and stage-1 limited gradients, as fvk_get_state */
+int fvk_batch_get_state(fvk_batch* b, int32_t i, double* q, double* grad);
+/* [instances, cells, faces, mesh blocks, active blocks, active instances] */
+int fvk_batch_get_info(fvk_batch* b, int64_t info[6]);
+/* 0: launch every step's kernels one by one; 1 (default): replay the graph */
+int fvk_batch_use_graph(fvk_batch* b, int32_t on);
+/* fvk_time_kernels over the active blocks of the whole batch */
+int fvk_batch_time_kernels(fvk_batch* b, int32_t reps, double avg_us[2]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,146 @@
+"""The batched form (fvk_batch_*) against single-instance runs.
+
+A batch concatenates its instances' device layouts and advances all of them
+with one launch per kernel and stage; each instance takes its own freestream,
+CFL, BC kinds, limiter switch and (GLOBAL dt) time-step maximum.  Every
+instance's results must equal, bit for bit, a single fvk_ctx run of the same
+mesh and options -- which for FP64 is the reference's own result
+(test_gpu_parity.py).  Covered: mixed meshes, Mach numbers, angles, limiter
+switches and BC sets in one batch; GLOBAL and LOCAL dt; Euler and NS; FP64
+and FP32-mixed; graph replay against step-by-step launches; retirement
+(retired instances keep their state, the others go on unchanged);
+run-to-residual-drop; option checks.
+"""
+import numpy as np
+import pytest
+
+from fv2d import gridgen as gg
+from fv2d import inputs
+from fv2d.lib import FP32_MIXED, Batch, FvkError, Solver
+from fvtest import mismatch, same
+
+pytestmark = pytest.mark.gpu
+
+
+def euler_cases(dt_type="LOCAL"):
+    out = []
+    for i, (nx, ny, mach, ang, lim) in enumerate([(32, 16, 0.5, 0.0, "SQUEEZE"),
+                                                  (48, 24, 0.7, 2.0, "NONE"),
+                                                  (40, 20, 0.3, 4.5, "SQUEEZE")]):
+        u, v = inputs.mach_velocity(mach, angle_deg=ang)
+        out.append((gg.bump(nx, ny), inputs.Case(u=u, v=v, poly=1, limiter=lim, dt_type=dt_type,
+                                                 CFL=0.8 - 0.1 * i)))
+    u, v = inputs.mach_velocity(2.5, angle_deg=1.0)
+    out.append((gg.ramp(64, 32), inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE",
+                                             dt_type=dt_type, CFL=0.7)))
+    u, v = inputs.mach_velocity(0.6)
+    # slip walls, freestream and extrapolation sides on a jittered mixed mesh
+    out.append((gg.box(12, 9, jitter=0.2, split_fraction=0.5, markers=(0, -1, -2, -1)),
+                inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE", dt_type=dt_type, CFL=0.5,
+                            BC1="SLIP_WALL", BC2="FREESTREAM")))
+    return out
+
+
+def ns_cases():
+    out = []
+    for nr, mach in [(16, 0.2), (24, 0.3)]:
+        u, v = inputs.mach_velocity(mach)
+        out.append((gg.ogrid(2 * nr, nr), inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE",
+                                                      eqn="NS", mu=1.225 * u / 100.0,
+                                                      BC1="NO_SLIP_WALL", dt_type="GLOBAL",
+                                                      CFL=0.5)))
+    return out
+
+
+def single(grid, case, steps, **ov):
+    s = Solver(grid, case, **ov)
+    s.init_freestream()
+    s.step(steps)
+    return s
+
+
+@pytest.mark.parametrize("precision", [0, FP32_MIXED])
+@pytest.mark.parametrize("dt_type", ["LOCAL", "GLOBAL"])
+def test_batch_equals_single_runs(precision, dt_type):
+    cases = euler_cases(dt_type)
+    steps = 25   # two eager steps, graph pairs, one eager step at the end
+    print(f"[gpu] batch euler {dt_type} precision={precision} n={len(cases)}", flush=True)
+    b = Batch(cases, precision=precision)
+    b.init()
+    b.step(steps)
+    r = b.residual_l2()
+    for i, (g, c) in enumerate(cases):
+        s = single(g, c, steps, precision=precision)
+        q, gr = b.state_and_grad(i)
+        q1, gr1 = s.state_and_grad()
+        assert same(q, q1), f"instance {i}: {mismatch(q, q1)}"
+        assert same(gr, gr1), f"instance {i} grad: {mismatch(gr, gr1)}"
+        assert np.isfinite(q).all(), f"instance {i}"
+        assert np.allclose(r[i], s.residual_l2(), rtol=1e-12, atol=0)
+
+
+def test_batch_ns_global_dt():
+    cases = ns_cases()
+    b = Batch(cases)
+    b.init()
+    b.step(12)
+    for i, (g, c) in enumerate(cases):
+        q1 = single(g, c, 12).state()
+        assert same(b.state(i), q1), f"instance {i}: {mismatch(b.state(i), q1)}"
+
+
+def test_batch_graph_equals_eager():
+    cases = euler_cases("GLOBAL")
+    out = []
+    for graph in (True, False):
+        b = Batch(cases, precision=FP32_MIXED)
+        b.use_graph(graph)
+        b.init()
+        b.step(17)
+        out.append([b.state_and_grad(i) for i in range(len(cases))])
+    for i in range(len(cases)):
+        assert same(out[0][i][0], out[1][i][0]) and same(out[0][i][1], out[1][i][1]), i
+
+
+def test_batch_retirement():
+    cases = euler_cases("GLOBAL")
+    b = Batch(cases)
+    b.init()
+    b.step(10)
+    flags = np.ones(len(cases), dtype=np.uint8)
+    flags[1] = 0
+    flags[3] = 0
+    b.set_active(flags)
+    assert b.info()["active_instances"] == len(cases) - 2
+    b.step(11)
+    for i, (g, c) in enumerate(cases):
+        n = 10 if flags[i] == 0 else 21
+        q1 = single(g, c, n).state()
+        assert same(b.state(i), q1), f"instance {i} ({n} steps): {mismatch(b.state(i), q1)}"
+
+
+def test_batch_run_to_drop():
+    cases = []
+    for nx, mach in [(24, 0.4), (32, 0.5), (40, 0.6)]:
+        u, v = inputs.mach_velocity(mach)
+        cases.append((gg.bump(nx, nx // 2), inputs.Case(u=u, v=v, poly=1, limiter="NONE",
+                                                        dt_type="LOCAL", CFL=0.8)))
+    b = Batch(cases)
+    b.init()
+    steps, r0, r1 = b.run_to_drop(1e-2, 4000, 50)
+    print(f"[gpu] run_to_drop steps={steps.tolist()} drop={(r1 / r0).tolist()}", flush=True)
+    assert b.info()["active_instances"] == 0
+    assert (r1 <= 1e-2 * r0).all() and (steps < 4000).all()
+    assert len(set(steps.tolist())) > 1     # instances retire at different steps
+    for i, (g, c) in enumerate(cases):
+        s = single(g, c, int(steps[i]))
+        assert same(b.state(i), s.state()), f"instance {i}"
+        assert np.isclose(s.residual_l2()[0], r1[i], rtol=1e-12, atol=0)
+
+
+def test_batch_rejects_mixed_kernel_options():
+    cases = euler_cases()
+    with pytest.raises(FvkError, match="OPTION"):
+        Batch([cases[0], (cases[1][0], inputs.Case(**{**cases[1][1].__dict__, "eqn": "NS"}))])
+    with pytest.raises(FvkError, match="OPTION"):
+        Batch(cases[:2], kernel_variant=1)
