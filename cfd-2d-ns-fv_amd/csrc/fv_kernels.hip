@@ -283,12 +283,9 @@ __device__ __forceinline__ void lsq_boundary(const StageArgsT<T>& a, const FlowT
 // face state the previous res() call left on this cell's side
 // (squeeze.cpp:68-121): Q_prev + grad_prev . (x_f - x_i), or -101 at first.
 template <typename T>
-__device__ __forceinline__ void face_phi(const StageArgsT<T>& a, const FlowT<T>& fl, double fx,
-                                         double fy, double xc,
-                                         double yc, const T q[4], const T qmin[4],
-                                         const T qmax[4], const T qp[4], const T gpx[4],
-                                         const T gpy[4], T ph[4]) {
-  const T dx = T(fx - xc), dy = T(fy - yc);
+__device__ __forceinline__ void face_phi(const StageArgsT<T>& a, const FlowT<T>& fl, T dx, T dy,
+                                         const T q[4], const T qmin[4], const T qmax[4],
+                                         const T qp[4], const T gpx[4], const T gpy[4], T ph[4]) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const T qf = a.sentinel ? fl.sent[k] : qp[k] + gpx[k] * dx + gpy[k] * dy;
@@ -297,6 +294,53 @@ __device__ __forceinline__ void face_phi(const StageArgsT<T>& a, const FlowT<T>&
     else if (qf > qmax[k]) p = (qmax[k] - q[k]) / (qf - q[k]);
     ph[k] = p;
   }
+}
+
+// Geometry of one face as the flux uses it, in the arithmetic type: normal,
+// length, face centre minus owner / neighbour centroid and, for NS, the unit
+// owner->neighbour vector and centroid distance (interior) or the owner's
+// wall distance (no-slip wall).  Every value is formed in FP64 and rounded
+// once, so the FP64 path keeps the reference's expressions and the FP32 path
+// may read the same values precomputed.
+template <typename T>
+struct FaceGeo {
+  T nx, ny, A;
+  T d1x, d1y, d2x, d2y;
+  T lx, ly, dist;
+};
+
+template <typename T, bool NS>
+__device__ __forceinline__ FaceGeo<T> geo_from(double nx, double ny, double A, double fx, double fy,
+                                               const CellQG<T>& c1, const CellQG<T>* c2, int kind) {
+  FaceGeo<T> g;
+  g.nx = T(nx);
+  g.ny = T(ny);
+  g.A = T(A);
+  g.d1x = T(fx - c1.x);   // reconstruction.cpp:26-29 (face_cell1_dx)
+  g.d1y = T(fy - c1.y);
+  g.d2x = g.d2y = g.lx = g.ly = g.dist = T(0);
+  if (c2) {
+    g.d2x = T(fx - c2->x);
+    g.d2y = T(fy - c2->y);
+    if (NS) {             // face_dist and l (readmesh.cpp:394-399)
+      const double tx = c2->x - c1.x, ty = c2->y - c1.y;
+      const double dist = sqrt(tx * tx + ty * ty);
+      g.lx = T(tx / dist);
+      g.ly = T(ty / dist);
+      g.dist = T(dist);
+    }
+  } else if (NS && kind == 3) {   // reconstruction.cpp:112-113
+    const double dx1 = fx - c1.x, dy1 = fy - c1.y;
+    g.dist = T(fabs(sqrt(dx1 * dx1 + dy1 * dy1)));
+  }
+  return g;
+}
+
+// the geometry of face slot s, from the per-slot FP64 arrays
+template <typename T, bool NS>
+__device__ __forceinline__ FaceGeo<T> slot_geo(const DevMesh& m, int s, const CellQG<T>& c1,
+                                               const CellQG<T>* c2, int kind) {
+  return geo_from<T, NS>(m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], c1, c2, kind);
 }
 
 // Inviscid face terms of one side in perturbation form (FP32-mixed path):
@@ -347,14 +391,13 @@ __device__ __forceinline__ void side_flux_delta(const FlowT<T>& fl, const T d[4]
 template <int ORDER, bool NS, typename T>
 __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>& fl,
                                           const CellQG<T>& c1,
-                                          const CellQG<T>* c2, int kind, double nx_d,
-                                          double ny_d, double A_d, double fx, double fy, T x[4],
-                                          T& sterm) {
+                                          const CellQG<T>* c2, int kind, const FaceGeo<T>& g,
+                                          T x[4], T& sterm) {
   const T gamma = fl.gamma, gm1 = fl.gm1;
-  const T nx = T(nx_d), ny = T(ny_d), A = T(A_d);
+  const T nx = g.nx, ny = g.ny, A = g.A;
   T L[4], R[4];               // face states (conserved, or perturbations)
   if (ORDER == 2) {           // reconstruction.cpp:26-29,73-76
-    const T dx = T(fx - c1.x), dy = T(fy - c1.y);
+    const T dx = g.d1x, dy = g.d1y;
 #pragma unroll
     for (int k = 0; k < 4; ++k) L[k] = c1.q[k] + c1.gx[k] * dx + c1.gy[k] * dy;
   } else {
@@ -367,7 +410,7 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
     side_flux_delta(fl, L, nx, ny, sl);
     if (c2) {
       if (ORDER == 2) {
-        const T dx = T(fx - c2->x), dy = T(fy - c2->y);
+        const T dx = g.d2x, dy = g.d2y;
 #pragma unroll
         for (int k = 0; k < 4; ++k) R[k] = c2->q[k] + c2->gx[k] * dx + c2->gy[k] * dy;
       } else {
@@ -404,7 +447,7 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
     to_prim(L, gamma, gm1, pl);
     if (c2) {
       if (ORDER == 2) {
-        const T dx = T(fx - c2->x), dy = T(fy - c2->y);
+        const T dx = g.d2x, dy = g.d2y;
 #pragma unroll
         for (int k = 0; k < 4; ++k) R[k] = c2->q[k] + c2->gx[k] * dx + c2->gy[k] * dy;
       } else {
@@ -455,8 +498,7 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
     if (!c2) {              // no-slip wall, reconstruction.cpp:110-123
       T crho, cu, cv, cP;
       cell_prim(c1.q, fl, a.fs_prims, crho, cu, cv, cP);
-      const double dx1 = fx - c1.x, dy1 = fy - c1.y;
-      const T dn = T(fabs(sqrt(dx1 * dx1 + dy1 * dy1)));
+      const T dn = g.dist;
       const T dudn = -cu / dn;
       const T dvdn = -cv / dn;
       ux = dudn * nx;
@@ -464,10 +506,8 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
       vx = dvdn * nx;
       vy = dvdn * ny;
     } else {                // interior, reconstruction.cpp:124-161
-      const double tx = c2->x - c1.x, ty = c2->y - c1.y;
-      const double dist_d = sqrt(tx * tx + ty * ty);   // face_dist, and the norm of l
-      const T dist = T(dist_d);
-      const T lx = T(tx / dist_d), ly = T(ty / dist_d);
+      const T dist = g.dist;                           // face_dist
+      const T lx = g.lx, ly = g.ly;
       const T mx = -ny, my = nx;                       // unit p1->p2 (readmesh.cpp:400-406)
       T ddx[4], ddy[4];
 #pragma unroll
@@ -654,7 +694,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_cell(StageArgsT<T> a) {
     for (int j = 0; j < deg; ++j) {
       const int s = base + j * 64;
       T ph[4];
-      face_phi(a, fl, m.s_fcx[s], m.s_fcy[s], xc, yc, q, qmin, qmax, qp, gpx, gpy, ph);
+      face_phi(a, fl, T(m.s_fcx[s] - xc), T(m.s_fcy[s] - yc), q, qmin, qmax, qp, gpx, gpy, ph);
 #pragma unroll
       for (int k = 0; k < 4; ++k) phi[k] = min2(ph[k], phi[k]);
     }
@@ -688,10 +728,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_cell(StageArgsT<T> a) {
         CellQG<T> nb;
         load_cell(a, code >> 1, ORDER == 2, nb);
         own = (code & 1) != 0;
-        if (own) face_flux<ORDER, NS>(a, fl, me, &nb, -1, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
-        else face_flux<ORDER, NS>(a, fl, nb, &me, -1, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
+        if (own) face_flux<ORDER, NS>(a, fl, me, &nb, -1, slot_geo<T, NS>(m, s, me, &nb, -1), x, st);
+        else face_flux<ORDER, NS>(a, fl, nb, &me, -1, slot_geo<T, NS>(m, s, nb, &me, -1), x, st);
       } else {
-        face_flux<ORDER, NS>(a, fl, me, (const CellQG<T>*)nullptr, (~code) & 3, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
+        const int kind = (~code) & 3;
+        face_flux<ORDER, NS>(a, fl, me, (const CellQG<T>*)nullptr, kind,
+                             slot_geo<T, NS>(m, s, me, nullptr, kind), x, st);
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) r[k] = r[k] + (own ? x[k] : -x[k]);
@@ -796,7 +838,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_slot(StageArgsT<T> a) {
       if (j < deg) {
         const int s = base + j * 64;
         T ph[4];
-        face_phi(a, fl, m.s_fcx[s], m.s_fcy[s], xc, yc, q, qmin, qmax, qp, gpx, gpy, ph);
+        face_phi(a, fl, T(m.s_fcx[s] - xc), T(m.s_fcy[s] - yc), q, qmin, qmax, qp, gpx, gpy, ph);
 #pragma unroll
         for (int k = 0; k < 4; ++k) t1[row][k][lane] = ph[k];
       }
@@ -836,7 +878,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
     const int nrows = (m.chunk_off[chunk + 1] - cbase) >> 6;
     const int base = cbase + lane;
     CellQG<T> me;
-    if (active) load_cell(a, c, ORDER == 2, me);
+    load_cell(a, c, ORDER == 2, me);   // padding cells too: c < ncp
     T r[4] = {0, 0, 0, 0}, sg = 0;
     for (int j0 = 0; j0 < nrows; j0 += kRows) {
       const int j = j0 + row;
@@ -849,10 +891,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
           CellQG<T> nb;
           load_cell(a, code >> 1, ORDER == 2, nb);
           own = (code & 1) != 0;
-          if (own) face_flux<ORDER, NS>(a, fl, me, &nb, -1, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
-          else face_flux<ORDER, NS>(a, fl, nb, &me, -1, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
+          if (own) face_flux<ORDER, NS>(a, fl, me, &nb, -1, slot_geo<T, NS>(m, s, me, &nb, -1), x, st);
+          else face_flux<ORDER, NS>(a, fl, nb, &me, -1, slot_geo<T, NS>(m, s, nb, &me, -1), x, st);
         } else {
-          face_flux<ORDER, NS>(a, fl, me, (const CellQG<T>*)nullptr, (~code) & 3, m.s_nx[s], m.s_ny[s], m.s_area[s], m.s_fcx[s], m.s_fcy[s], x, st);
+          const int kind = (~code) & 3;
+          face_flux<ORDER, NS>(a, fl, me, (const CellQG<T>*)nullptr, kind,
+                               slot_geo<T, NS>(m, s, me, nullptr, kind), x, st);
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) t1[row][k][lane] = own ? x[k] : -x[k];
@@ -881,30 +925,32 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
 // ---------------------------------------------------------------------------
 // LDS-staged shape: one block per mesh block (blockDim == blk, or 2*blk for
 // pass B).  LDS holds NQ rows of T per local cell (state, and for pass B the
-// gradient), then two rows of FP64 centroids, then (pass B) the face terms.
-// Every global load a thread needs is issued before the block's first
-// barrier, so a block pays one round trip to memory (two for halo cells,
-// whose index is read first) instead of one per face slot.
+// gradient), then, where the kernel needs them, two rows of FP64 centroids,
+// then (pass B) the face terms.  The FP32 pass B reads per-face offsets
+// instead of centroids (FaceGeo from the bf32 arrays).  Every global load a
+// thread needs is issued before the block's first barrier, so a block pays
+// one round trip to memory (two for halo cells, whose index is read first)
+// instead of one per face slot.
 // ---------------------------------------------------------------------------
 constexpr int kPF = 4;   // face slots per cell whose data are prefetched
 constexpr int kFPF = 2;  // block faces per thread whose records are prefetched
 
-template <typename T, int NQ>
+template <typename T, int NQ, bool XY>
 struct LdsCells {
   T* q;        // [NQ][stride]
-  double* xy;  // [2][stride]
+  double* xy;  // [2][stride] (XY only)
   int stride;
   __device__ LdsCells(unsigned char* base, int s)
       : q(reinterpret_cast<T*>(base)),
         xy(reinterpret_cast<double*>(base + ((NQ * s * (int)sizeof(T) + 7) & ~7))),
         stride(s) {}
   __device__ size_t bytes() const {
-    return ((NQ * stride * sizeof(T) + 7) & ~size_t(7)) + 2 * stride * sizeof(double);
+    return ((NQ * stride * sizeof(T) + 7) & ~size_t(7)) + (XY ? 2 * stride * sizeof(double) : 0);
   }
 };
 
-template <typename T, int NQ>
-__device__ __forceinline__ void stage_block(const StageArgsT<T>& a, int b, LdsCells<T, NQ>& L) {
+template <typename T, int NQ, bool XY>
+__device__ __forceinline__ void stage_block(const StageArgsT<T>& a, int b, LdsCells<T, NQ, XY>& L) {
   const DevMesh& m = a.m;
   const int ncp = m.ncp;
   const int hb = m.halo_ptr[b], nh = m.halo_ptr[b + 1] - hb;
@@ -918,8 +964,27 @@ __device__ __forceinline__ void stage_block(const StageArgsT<T>& a, int b, LdsCe
 #pragma unroll
       for (int k = 0; k < 8; ++k) L.q[(4 + k) * st + i] = a.g[k * ncp + gc];
     }
-    L.xy[i] = m.ccx[gc];
-    L.xy[st + i] = m.ccy[gc];
+    if (XY) {
+      L.xy[i] = m.ccx[gc];
+      L.xy[st + i] = m.ccy[gc];
+    }
+  }
+}
+
+// Face data of slot s that the limiter needs: the FP64 face centre (FP64
+// path; offset from the centroid taken after staging), or the precomputed
+// FP32 offset from the centroid (FP32 path).
+template <typename T>
+using SlotPos = typename std::conditional<kDelta<T>, float, double>::type;
+
+template <typename T>
+__device__ __forceinline__ void slot_pos(const DevMesh& m, int s, SlotPos<T>& px, SlotPos<T>& py) {
+  if constexpr (kDelta<T>) {
+    px = m.s_dfx32[s];
+    py = m.s_dfy32[s];
+  } else {
+    px = m.s_fcx[s];
+    py = m.s_fcy[s];
   }
 }
 
@@ -931,22 +996,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_lds(StageArgsT<T> a) {
   if (!block_of(a, logical_block(a.xcd_blocks), b, inst)) return;   // uniform over the block
   const FlowT<T>& fl = a.flows[inst];
   const bool squeeze = SQUEEZE && fl.squeeze;
-  LdsCells<T, 4> L(lds_raw, m.blk + m.max_halo);
+  LdsCells<T, 4, true> L(lds_raw, m.blk + m.max_halo);
   const int st = L.stride;
   const int i = threadIdx.x;
   const int c = b * m.blk + i;
   const int deg = m.deg[c];
   const int base = m.chunk_off[c >> 6] + (c & 63);
-  // prefetch: slot codes and face centres, lagged state, LSQ inverse
+  // prefetch: slot codes and face positions, lagged state, LSQ inverse
   int lc[kPF];
-  double pfx[kPF], pfy[kPF];
+  SlotPos<T> pfx[kPF], pfy[kPF];
 #pragma unroll
   for (int j = 0; j < kPF; ++j) {
     if (j < deg) {
       const int s = base + j * 64;
       lc[j] = m.lcode[s];
-      pfx[j] = m.s_fcx[s];
-      pfy[j] = m.s_fcy[s];
+      slot_pos<T>(m, s, pfx[j], pfy[j]);
     }
   }
   T qp[4], gpx[4], gpy[4];
@@ -962,7 +1026,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_lds(StageArgsT<T> a) {
   const double xc = L.xy[i], yc = L.xy[st + i];
   T sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, qmin[4], qmax[4];
   init_minmax(qmin, qmax);
-  auto slot = [&](int j, int code, double fx, double fy) {
+  auto slot = [&](int j, int code) {
     T px[4], py[4], tmin[4], tmax[4];
     if (code >= 0) {
       const int nb = code >> 1;
@@ -972,31 +1036,34 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_lds(StageArgsT<T> a) {
       lsq_interior(code & 1, q, xc, yc, qn, L.xy[nb], L.xy[st + nb], px, py, tmin, tmax);
     } else {
       const int s = base + j * 64;
-      lsq_boundary(a, fl, code, q, xc, yc, m.s_nx[s], m.s_ny[s], fx, fy, px, py, tmin, tmax);
+      lsq_boundary(a, fl, code, q, xc, yc, m.s_nx[s], m.s_ny[s], m.s_fcx[s], m.s_fcy[s], px, py,
+                   tmin, tmax);
     }
     accumulate_lsq(sx, sy, qmin, qmax, px, py, tmin, tmax);
   };
 #pragma unroll
   for (int j = 0; j < kPF; ++j)
-    if (j < deg) slot(j, lc[j], pfx[j], pfy[j]);
-  for (int j = kPF; j < deg; ++j) {
-    const int s = base + j * 64;
-    slot(j, m.lcode[s], m.s_fcx[s], m.s_fcy[s]);
-  }
+    if (j < deg) slot(j, lc[j]);
+  for (int j = kPF; j < deg; ++j) slot(j, m.lcode[base + j * 64]);
   T gx[4], gy[4];
   lsq_solve(ixx, iyy, ixy, sx, sy, gx, gy);
   if (squeeze) {
     T phi[4] = {1, 1, 1, 1};
-    auto lim = [&](double fx, double fy) {
+    auto lim = [&](SlotPos<T> px, SlotPos<T> py) {
       T ph[4];
-      face_phi(a, fl, fx, fy, xc, yc, q, qmin, qmax, qp, gpx, gpy, ph);
+      if constexpr (kDelta<T>) face_phi(a, fl, px, py, q, qmin, qmax, qp, gpx, gpy, ph);
+      else face_phi(a, fl, T(px - xc), T(py - yc), q, qmin, qmax, qp, gpx, gpy, ph);
 #pragma unroll
       for (int k = 0; k < 4; ++k) phi[k] = min2(ph[k], phi[k]);
     };
 #pragma unroll
     for (int j = 0; j < kPF; ++j)
       if (j < deg) lim(pfx[j], pfy[j]);
-    for (int j = kPF; j < deg; ++j) lim(m.s_fcx[base + j * 64], m.s_fcy[base + j * 64]);
+    for (int j = kPF; j < deg; ++j) {
+      SlotPos<T> px, py;
+      slot_pos<T>(m, base + j * 64, px, py);
+      lim(px, py);
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       gx[k] = gx[k] * phi[k];
@@ -1006,24 +1073,51 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_lds(StageArgsT<T> a) {
   write_grad(a, c, gx, gy);
 }
 
-// one block face: owner/neighbour codes and geometry
+// One block face: owner and neighbour (local indices, or the boundary code)
+// and its geometry -- FP64 for the FP64 path (FaceGeo built once the
+// centroids are in LDS), precomputed FP32 for the FP32 path.
+template <typename T, bool NS>
 struct FaceRec {
   int ia, ib;
   double nx, ny, A, fx, fy;
 };
 
-__device__ __forceinline__ void load_face(const DevMesh& m, int e, FaceRec& f) {
+template <bool NS>
+struct FaceRec<float, NS> {
+  int ia, ib;
+  FaceGeo<float> g;
+};
+
+template <typename T, bool NS>
+__device__ __forceinline__ void load_face(const DevMesh& m, int e, FaceRec<T, NS>& f) {
   f.ia = m.bf_a[e];
   f.ib = m.bf_b[e];
-  f.nx = m.bf_nx[e];
-  f.ny = m.bf_ny[e];
-  f.A = m.bf_area[e];
-  f.fx = m.bf_fcx[e];
-  f.fy = m.bf_fcy[e];
+  if constexpr (kDelta<T>) {
+    f.g.nx = m.bf32_nx[e];
+    f.g.ny = m.bf32_ny[e];
+    f.g.A = m.bf32_area[e];
+    f.g.d1x = m.bf32_d1x[e];
+    f.g.d1y = m.bf32_d1y[e];
+    f.g.d2x = m.bf32_d2x[e];
+    f.g.d2y = m.bf32_d2y[e];
+    if (NS) {
+      f.g.lx = m.bf32_lx[e];
+      f.g.ly = m.bf32_ly[e];
+      f.g.dist = m.bf32_dist[e];
+    } else {
+      f.g.lx = f.g.ly = f.g.dist = 0.0f;
+    }
+  } else {
+    f.nx = m.bf_nx[e];
+    f.ny = m.bf_ny[e];
+    f.A = m.bf_area[e];
+    f.fx = m.bf_fcx[e];
+    f.fy = m.bf_fcy[e];
+  }
 }
 
-template <typename T, int ORDER, int NQ>
-__device__ __forceinline__ void lds_cell(const LdsCells<T, NQ>& L, int l, CellQG<T>& d) {
+template <typename T, int ORDER, int NQ, bool XY>
+__device__ __forceinline__ void lds_cell(const LdsCells<T, NQ, XY>& L, int l, CellQG<T>& d) {
   const int st = L.stride;
 #pragma unroll
   for (int k = 0; k < 4; ++k) d.q[k] = L.q[k * st + l];
@@ -1032,8 +1126,8 @@ __device__ __forceinline__ void lds_cell(const LdsCells<T, NQ>& L, int l, CellQG
     d.gx[k] = ORDER == 2 ? L.q[(4 + k) * st + l] : T(0);
     d.gy[k] = ORDER == 2 ? L.q[(8 + k) * st + l] : T(0);
   }
-  d.x = L.xy[l];
-  d.y = L.xy[st + l];
+  d.x = XY ? L.xy[l] : 0.0;
+  d.y = XY ? L.xy[st + l] : 0.0;
 }
 
 template <typename T, int ORDER, bool NS>
@@ -1042,13 +1136,14 @@ __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell) void k_pass_b_lds
   // only at stage 3, where the next step's dt needs it)
   extern __shared__ unsigned char lds_raw[];
   constexpr int NQ = ORDER == 2 ? 12 : 4;
+  constexpr bool XY = !kDelta<T>;
   const DevMesh& m = a.m;
   int b = 0, inst = 0;
   const bool live = block_of(a, logical_block(a.xcd_blocks), b, inst);   // uniform
   const FlowT<T>& fl = a.flows[inst];
   double sig_new = 0.0;
   if (live) {
-    LdsCells<T, NQ> L(lds_raw, m.blk + m.max_halo);
+    LdsCells<T, NQ, XY> L(lds_raw, m.blk + m.max_halo);
     T* lf = reinterpret_cast<T*>(lds_raw + L.bytes());
     const int fstride = m.max_bfaces;
     const int fb = m.bf_ptr[b], nfb = m.bf_ptr[b + 1] - fb;
@@ -1061,7 +1156,7 @@ __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell) void k_pass_b_lds
     const int ncp = m.ncp;
     // prefetch: this thread's faces, its cell's slot-to-face map and the
     // cell data of the stage update
-    FaceRec fr[kFPF];
+    FaceRec<T, NS> fr[kFPF];
 #pragma unroll
     for (int t = 0; t < kFPF; ++t) {
       const int e = i + t * blockDim.x;
@@ -1081,15 +1176,22 @@ __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell) void k_pass_b_lds
     stage_block(a, b, L);
     __syncthreads();
     // one thread per face of the block
-    auto face = [&](int e, const FaceRec& f) {
+    auto face = [&](int e, const FaceRec<T, NS>& f) {
       CellQG<T> c1, c2;
       lds_cell<T, ORDER>(L, f.ia, c1);
       T x[4], st;
       if (f.ib >= 0) {
         lds_cell<T, ORDER>(L, f.ib, c2);
-        face_flux<ORDER, NS>(a, fl, c1, &c2, -1, f.nx, f.ny, f.A, f.fx, f.fy, x, st);
+        FaceGeo<T> g;
+        if constexpr (kDelta<T>) g = f.g;
+        else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, &c2, -1);
+        face_flux<ORDER, NS>(a, fl, c1, &c2, -1, g, x, st);
       } else {
-        face_flux<ORDER, NS>(a, fl, c1, (const CellQG<T>*)nullptr, (~f.ib) & 3, f.nx, f.ny, f.A, f.fx, f.fy, x, st);
+        const int kind = (~f.ib) & 3;
+        FaceGeo<T> g;
+        if constexpr (kDelta<T>) g = f.g;
+        else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, nullptr, kind);
+        face_flux<ORDER, NS>(a, fl, c1, (const CellQG<T>*)nullptr, kind, g, x, st);
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) lf[k * fstride + e] = x[k];
@@ -1101,7 +1203,7 @@ __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell) void k_pass_b_lds
       if (e < nfb) face(e, fr[t]);
     }
     for (int e = i + kFPF * blockDim.x; e < nfb; e += blockDim.x) {
-      FaceRec f;
+      FaceRec<T, NS> f;
       load_face(m, fb + e, f);
       face(e, f);
     }
@@ -1197,8 +1299,8 @@ void shape(const StageArgsT<T>& a, int nlogical, int* grid, int* per_xcd) {
   }
 }
 
-size_t lds_cells_bytes(int stride, int nq, size_t tsize) {
-  return ((nq * (size_t)stride * tsize + 7) & ~size_t(7)) + 2 * (size_t)stride * 8;
+size_t lds_cells_bytes(int stride, int nq, size_t tsize, bool xy = true) {
+  return ((nq * (size_t)stride * tsize + 7) & ~size_t(7)) + (xy ? 2 * (size_t)stride * 8 : 0);
 }
 
 }  // namespace
@@ -1208,7 +1310,7 @@ size_t lds_bytes_pass_a(const DevMesh& m, size_t tsize) {
 }
 
 size_t lds_bytes_pass_b(const DevMesh& m, int order, int stage, size_t tsize) {
-  return lds_cells_bytes(m.blk + m.max_halo, order == 2 ? 12 : 4, tsize) +
+  return lds_cells_bytes(m.blk + m.max_halo, order == 2 ? 12 : 4, tsize, tsize == 8) +
          (stage == 3 ? 5 : 4) * (size_t)m.max_bfaces * tsize;
 }
 

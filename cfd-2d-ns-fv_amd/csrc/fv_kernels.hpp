@@ -48,6 +48,20 @@ struct DevMesh {
   const double* bf_area;
   const double* bf_fcx;
   const double* bf_fcy;
+  // FP32-mixed only (null otherwise): single-precision geometry, each value
+  // computed in FP64 and rounded once
+  const float* s_dfx32;        // per slot: face centre - cell centroid
+  const float* s_dfy32;
+  const float* bf32_nx;        // per block face: normal, length
+  const float* bf32_ny;
+  const float* bf32_area;
+  const float* bf32_d1x;       // face centre - owner centroid
+  const float* bf32_d1y;
+  const float* bf32_d2x;       // face centre - neighbour centroid (interior)
+  const float* bf32_d2y;
+  const float* bf32_lx;        // NS: unit owner->neighbour, centroid distance
+  const float* bf32_ly;        //     (interior); owner's wall distance (no-slip)
+  const float* bf32_dist;
 };
 
 // Per-instance constants: gas and freestream in the arithmetic type
@@ -102,6 +116,8 @@ struct StageArgsT {
 enum { kVariantCell = 1, kVariantSlot = 2, kVariantLds = 3 };
 
 // LDS bytes the LDS-staged kernels need for this mesh and state type size
+// (pass B stages centroids only for FP64: the FP32 kernels read per-face
+// offsets instead)
 size_t lds_bytes_pass_a(const DevMesh& m, size_t tsize);
 size_t lds_bytes_pass_b(const DevMesh& m, int order, int stage, size_t tsize);
 

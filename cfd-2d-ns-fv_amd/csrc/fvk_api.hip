@@ -698,6 +698,23 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   if ((rc = upload_cat(c, &HM::c_ixx, &d.ixx))) return rc;
   if ((rc = upload_cat(c, &HM::c_iyy, &d.iyy))) return rc;
   if ((rc = upload_cat(c, &HM::c_ixy, &d.ixy))) return rc;
+  if (c->fp32) {
+    // the FP32 kernels' single-precision geometry (mesh_prep.hpp)
+    if ((rc = upload_cat(c, &HM::s_dfx32, &d.s_dfx32))) return rc;
+    if ((rc = upload_cat(c, &HM::s_dfy32, &d.s_dfy32))) return rc;
+    if ((rc = upload_cat(c, &HM::bf32_nx, &d.bf32_nx))) return rc;
+    if ((rc = upload_cat(c, &HM::bf32_ny, &d.bf32_ny))) return rc;
+    if ((rc = upload_cat(c, &HM::bf32_area, &d.bf32_area))) return rc;
+    if ((rc = upload_cat(c, &HM::bf32_d1x, &d.bf32_d1x))) return rc;
+    if ((rc = upload_cat(c, &HM::bf32_d1y, &d.bf32_d1y))) return rc;
+    if ((rc = upload_cat(c, &HM::bf32_d2x, &d.bf32_d2x))) return rc;
+    if ((rc = upload_cat(c, &HM::bf32_d2y, &d.bf32_d2y))) return rc;
+    if (o0.eqn == 2) {
+      if ((rc = upload_cat(c, &HM::bf32_lx, &d.bf32_lx))) return rc;
+      if ((rc = upload_cat(c, &HM::bf32_ly, &d.bf32_ly))) return rc;
+      if ((rc = upload_cat(c, &HM::bf32_dist, &d.bf32_dist))) return rc;
+    }
+  }
   if (c->variant == fvk::kVariantLds) {
     // the LDS-staged kernels need a block plus its halo (the largest) in LDS
     int dev = 0;

@@ -468,6 +468,55 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
       h.s_fcx[s] = h.fcx[f]; h.s_fcy[s] = h.fcy[f];
     }
   }
+  if (o.precision == FVK_FP32_MIXED) {
+    // single-precision geometry of the FP32-mixed kernels
+    h.s_dfx32.assign(nslots, 0.0f);
+    h.s_dfy32.assign(nslots, 0.0f);
+    for (int32_t i = 0; i < h.ncp; ++i) {
+      const int32_t c = h.perm[i];
+      if (c < 0) continue;
+      const int32_t base = h.chunk_off[i / kChunk] + (i % kChunk);
+      for (int32_t j = h.cf_ptr[c]; j < h.cf_ptr[c + 1]; ++j) {
+        const int32_t f = h.cf_idx[j];
+        const int64_t s = base + (int64_t)(j - h.cf_ptr[c]) * kChunk;
+        h.s_dfx32[s] = (float)(h.fcx[f] - h.ccx[c]);
+        h.s_dfy32[s] = (float)(h.fcy[f] - h.ccy[c]);
+      }
+    }
+    const bool ns = o.eqn == 2;
+    h.bf32_nx.resize(nbf); h.bf32_ny.resize(nbf); h.bf32_area.resize(nbf);
+    h.bf32_d1x.resize(nbf); h.bf32_d1y.resize(nbf);
+    h.bf32_d2x.assign(nbf, 0.0f); h.bf32_d2y.assign(nbf, 0.0f);
+    if (ns) {
+      h.bf32_lx.assign(nbf, 0.0f); h.bf32_ly.assign(nbf, 0.0f); h.bf32_dist.assign(nbf, 0.0f);
+    }
+    for (int32_t b = 0; b < h.nblocks; ++b) {
+      size_t e = h.bf_ptr[b];
+      for (int32_t f : bfaces[b]) {
+        const int32_t c1 = h.fc1[f], c2 = h.fc2[f];
+        h.bf32_nx[e] = (float)h.nx[f];
+        h.bf32_ny[e] = (float)h.ny[f];
+        h.bf32_area[e] = (float)h.area[f];
+        const double dx1 = h.fcx[f] - h.ccx[c1], dy1 = h.fcy[f] - h.ccy[c1];
+        h.bf32_d1x[e] = (float)dx1;
+        h.bf32_d1y[e] = (float)dy1;
+        if (c2 >= 0) {
+          h.bf32_d2x[e] = (float)(h.fcx[f] - h.ccx[c2]);
+          h.bf32_d2y[e] = (float)(h.fcy[f] - h.ccy[c2]);
+          if (ns) {   // reconstruction.cpp:124-161 (face_dist, l)
+            const double tx = h.ccx[c2] - h.ccx[c1], ty = h.ccy[c2] - h.ccy[c1];
+            const double dist = std::sqrt(tx * tx + ty * ty);
+            h.bf32_lx[e] = (float)(tx / dist);
+            h.bf32_ly[e] = (float)(ty / dist);
+            h.bf32_dist[e] = (float)dist;
+          }
+        } else if (ns && c2 == FVK_BC_NO_SLIP_WALL) {   // reconstruction.cpp:110-123
+          h.bf32_dist[e] = (float)std::fabs(std::sqrt(dx1 * dx1 + dy1 * dy1));
+        }
+        ++e;
+      }
+    }
+  }
   auto permute = [&](const std::vector<double>& src, std::vector<double>& dst, double pad) {
     dst.assign(h.ncp, pad);
     for (int32_t i = 0; i < h.ncp; ++i)

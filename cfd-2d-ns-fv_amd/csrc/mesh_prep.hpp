@@ -74,6 +74,16 @@ struct HostMesh {
   std::vector<int32_t> bf_ptr;                        // block face lists
   std::vector<int32_t> bf_a, bf_b;                    // local owner; local neighbour or boundary code
   std::vector<double> bf_nx, bf_ny, bf_area, bf_fcx, bf_fcy;
+  // FP32-mixed only: the single-precision geometry the FP32 kernels use,
+  // each value computed in FP64 from the arrays above and rounded once
+  // (exactly what an FP32 kernel would compute from them):
+  //   per slot    face centre - cell centroid;
+  //   per block face  normal, length, face centre - owner / neighbour
+  //   centroid; NS: unit owner->neighbour vector and centroid distance
+  //   (interior), wall distance of the owner centroid (no-slip walls)
+  std::vector<float> s_dfx32, s_dfy32;
+  std::vector<float> bf32_nx, bf32_ny, bf32_area, bf32_d1x, bf32_d1y, bf32_d2x, bf32_d2y;
+  std::vector<float> bf32_lx, bf32_ly, bf32_dist;
   std::string warnings;
 };
 
