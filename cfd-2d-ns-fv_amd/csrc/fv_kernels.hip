@@ -135,21 +135,55 @@ __device__ __forceinline__ void cell_prim(const T qs[4], const FlowT<T>& fl, int
   P = (E - (T(0.5) * rho * (u * u + v * v))) * fl.gm1;
 }
 
+// Cell-major state: the 4 values of a cell's state (or residual) are
+// contiguous, and so are the 8 of its gradient (gx[4], gy[4]), so a cell is
+// read and written with 16-byte vector accesses (one per 4 floats, two per 4
+// doubles) -- coalesced across a wavefront's consecutive cells, and a single
+// access per halo or neighbour cell where a component-major layout needs four.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void ld4(const float* p, float v[4]) {
+  const f4v x = *reinterpret_cast<const f4v*>(p);
+  v[0] = x[0]; v[1] = x[1]; v[2] = x[2]; v[3] = x[3];
+}
+__device__ __forceinline__ void ld4(const double* p, double v[4]) {
+  const d2v x = reinterpret_cast<const d2v*>(p)[0];
+  const d2v y = reinterpret_cast<const d2v*>(p)[1];
+  v[0] = x[0]; v[1] = x[1]; v[2] = y[0]; v[3] = y[1];
+}
+__device__ __forceinline__ void st4(float* p, const float v[4]) {
+  f4v x;
+  x[0] = v[0]; x[1] = v[1]; x[2] = v[2]; x[3] = v[3];
+  *reinterpret_cast<f4v*>(p) = x;
+}
+__device__ __forceinline__ void st4(double* p, const double v[4]) {
+  d2v x, y;
+  x[0] = v[0]; x[1] = v[1]; y[0] = v[2]; y[1] = v[3];
+  reinterpret_cast<d2v*>(p)[0] = x;
+  reinterpret_cast<d2v*>(p)[1] = y;
+}
+
 template <typename T>
-__device__ __forceinline__ void load_q(const T* __restrict__ qa, int ncp, int c, T q[4]) {
-#pragma unroll
-  for (int k = 0; k < 4; ++k) q[k] = qa[k * ncp + c];
+__device__ __forceinline__ void load_q(const T* __restrict__ qa, int c, T q[4]) {
+  ld4(qa + 4 * (size_t)c, q);
+}
+
+template <typename T>
+__device__ __forceinline__ void load_g(const T* __restrict__ ga, int c, T gx[4], T gy[4]) {
+  ld4(ga + 8 * (size_t)c, gx);
+  ld4(ga + 8 * (size_t)c + 4, gy);
 }
 
 template <typename T>
 __device__ __forceinline__ void load_cell(const StageArgsT<T>& a, int c, bool order2,
                                           CellQG<T>& d) {
-  const int ncp = a.m.ncp;
-  load_q(a.qc, ncp, c, d.q);
+  load_q(a.qc, c, d.q);
+  if (order2) {
+    load_g(a.g, c, d.gx, d.gy);
+  } else {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    d.gx[k] = order2 ? a.g[k * ncp + c] : T(0);
-    d.gy[k] = order2 ? a.g[(4 + k) * ncp + c] : T(0);
+    for (int k = 0; k < 4; ++k) d.gx[k] = d.gy[k] = T(0);
   }
   d.x = a.m.ccx[c];
   d.y = a.m.ccy[c];
@@ -255,7 +289,7 @@ __device__ __forceinline__ void lsq_boundary(const StageArgsT<T>& a, const FlowT
     for (int k = 0; k < 4; ++k) d[k] = T(0);
   } else {                    // NO_SLIP_WALL: state of the cell whose index is the face index
     T qs[4];
-    load_q(a.qc, a.m.ncp, v >> 2, qs);
+    load_q(a.qc, v >> 2, qs);
     if constexpr (kDelta<T>) {  // W = (rho_s, -m_s, E_s)
       d[0] = qs[0] - q[0];
       d[1] = (-qs[1] - T(2.0) * fl.qinf[1]) - q[1];
@@ -552,7 +586,6 @@ template <typename T>
 __device__ __forceinline__ void cell_update(const StageArgsT<T>& a, const FlowT<T>& fl, int inst,
                                             int c, const T q[4], const T qn[4], T r[4], T sg,
                                             double V_d, T sig_old) {
-  const int ncp = a.m.ncp;
   const T V = T(V_d);
 #pragma unroll
   for (int k = 0; k < 4; ++k) r[k] = r[k] / V;
@@ -570,14 +603,10 @@ __device__ __forceinline__ void cell_update(const StageArgsT<T>& a, const FlowT<
     for (int k = 0; k < 4; ++k)
       out[k] = T(1.0 / 3.0) * qn[k] + T(2.0 / 3.0) * q[k] + T(2.0 / 3.0) * dt * r[k];
   }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) a.qout[k * ncp + c] = out[k];
+  st4(a.qout + 4 * (size_t)c, out);
   if (a.stage == 3) {
     a.sigma[c] = sg;
-    if (a.store_res) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) a.res[k * ncp + c] = r[k];
-    }
+    if (a.store_res) st4(a.res + 4 * (size_t)c, r);
   }
 }
 
@@ -586,7 +615,7 @@ template <typename T>
 __device__ __forceinline__ void cell_update_ld(const StageArgsT<T>& a, const FlowT<T>& fl, int c,
                                                const T q[4], T r[4], T sg) {
   T qn[4] = {0, 0, 0, 0};
-  if (a.stage > 1) load_q(a.qn, a.m.ncp, c, qn);
+  if (a.stage > 1) load_q(a.qn, c, qn);
   const T sig_old = a.dt_global ? T(1) : a.sigma[c];
   cell_update(a, fl, 0, c, q, qn, r, sg, a.m.vol[c], sig_old);
 }
@@ -611,24 +640,17 @@ __device__ __forceinline__ void lsq_solve(double ixx, double iyy, double ixy, co
 template <typename T>
 __device__ __forceinline__ void write_grad(const StageArgsT<T>& a, int c, const T gx[4],
                                            const T gy[4]) {
-  const int ncp = a.m.ncp;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    a.g[k * ncp + c] = gx[k];
-    a.g[(4 + k) * ncp + c] = gy[k];
-  }
+  st4(a.g + 8 * (size_t)c, gx);
+  st4(a.g + 8 * (size_t)c + 4, gy);
 }
 
 template <typename T>
 __device__ __forceinline__ void load_lagged(const StageArgsT<T>& a, int c, T qp[4], T gpx[4],
                                             T gpy[4]) {
-  const int ncp = a.m.ncp;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    qp[k] = a.sentinel ? T(0) : a.qp[k * ncp + c];
-    gpx[k] = a.sentinel ? T(0) : a.gp[k * ncp + c];
-    gpy[k] = a.sentinel ? T(0) : a.gp[(4 + k) * ncp + c];
-  }
+  // the buffers always hold finite values (zero-filled at creation), and
+  // face_phi ignores them while the sentinel is armed
+  load_q(a.qp, c, qp);
+  load_g(a.gp, c, gpx, gpy);
 }
 
 // the reference's min/max start values 1e11 / -1e11 (squeeze.cpp:10-11)
@@ -666,7 +688,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_cell(StageArgsT<T> a) {
   const int deg = m.deg[c];
   if (deg == 0) return;
   T q[4];
-  load_q(a.qc, m.ncp, c, q);
+  load_q(a.qc, c, q);
   const double xc = m.ccx[c], yc = m.ccy[c];
   const int base = m.chunk_off[c >> 6] + (c & 63);
   T sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, qmin[4], qmax[4];
@@ -678,7 +700,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_cell(StageArgsT<T> a) {
     if (code >= 0) {
       const int nb = code >> 1;
       T qn[4];
-      load_q(a.qc, m.ncp, nb, qn);
+      load_q(a.qc, nb, qn);
       lsq_interior(code & 1, q, xc, yc, qn, m.ccx[nb], m.ccy[nb], px, py, tmin, tmax);
     } else {
       lsq_boundary(a, fl, code, q, xc, yc, m.s_nx[s], m.s_ny[s], m.s_fcx[s], m.s_fcy[s], px, py,
@@ -767,7 +789,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_slot(StageArgsT<T> a) {
   T q[4] = {0, 0, 0, 0};
   double xc = 0, yc = 0;
   if (active) {
-    load_q(a.qc, m.ncp, c, q);
+    load_q(a.qc, c, q);
     xc = m.ccx[c];
     yc = m.ccy[c];
   }
@@ -782,7 +804,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_slot(StageArgsT<T> a) {
       if (code >= 0) {
         const int nb = code >> 1;
         T qn[4];
-        load_q(a.qc, m.ncp, nb, qn);
+        load_q(a.qc, nb, qn);
         lsq_interior(code & 1, q, xc, yc, qn, m.ccx[nb], m.ccy[nb], px, py, tmin, tmax);
       } else {
         lsq_boundary(a, fl, code, q, xc, yc, m.s_nx[s], m.s_ny[s], m.s_fcx[s], m.s_fcy[s], px, py,
@@ -932,8 +954,35 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
 // one round trip to memory (two for halo cells, whose index is read first)
 // instead of one per face slot.
 // ---------------------------------------------------------------------------
-constexpr int kPF = 4;   // face slots per cell whose data are prefetched
-constexpr int kFPF = 2;  // block faces per thread whose records are prefetched
+// Tuning constants of the LDS-staged kernels (measured on MI355X):
+//   kPF / kFPF  face slots per cell / block faces per thread prefetched
+//               before the staging barrier;
+//   kMinBlocksA / kMinBlocksB  minimum resident blocks per CU the register
+//               allocation must allow, per state type (FP64, FP32).
+#ifndef FVK_PF
+#define FVK_PF 4
+#endif
+#ifndef FVK_FPF
+#define FVK_FPF 2
+#endif
+#ifndef FVK_MINBLK_A64
+#define FVK_MINBLK_A64 1
+#endif
+#ifndef FVK_MINBLK_A32
+#define FVK_MINBLK_A32 1
+#endif
+#ifndef FVK_MINBLK_B64
+#define FVK_MINBLK_B64 1
+#endif
+#ifndef FVK_MINBLK_B32
+#define FVK_MINBLK_B32 1
+#endif
+constexpr int kPF = FVK_PF;
+constexpr int kFPF = FVK_FPF;
+template <typename T>
+constexpr int kMinBlocksA = kDelta<T> ? FVK_MINBLK_A32 : FVK_MINBLK_A64;
+template <typename T>
+constexpr int kMinBlocksB = kDelta<T> ? FVK_MINBLK_B32 : FVK_MINBLK_B64;
 
 template <typename T, int NQ, bool XY>
 struct LdsCells {
@@ -952,17 +1001,23 @@ struct LdsCells {
 template <typename T, int NQ, bool XY>
 __device__ __forceinline__ void stage_block(const StageArgsT<T>& a, int b, LdsCells<T, NQ, XY>& L) {
   const DevMesh& m = a.m;
-  const int ncp = m.ncp;
   const int hb = m.halo_ptr[b], nh = m.halo_ptr[b + 1] - hb;
   const int n = m.blk + nh;
   const int st = L.stride;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int gc = i < m.blk ? b * m.blk + i : m.halo_idx[hb + i - m.blk];
+    T v[4];
+    load_q(a.qc, gc, v);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) L.q[k * st + i] = a.qc[k * ncp + gc];
+    for (int k = 0; k < 4; ++k) L.q[k * st + i] = v[k];
     if (NQ == 12) {
+      T gx[4], gy[4];
+      load_g(a.g, gc, gx, gy);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) L.q[(4 + k) * st + i] = a.g[k * ncp + gc];
+      for (int k = 0; k < 4; ++k) {
+        L.q[(4 + k) * st + i] = gx[k];
+        L.q[(8 + k) * st + i] = gy[k];
+      }
     }
     if (XY) {
       L.xy[i] = m.ccx[gc];
@@ -989,7 +1044,7 @@ __device__ __forceinline__ void slot_pos(const DevMesh& m, int s, SlotPos<T>& px
 }
 
 template <typename T, bool SQUEEZE>
-__global__ __launch_bounds__(kMaxBlock) void k_pass_a_lds(StageArgsT<T> a) {
+__global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageArgsT<T> a) {
   extern __shared__ unsigned char lds_raw[];
   const DevMesh& m = a.m;
   int b, inst;
@@ -1131,7 +1186,8 @@ __device__ __forceinline__ void lds_cell(const LdsCells<T, NQ, XY>& L, int l, Ce
 }
 
 template <typename T, int ORDER, bool NS>
-__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell) void k_pass_b_lds(StageArgsT<T> a) {
+__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
+void k_pass_b_lds(StageArgsT<T> a) {
   // cell rows, then [4 or 5][max_bfaces] face terms (the spectral-sum term
   // only at stage 3, where the next step's dt needs it)
   extern __shared__ unsigned char lds_raw[];
@@ -1153,7 +1209,6 @@ __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell) void k_pass_b_lds
     const int c = b * m.blk + (cell_thread ? i : 0);
     const int deg = cell_thread ? m.deg[c] : 0;
     const int base = m.chunk_off[c >> 6] + (c & 63);
-    const int ncp = m.ncp;
     // prefetch: this thread's faces, its cell's slot-to-face map and the
     // cell data of the stage update
     FaceRec<T, NS> fr[kFPF];
@@ -1169,7 +1224,7 @@ __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell) void k_pass_b_lds
     T qn[4] = {0, 0, 0, 0}, sig_old = T(1);
     double V = 1.0;
     if (deg > 0) {
-      if (a.stage > 1) load_q(a.qn, ncp, c, qn);
+      if (a.stage > 1) load_q(a.qn, c, qn);
       V = m.vol[c];
       if (!a.dt_global) sig_old = a.sigma[c];
     }
@@ -1264,7 +1319,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_res_sq(const T* __restrict__ res,
   double s[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const double r = double(res[k * ncp + c]);
+    const double r = double(res[4 * (size_t)c + k]);
     s[k] = r * r;
   }
 #pragma unroll

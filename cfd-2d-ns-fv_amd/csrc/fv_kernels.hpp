@@ -79,7 +79,8 @@ struct FlowT {
   int32_t first_block;         // first mesh block of the instance
 };
 
-// One RK stage.  State arrays are SoA blocks of ncp values: q[k*ncp + cell].
+// One RK stage.  State arrays are cell-major: q[4*cell + k], g[8*cell + k]
+// (gx[0..3], gy[0..3]), res[4*cell + k].
 // A batch concatenates its instances' padded cell ranges and block lists;
 // per-block instance ids select the instance's constants and dt maximum.
 template <typename T>
@@ -98,17 +99,17 @@ struct StageArgsT {
   int32_t xcd_remap;           // give each XCD a contiguous range of blocks
   int32_t xcd_blocks;          // blocks per XCD range (set by the launcher)
   int32_t variant;             // kVariantCell, kVariantSlot or kVariantLds
-  const T* qc;                 // stage input state (4 blocks)
+  const T* qc;                 // stage input state (4 per cell)
   const T* qp;                 // previous stage's input state (lagged limiter)
-  const T* gp;                 // previous stage's limited gradient (8 blocks)
-  T* g;                        // this stage's limited gradient (8 blocks)
+  const T* gp;                 // previous stage's limited gradient (8 per cell)
+  T* g;                        // this stage's limited gradient (8 per cell)
   const T* qn;                 // state at the start of the step
   T* qout;                     // stage output state
   T* sigma;                    // per-cell spectral sum of the last stage 3
   unsigned long long* smax_cur;   // per instance: max spectral sum used by this step
                                   // (GLOBAL dt), FP64 bits
   unsigned long long* smax_next;  // per instance: max spectral sum written by stage 3
-  T* res;                      // residual (4 blocks)
+  T* res;                      // residual (4 per cell)
 };
 
 // launch shapes: one thread per cell; one thread per (cell, face slot); or

@@ -414,21 +414,23 @@ int engine_step(Engine* c, int32_t nsteps) {
   return FVK_OK;
 }
 
-// instance i's rows of a [nrows][ncp] device array of the arithmetic type,
-// as doubles [nrows][ncp_i] in the instance's padded numbering
+// instance i's values of a cell-major [ncp][nrows] device array of the
+// arithmetic type, as doubles [nrows][ncp_i] in the instance's padded numbering
 int download_rows(Engine* c, const void* src, int nrows, size_t i, std::vector<double>* out) {
   const Instance& in = c->in[i];
-  const size_t ncp = c->dm.ncp, n = in.hm.ncp;
-  out->resize(nrows * n);
-  std::vector<float> f(c->fp32 ? n : 0);
-  for (int k = 0; k < nrows; ++k) {
-    const void* p = static_cast<const unsigned char*>(src) + (k * ncp + in.cell_off) * c->tsize;
-    if (c->fp32) {
-      FVK_HIP(c, hipMemcpy(f.data(), p, n * 4, hipMemcpyDeviceToHost));
-      for (size_t j = 0; j < n; ++j) (*out)[k * n + j] = f[j];
-    } else {
-      FVK_HIP(c, hipMemcpy(out->data() + k * n, p, n * 8, hipMemcpyDeviceToHost));
-    }
+  const size_t n = in.hm.ncp, count = n * nrows;
+  const void* p = static_cast<const unsigned char*>(src) + (size_t)in.cell_off * nrows * c->tsize;
+  out->resize(count);
+  if (c->fp32) {
+    std::vector<float> f(count);
+    FVK_HIP(c, hipMemcpy(f.data(), p, count * 4, hipMemcpyDeviceToHost));
+    for (size_t j = 0; j < n; ++j)
+      for (int k = 0; k < nrows; ++k) (*out)[k * n + j] = f[j * nrows + k];
+  } else {
+    std::vector<double> d(count);
+    FVK_HIP(c, hipMemcpy(d.data(), p, count * 8, hipMemcpyDeviceToHost));
+    for (size_t j = 0; j < n; ++j)
+      for (int k = 0; k < nrows; ++k) (*out)[k * n + j] = d[j * nrows + k];
   }
   return FVK_OK;
 }
@@ -466,26 +468,24 @@ int get_state(Engine* c, size_t i, double* q, double* grad) {
 int put_state(Engine* c, size_t i, const double* q) {
   const Instance& in = c->in[i];
   const fvk::HostMesh& h = in.hm;
-  const size_t n = h.ncp, ncp = c->dm.ncp;
-  for (int k = 0; k < 4; ++k) {
-    void* dst = elem(c, c->q[0], k * ncp + in.cell_off);
-    if (!q) {
-      // FP64: Q_inf; FP32-mixed: the perturbation 0
-      int rc = fill(c, dst, (int64_t)n, c->fp32 ? 0.0 : in.qinf[k]);
-      if (rc) return rc;
-      continue;
-    }
-    if (c->fp32) {
-      std::vector<float> buf(n, 0.0f);
+  const size_t n = h.ncp;
+  void* dst = elem(c, c->q[0], 4 * (size_t)in.cell_off);
+  if (c->fp32) {
+    // the perturbation from freestream (0 for the freestream start)
+    std::vector<float> buf(4 * n, 0.0f);
+    if (q)
       for (int32_t j = 0; j < h.ncp; ++j)
-        if (h.perm[j] >= 0) buf[j] = float(q[k * (size_t)h.nc + h.perm[j]] - in.qinf[k]);
-      FVK_HIP(c, hipMemcpy(dst, buf.data(), n * 4, hipMemcpyHostToDevice));
-    } else {
-      std::vector<double> buf(n, 1.0);
-      for (int32_t j = 0; j < h.ncp; ++j)
-        if (h.perm[j] >= 0) buf[j] = q[k * (size_t)h.nc + h.perm[j]];
-      FVK_HIP(c, hipMemcpy(dst, buf.data(), n * 8, hipMemcpyHostToDevice));
-    }
+        if (h.perm[j] >= 0)
+          for (int k = 0; k < 4; ++k)
+            buf[4 * (size_t)j + k] = float(q[k * (size_t)h.nc + h.perm[j]] - in.qinf[k]);
+    FVK_HIP(c, hipMemcpy(dst, buf.data(), buf.size() * 4, hipMemcpyHostToDevice));
+  } else {
+    // padding cells hold Q_inf (or 1.0 after a restart); they are never read
+    std::vector<double> buf(4 * n);
+    for (int32_t j = 0; j < h.ncp; ++j)
+      for (int k = 0; k < 4; ++k)
+        buf[4 * (size_t)j + k] = !q ? in.qinf[k] : h.perm[j] >= 0 ? q[k * (size_t)h.nc + h.perm[j]] : 1.0;
+    FVK_HIP(c, hipMemcpy(dst, buf.data(), buf.size() * 8, hipMemcpyHostToDevice));
   }
   return FVK_OK;
 }
