@@ -39,6 +39,14 @@
 
 #include "fv_kernels.hpp"
 
+// FP32-mixed arithmetic: 1 = reciprocal and hardware square root (1 ulp) in
+// the perturbation flux, fused multiply-adds in the FP32 face states and
+// flux; 0 = the FP64 path's operation sequence in FP32.  The FP64 path never
+// changes.
+#ifndef FVK_FAST32
+#define FVK_FAST32 1
+#endif
+
 namespace fvk {
 namespace {
 
@@ -330,6 +338,17 @@ __device__ __forceinline__ void face_phi(const StageArgsT<T>& a, const FlowT<T>&
   }
 }
 
+// Face state q + gx*dx + gy*dy (reconstruction.cpp:26-29,73-76): the FP64
+// path keeps the reference's two roundings per term; the FP32 path fuses
+// them where FVK_FAST32 allows.
+template <typename T>
+__device__ __forceinline__ T recon(T q, T gx, T gy, T dx, T dy) {
+#if FVK_FAST32
+  if constexpr (kDelta<T>) return __builtin_fmaf(gy, dy, __builtin_fmaf(gx, dx, q));
+#endif
+  return q + gx * dx + gy * dy;
+}
+
 // Geometry of one face as the flux uses it, in the arithmetic type: normal,
 // length, face centre minus owner / neighbour centroid and, for NS, the unit
 // owner->neighbour vector and centroid distance (interior) or the owner's
@@ -394,6 +413,29 @@ struct SideFlux {
 template <typename T>
 __device__ __forceinline__ void side_flux_delta(const FlowT<T>& fl, const T d[4], T nx, T ny,
                                                 SideFlux<T>& s) {
+#if FVK_FAST32
+  // one reciprocal of rho and the hardware square root (1 ulp each) and
+  // explicit fused multiply-adds (explicit, so that every launch shape
+  // evaluates the same operations)
+  s.rho = fl.qinf[0] + d[0];
+  const T inv = __builtin_amdgcn_rcpf(s.rho);
+  const T mx = fl.qinf[1] + d[1], my = fl.qinf[2] + d[2];
+  s.u = mx * inv;
+  s.v = my * inv;
+  s.Vn = __builtin_fmaf(s.v, ny, s.u * nx);
+  const T vn_inf = __builtin_fmaf(fl.v, ny, fl.u * nx);
+  const T dmn = __builtin_fmaf(d[2], ny, d[1] * nx);
+  const T dVn = __builtin_fmaf(-vn_inf, d[0], dmn) * inv;
+  const T uu = __builtin_fmaf(s.v, fl.v, s.u * fl.u);
+  const T dK = __builtin_fmaf(d[1], s.u + fl.u, __builtin_fmaf(d[2], s.v + fl.v, -d[0] * uu));
+  const T dP = fl.gm1 * __builtin_fmaf(T(-0.5), dK, d[3]);
+  s.P = fl.P + dP;
+  s.c = __builtin_amdgcn_sqrtf(fl.gamma * (s.P * inv));
+  s.dF[0] = dmn;
+  s.dF[1] = __builtin_fmaf(dP, nx, __builtin_fmaf(fl.qinf[1], dVn, d[1] * s.Vn));
+  s.dF[2] = __builtin_fmaf(dP, ny, __builtin_fmaf(fl.qinf[2], dVn, d[2] * s.Vn));
+  s.dF[3] = __builtin_fmaf(s.Vn, d[3] + dP, dVn * (fl.qinf[3] + fl.P));
+#else
   s.rho = fl.qinf[0] + d[0];
   const T mx = fl.qinf[1] + d[1], my = fl.qinf[2] + d[2];
   s.u = mx / s.rho;
@@ -410,6 +452,7 @@ __device__ __forceinline__ void side_flux_delta(const FlowT<T>& fl, const T d[4]
   s.dF[1] = d[1] * s.Vn + fl.qinf[1] * dVn + dP * nx;
   s.dF[2] = d[2] * s.Vn + fl.qinf[2] * dVn + dP * ny;
   s.dF[3] = dVn * (fl.qinf[3] + fl.P) + s.Vn * (d[3] + dP);
+#endif
 }
 
 // Pass B: one face with owner c1 and neighbour c2 (interior) or a BC ghost of
@@ -433,7 +476,7 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
   if (ORDER == 2) {           // reconstruction.cpp:26-29,73-76
     const T dx = g.d1x, dy = g.d1y;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) L[k] = c1.q[k] + c1.gx[k] * dx + c1.gy[k] * dy;
+    for (int k = 0; k < 4; ++k) L[k] = recon(c1.q[k], c1.gx[k], c1.gy[k], dx, dy);
   } else {
 #pragma unroll
     for (int k = 0; k < 4; ++k) L[k] = c1.q[k];
@@ -446,7 +489,7 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
       if (ORDER == 2) {
         const T dx = g.d2x, dy = g.d2y;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) R[k] = c2->q[k] + c2->gx[k] * dx + c2->gy[k] * dy;
+        for (int k = 0; k < 4; ++k) R[k] = recon(c2->q[k], c2->gx[k], c2->gy[k], dx, dy);
       } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) R[k] = c2->q[k];
@@ -483,7 +526,7 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
       if (ORDER == 2) {
         const T dx = g.d2x, dy = g.d2y;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) R[k] = c2->q[k] + c2->gx[k] * dx + c2->gy[k] * dy;
+        for (int k = 0; k < 4; ++k) R[k] = recon(c2->q[k], c2->gx[k], c2->gy[k], dx, dy);
       } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) R[k] = c2->q[k];
@@ -574,6 +617,12 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const T F = (FL[k] + FR[k]) / 2 - lam * (R[k] - L[k]);
+#if FVK_FAST32
+    if constexpr (kDelta<T>) {
+      x[k] = (Fv[k] - F) * A;
+      continue;
+    }
+#endif
     x[k] = (Fv[k] * A) - (F * A);
   }
   sterm = (fabs(vn_avg) + c_avg) * A;

@@ -365,8 +365,8 @@ int capture_graph(Engine* c) {
   hipGraph_t g = nullptr;
   FVK_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   auto plain = [](int, auto&& fn) -> int { return fn() == hipSuccess ? FVK_OK : FVK_ERR_HIP; };
-  int rc = run_step(c, true, plain);
-  if (!rc) rc = run_step(c, true, plain);
+  int rc = run_step(c, false, plain);
+  if (!rc) rc = run_step(c, false, plain);
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
@@ -391,11 +391,11 @@ int engine_step(Engine* c, int32_t nsteps) {
   if (c->batched && c->nactive == 0) return FVK_OK;   // every instance retired
   int32_t left = nsteps;
   while (left > 0) {
+    // the graph's steps keep no residual: the call's last step (eager) does
     const bool graphable = c->batched && c->use_graph && !c->sentinel && !c->fs_prims &&
-                           c->smax_cur == 0 && left >= 2;
+                           c->smax_cur == 0 && left >= 3;
     if (!graphable) {
-      // a batch always keeps the residual (per-instance convergence checks)
-      int rc = plain_step(c, c->batched || left == 1);
+      int rc = plain_step(c, left == 1);
       if (rc) return rc;
       --left;
       continue;
@@ -407,7 +407,6 @@ int engine_step(Engine* c, int32_t nsteps) {
     FVK_HIP(c, hipGraphLaunch(c->graph, c->stream));
     for (Instance& i : c->in)
       if (i.active) i.steps += 2;
-    c->res_valid = true;
     left -= 2;
   }
   FVK_HIP(c, hipStreamSynchronize(c->stream));
