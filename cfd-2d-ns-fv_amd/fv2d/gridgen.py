@@ -331,6 +331,135 @@ def ogrid(ni: int = 1024, nj: int = 512, r0: float = 0.5, r1: float = 30.0,
     return build_grid(x, y, cells, nv, marker, f"ogrid{ni}x{nj}")
 
 
+# --------------------------------------------------------------------------
+# centroidal dual of a Delaunay triangulation around a NACA0012 section
+# --------------------------------------------------------------------------
+def naca0012_half_thickness(xr: np.ndarray) -> np.ndarray:
+    """Half thickness of the NACA 0012 section at chord fraction xr in [0, 1]
+    (closed trailing edge form of the 4-digit thickness polynomial)."""
+    xr = np.clip(xr, 0.0, 1.0)
+    return 5.0 * 0.12 * (0.2969 * np.sqrt(xr) - 0.1260 * xr - 0.3516 * xr ** 2
+                         + 0.2843 * xr ** 3 - 0.1036 * xr ** 4)
+
+
+def airfoil_dual(n_points: int = 8_000_000, seed: int = DEFAULT_SEED, length: float = 4.0,
+                 chord: float = 1.0, x_le: float = 1.0, y_c: float = 0.5,
+                 jitter: float = 0.25, log: Callable[[str], None] = None) -> Grid:
+    """Polygonal mesh: the centroidal dual of the Delaunay triangulation of
+    about n_points seed-jittered lattice points in [0, length] x [0, 1] with a
+    NACA0012 section (chord `chord`, leading edge at (x_le, y_c), zero angle)
+    cut out.  Each dual cell joins the centroids of the triangles around one
+    Delaunay vertex, so it has as many sides as the vertex has neighbours
+    (3 to about 9).  Vertices on the convex hull, on the section and of
+    triangles removed inside it get no cell; the dual faces towards them are
+    boundary faces: SLIP wall (BC1, file 0) towards the section, FREESTREAM
+    (BC2, file -1) towards the outer hull."""
+    from scipy.spatial import Delaunay
+    ny = max(8, int(round(math.sqrt(n_points / length))))
+    nx = int(round(length * ny))
+    h = 1.0 / ny
+    X, Y = np.meshgrid((np.arange(nx) + 0.5) * (length / nx), (np.arange(ny) + 0.5) * h)
+    n = X.size
+    px = X.ravel() + (splitmix_u01(seed, 21, n) - 0.5) * 2.0 * jitter * h
+    py = Y.ravel() + (splitmix_u01(seed, 22, n) - 0.5) * 2.0 * jitter * h
+    # drop lattice points inside the section or within half a spacing of it
+    xr = (px - x_le) / chord
+    near = (xr > -0.5 * h / chord) & (xr < 1.0 + 0.5 * h / chord) & \
+        (np.abs(py - y_c) < chord * naca0012_half_thickness(xr) + 0.5 * h)
+    px, py = px[~near], py[~near]
+    # surface points about one lattice spacing apart in arc length
+    tf = 0.5 * (1.0 - np.cos(np.linspace(0.0, math.pi, 20001)))
+    arc = np.r_[0.0, np.cumsum(np.hypot(np.diff(tf) * chord,
+                                        np.diff(chord * naca0012_half_thickness(tf))))]
+    ns = max(16, int(math.ceil(arc[-1] / h)) + 1)
+    t = np.interp(np.linspace(0.0, arc[-1], ns), arc, tf)
+    yt = chord * naca0012_half_thickness(t)
+    sx = np.concatenate([x_le + chord * t, x_le + chord * t[1:-1]])
+    sy = np.concatenate([y_c + yt, y_c - yt[1:-1]])
+    # points on the rectangle's sides, half a spacing out of phase with the
+    # lattice rows and columns, so that the hull is the rectangle itself
+    bx = np.arange(nx + 1) * (length / nx)
+    by = np.arange(1, ny) * h
+    rx = np.concatenate([bx, bx, np.zeros(ny - 1), np.full(ny - 1, length)])
+    ry = np.concatenate([np.zeros(nx + 1), np.ones(nx + 1), by, by])
+    n_lat = px.size
+    px = np.concatenate([px, rx, sx])
+    py = np.concatenate([py, ry, sy])
+    n_lat += rx.size
+    if log:
+        log(f"airfoil_dual: Delaunay of {px.size} points")
+    tri = Delaunay(np.stack([px, py], axis=1)).simplices.astype(np.int64)
+    if log:
+        log(f"airfoil_dual: {tri.shape[0]} triangles; building the dual")
+    cx = px[tri].mean(axis=1)
+    cy = py[tri].mean(axis=1)
+    xr = (cx - x_le) / chord
+    inside = (xr >= 0.0) & (xr <= 1.0) & (np.abs(cy - y_c) < chord * naca0012_half_thickness(xr))
+    # boundary vertices: hull, section surface, vertices of removed triangles
+    npts = px.size
+    bnd = np.zeros(npts, dtype=bool)
+    wall = np.zeros(npts, dtype=bool)
+    wall[n_lat:] = True
+    wall[tri[inside].ravel()] = True
+    deg_all = np.bincount(tri.ravel(), minlength=npts)
+    tri = tri[~inside]
+    cx, cy = cx[~inside], cy[~inside]
+    # a vertex is interior when its triangle fan closes: every edge at it is
+    # shared by two kept triangles (hull vertices and vertices next to the
+    # cut-out have an edge with one triangle only)
+    e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
+    e.sort(axis=1)
+    key = e[:, 0] * npts + e[:, 1]
+    uk, cnt = np.unique(key, return_counts=True)
+    open_e = uk[cnt == 1]
+    bnd[open_e // npts] = True
+    bnd[open_e % npts] = True
+    bnd |= wall
+    del deg_all
+    # fans: the triangles around each interior vertex, counter-clockwise
+    vert = tri.ravel()
+    tid = np.repeat(np.arange(tri.shape[0]), 3)
+    keep = ~bnd[vert]
+    vert, tid = vert[keep], tid[keep]
+    ang = np.arctan2(cy[tid] - py[vert], cx[tid] - px[vert])
+    order = np.lexsort((ang, vert))
+    vert, tid = vert[order], tid[order]
+    starts = np.flatnonzero(np.r_[True, vert[1:] != vert[:-1]])
+    counts = np.diff(np.r_[starts, vert.size])
+    ncell = starts.size
+    m = int(counts.max())
+    pos = np.arange(vert.size) - np.repeat(starts, counts)
+    cells = np.full((ncell, m), -1, dtype=np.int64)
+    cells[np.repeat(np.arange(ncell), counts), pos] = tid
+    # boundary dual edges: between consecutive fan triangles t_k, t_k+1 of
+    # vertex v lies the Delaunay edge (v, w); w is a boundary vertex there
+    nxt = np.where(pos + 1 < np.repeat(counts, counts), np.arange(vert.size) + 1,
+                   np.repeat(starts, counts))
+    ta, tb = tid, tid[nxt]
+    va = tri[ta]
+    vb = tri[tb]
+    common = (va[:, :, None] == vb[:, None, :]).any(axis=2)
+    other = np.where(common & (va != vert[:, None]), va, -1).max(axis=1)
+    is_b = bnd[other]
+    lo = np.minimum(ta, tb)[is_b]
+    hi = np.maximum(ta, tb)[is_b]
+    mk = np.where(wall[other[is_b]], 0, -1)
+    bkey = lo * (tri.shape[0] + 1) + hi
+    srt = np.argsort(bkey)
+    bkey, mk = bkey[srt], mk[srt]
+
+    def marker(a, b):
+        k = np.minimum(a, b) * (tri.shape[0] + 1) + np.maximum(a, b)
+        i = np.searchsorted(bkey, k)
+        if np.any(i >= bkey.size) or np.any(bkey[np.minimum(i, bkey.size - 1)] != k):
+            raise ValueError("boundary dual edge without a Delaunay boundary vertex")
+        return mk[i]
+
+    if log:
+        log(f"airfoil_dual: {ncell} cells; faces and canonical order")
+    return build_grid(cx, cy, cells, counts, marker, f"airfoil_dual{n_points}")
+
+
 def box(nx: int = 4, ny: int = 3, jitter: float = 0.0, seed: int = DEFAULT_SEED,
         split_fraction: float = 0.0, markers=(0, -1, -2, -3)) -> Grid:
     """Unit-square test grid; boundary sides bottom/right/top/left carry the

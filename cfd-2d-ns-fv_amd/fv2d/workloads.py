@@ -6,6 +6,10 @@ one seed (gridgen.DEFAULT_SEED unless given).
               order, SQUEEZE, LOCAL dt, CFL 0.8, M 2.5
 * cylinder -- O-grid 1024x512 around a unit cylinder, laminar NS at Re 100,
               M 0.2, second order, SQUEEZE, GLOBAL dt, CFL 0.5
+* airfoil  -- centroidal dual of the Delaunay triangulation of 8M jittered
+              points in [0,4]x[0,1] around a NACA0012 section (chord 1): about
+              7.8M polygonal cells of 4 to 9 sides, Euler, second order,
+              SQUEEZE, LOCAL dt, CFL 0.5, M 0.7 at 1.25 degrees
 * batch    -- 256 independent bump-channel or ramp instances with cell counts
               log-uniform in [2k, 200k] (about 10M cells in all), Mach uniform
               in [0.3, 0.8] (channel) or [1.5, 3.0] (ramp), flow angle uniform
@@ -41,6 +45,15 @@ def cylinder():
     return ("ogrid1024x512 NS Re100 o2 SQUEEZE GLOBAL CFL0.5 M0.2", gg.ogrid(1024, 512),
             inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE", eqn="NS", mu=1.225 * u / 100.0,
                         BC1="NO_SLIP_WALL", dt_type="GLOBAL", CFL=0.5))
+
+
+def airfoil(n_points: int = 8_000_000, log=None):
+    u, v = inputs.mach_velocity(0.7, angle_deg=1.25)
+    grid = gg.airfoil_dual(n_points, log=log)
+    return (f"airfoil dual of Delaunay({n_points} points) NACA0012 Euler o2 SQUEEZE LOCAL "
+            "CFL0.5 M0.7 AoA1.25", grid,
+            inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE", dt_type="LOCAL", CFL=0.5,
+                        BC1="SLIP_WALL", BC2="FREESTREAM"))
 
 
 @dataclasses.dataclass
@@ -92,7 +105,7 @@ def batch(n: int = 256, seed: int = gg.DEFAULT_SEED):
     return name, plan
 
 
-SINGLE = {"channel": channel, "ramp": ramp, "cylinder": cylinder}
+SINGLE = {"channel": channel, "ramp": ramp, "cylinder": cylinder, "airfoil": airfoil}
 
 
 def cells_of(plan) -> np.ndarray:

@@ -161,3 +161,27 @@ def test_reference_grid_file_reads():
     assert gg.check_canonical(g) == []
     vol = F.host_geometry(g, inputs.Case(BC1="NO_SLIP_WALL"), "cell_vol")
     assert np.all(vol > 0)
+
+
+def test_airfoil_dual_generator():
+    """The polygonal Delaunay-dual mesh: 4- to 9-sided cells, a slip wall
+    around the section and freestream outside, closed cells."""
+    g = gg.airfoil_dual(20000)
+    deg = np.bincount(g.face_c1, minlength=g.num_cells)
+    deg += np.bincount(g.face_c2[g.face_c2 >= 0], minlength=g.num_cells)
+    assert deg.min() >= 3 and deg.max() <= 9 and (deg > 6).any()
+    assert set(np.unique(g.bc_marker[g.face_c2 < 0]).tolist()) == {0, -1}
+    assert 2.9 < g.num_faces / g.num_cells < 3.1
+    # wall faces lie on the section's band, freestream faces near the outer rectangle
+    mx = 0.5 * (g.x[g.face_p1] + g.x[g.face_p2])
+    wall = g.bc_marker == 0
+    assert (mx[wall] > 0.99).all() and (mx[wall] < 2.01).all()
+    # sum over each cell's faces of the outward n*A vanishes (closed polygons)
+    case = inputs.Case(poly=1, limiter="SQUEEZE")
+    nx, ny, area = (F.host_geometry(g, case, k) for k in ("face_nx", "face_ny", "face_area"))
+    sx = np.bincount(g.face_c1, nx * area, g.num_cells)
+    sy = np.bincount(g.face_c1, ny * area, g.num_cells)
+    inner = g.face_c2 >= 0
+    sx -= np.bincount(g.face_c2[inner], (nx * area)[inner], g.num_cells)
+    sy -= np.bincount(g.face_c2[inner], (ny * area)[inner], g.num_cells)
+    assert np.abs(sx).max() < 1e-15 and np.abs(sy).max() < 1e-15
