@@ -241,6 +241,18 @@ def errors_vs_fp64(q32, q64, gamma):
             "p_l2_rel": l2(p(q32), p(q64)), "p_linf_rel": linf(p(q32), p(q64))}
 
 
+PERTURBATION = 1e-7
+
+
+def perturbed_freestream(grid, case, seed: int = 7):
+    """Q_inf * (1 + 1e-7 u), u uniform in [-1, 1]: an FP32-rounding-sized
+    perturbation of the freestream start."""
+    qinf = np.array([case.rho, case.rho * case.u, case.rho * case.v,
+                     case.P / (case.gamma - 1) + 0.5 * case.rho * (case.u ** 2 + case.v ** 2)])
+    u = np.random.default_rng(seed).uniform(-1.0, 1.0, (4, grid.num_cells))
+    return qinf[:, None] * (1.0 + PERTURBATION * u)
+
+
 def roofline(per_launch, algo, head, args, child):
     """Roofline object of the dominant kernel (longest average launch)."""
     dom = max(per_launch, key=per_launch.get)
@@ -307,9 +319,19 @@ def bench_single(args):
         out["fp64"] = {"value": r64["value"], "ms_per_step": r64["wall"] * 1e3 / args.steps,
                        "avg_launch_us": {k: round(v, 2) for k, v in r64["per_launch"].items()}}
         out["speedup_vs_fp64"] = round(r["value"] / r64["value"], 4)
-        err = errors_vs_fp64(q, r64["solver"].state(), case.gamma)
+        q64 = r64["solver"].state()
+        err = errors_vs_fp64(q, q64, case.gamma)
         out["error_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in err.items()}
         out["error_vs_fp64"]["after_steps"] = args.warmup + args.steps
+        # the FP64 scheme's own response to an FP32-rounding-sized perturbation
+        log("FP64 sensitivity floor run")
+        fl = Solver(grid, case)
+        fl.set_state(perturbed_freestream(grid, case))
+        fl.step(args.warmup + args.steps)
+        floor = errors_vs_fp64(fl.state(), q64, case.gamma)
+        out["fp64_sensitivity_floor"] = {k: float(f"{v:.3e}") for k, v in floor.items()}
+        out["fp64_sensitivity_floor"]["how"] = ("FP64 from Q_inf(1+1e-7 u), u in [-1,1], "
+                                                "against FP64 from Q_inf, same steps")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline([(grid, case)])
     return out
@@ -413,6 +435,23 @@ def bench_batch(args):
                     errs.append(errors_vs_fp64(q32, q64, c.gamma))
             rho = np.array([e["rho_l2_rel"] for e in errs])
             pl2 = np.array([e["p_l2_rel"] for e in errs])
+            # the FP64 scheme's own response to an FP32-rounding-sized perturbation
+            log("FP64 sensitivity floor batch")
+            fb = Batch(cases, block_size=args.block_size)
+            fb.init([perturbed_freestream(g, c, seed=11 + i) for i, (g, c) in enumerate(cases)])
+            fb.run_to_drop(args.drop, args.steps, args.check_every)
+            fl_rho = []
+            for i, (g, c) in enumerate(cases):
+                q64 = r64["batch"].state(i)
+                qf = fb.state(i)
+                if np.isfinite(q64).all() and np.isfinite(qf).all():
+                    fl_rho.append(errors_vs_fp64(qf, q64, c.gamma)["rho_l2_rel"])
+            fb.close()
+            fl_rho = np.array(fl_rho)
+            out["fp64_sensitivity_floor"] = {
+                "rho_l2_rel_median": float(f"{np.median(fl_rho):.3e}"),
+                "rho_l2_rel_max": float(f"{fl_rho.max():.3e}"),
+                "how": "FP64 batch from Q_inf(1+1e-7 u) against the FP64 batch from Q_inf"}
             out["error_vs_fp64"] = {
                 "rho_l2_rel_median": float(f"{np.median(rho):.3e}"),
                 "rho_l2_rel_max": float(f"{rho.max():.3e}"),
