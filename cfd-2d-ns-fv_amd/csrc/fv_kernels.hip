@@ -1139,9 +1139,11 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
       for (int k = 0; k < 4; ++k) qn[k] = L.q[k * st + nb];
       lsq_interior(code & 1, q, xc, yc, qn, L.xy[nb], L.xy[st + nb], px, py, tmin, tmax);
     } else {
+      // the 16-bit local code holds the kind only; the global code also
+      // names the no-slip ghost's source cell
       const int s = base + j * 64;
-      lsq_boundary(a, fl, code, q, xc, yc, m.s_nx[s], m.s_ny[s], m.s_fcx[s], m.s_fcy[s], px, py,
-                   tmin, tmax);
+      lsq_boundary(a, fl, m.code[s], q, xc, yc, m.s_nx[s], m.s_ny[s], m.s_fcx[s], m.s_fcy[s],
+                   px, py, tmin, tmax);
     }
     accumulate_lsq(sx, sy, qmin, qmax, px, py, tmin, tmax);
   };

@@ -25,8 +25,8 @@ struct DevMesh {
   const int32_t* chunk_off;    // slot offset of each 64-cell chunk
   const uint8_t* deg;          // faces per cell (0 = padding)
   const int32_t* code;         // per slot: global neighbour/owner or boundary code
-  const int32_t* lcode;        // per slot: block-local neighbour/owner or boundary code
-  const int32_t* sface;        // per slot: (block face index << 1) | owner
+  const int16_t* lcode;        // per slot: (block-local neighbour << 1) | owner, or ~kind
+  const int16_t* sface;        // per slot: (block face index << 1) | owner
   const double* s_nx;          // per slot: face unit normal (out of the owner)
   const double* s_ny;
   const double* s_area;        // per slot: face length
@@ -41,8 +41,8 @@ struct DevMesh {
   const int32_t* halo_ptr;     // per block: halo cells
   const int32_t* halo_idx;
   const int32_t* bf_ptr;       // per block: face list
-  const int32_t* bf_a;         // local owner index
-  const int32_t* bf_b;         // local neighbour index or boundary code
+  const int16_t* bf_a;         // local owner index
+  const int16_t* bf_b;         // local neighbour index or ~kind
   const double* bf_nx;
   const double* bf_ny;
   const double* bf_area;

@@ -660,27 +660,22 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   }
   using HM = fvk::HostMesh;
   int rc;
-  auto ident = [](const Instance&, int32_t x) { return x; };
   if ((rc = upload_cat<int32_t>(c, &HM::chunk_off, &d.chunk_off,
                                 [](const Instance& in, int32_t x) { return x + in.slot_off; }, true))) return rc;
   if ((rc = upload_cat(c, &HM::deg, &d.deg))) return rc;
   if ((rc = upload_cat<int32_t>(c, &HM::code, &d.code, [](const Instance& in, int32_t x) {
          return x >= 0 ? x + (in.cell_off << 1) : shift_boundary(x, in.cell_off);
        }))) return rc;
-  if ((rc = upload_cat<int32_t>(c, &HM::lcode, &d.lcode, [](const Instance& in, int32_t x) {
-         return x >= 0 ? x : shift_boundary(x, in.cell_off);
-       }))) return rc;
-  if ((rc = upload_cat<int32_t>(c, &HM::sface, &d.sface, ident))) return rc;
+  if ((rc = upload_cat(c, &HM::lcode, &d.lcode))) return rc;    // block-local: no offsets
+  if ((rc = upload_cat(c, &HM::sface, &d.sface))) return rc;
   if ((rc = upload_cat<int32_t>(c, &HM::halo_ptr, &d.halo_ptr,
                                 [](const Instance& in, int32_t x) { return x + in.halo_off; }, true))) return rc;
   if ((rc = upload_cat<int32_t>(c, &HM::halo_idx, &d.halo_idx,
                                 [](const Instance& in, int32_t x) { return x + in.cell_off; }))) return rc;
   if ((rc = upload_cat<int32_t>(c, &HM::bf_ptr, &d.bf_ptr,
                                 [](const Instance& in, int32_t x) { return x + in.bface_off; }, true))) return rc;
-  if ((rc = upload_cat<int32_t>(c, &HM::bf_a, &d.bf_a, ident))) return rc;
-  if ((rc = upload_cat<int32_t>(c, &HM::bf_b, &d.bf_b, [](const Instance& in, int32_t x) {
-         return x >= 0 ? x : shift_boundary(x, in.cell_off);
-       }))) return rc;
+  if ((rc = upload_cat(c, &HM::bf_a, &d.bf_a))) return rc;
+  if ((rc = upload_cat(c, &HM::bf_b, &d.bf_b))) return rc;
   if ((rc = upload_cat(c, &HM::bf_nx, &d.bf_nx))) return rc;
   if ((rc = upload_cat(c, &HM::bf_ny, &d.bf_ny))) return rc;
   if ((rc = upload_cat(c, &HM::bf_area, &d.bf_area))) return rc;
@@ -722,7 +717,10 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
       return hip_err(c, e, "hipGetDeviceProperties");
     const size_t need = std::max(fvk::lds_bytes_pass_a(d, c->tsize),
                                  fvk::lds_bytes_pass_b(d, o0.order, 3, c->tsize));
-    if (need > prop.sharedMemPerBlock) {
+    // (16-bit local codes: blk + max_halo and max_bfaces below 2^14, far
+    // above what fits in LDS)
+    const bool codes_fit = d.blk + d.max_halo < (1 << 14) && d.max_bfaces < (1 << 14);
+    if (need > prop.sharedMemPerBlock || !codes_fit) {
       if (o0.kernel_variant == fvk::kVariantLds || batched)
         return set_err(c, FVK_ERR_MESH, "mesh blocks need " + std::to_string(need) +
                                             " bytes of LDS, more than the device allows");

@@ -417,8 +417,8 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
   for (int32_t b = 0; b < h.nblocks; ++b) {
     size_t e = h.bf_ptr[b];
     for (int32_t f : bfaces[b]) {
-      h.bf_a[e] = local_of(b, h.iperm[h.fc1[f]]);
-      h.bf_b[e] = (h.fc2[f] >= 0) ? local_of(b, h.iperm[h.fc2[f]]) : boundary_code(f);
+      h.bf_a[e] = (int16_t)local_of(b, h.iperm[h.fc1[f]]);
+      h.bf_b[e] = (int16_t)((h.fc2[f] >= 0) ? local_of(b, h.iperm[h.fc2[f]]) : ~(-h.fc2[f] - 1));
       h.bf_nx[e] = h.nx[f]; h.bf_ny[e] = h.ny[f]; h.bf_area[e] = h.area[f];
       h.bf_fcx[e] = h.fcx[f]; h.bf_fcy[e] = h.fcy[f];
       ++e;
@@ -457,13 +457,13 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
         owner = (h.fc1[f] == c);
         const int32_t other = h.iperm[owner ? c2 : h.fc1[f]];
         h.code[s] = slot_code_interior(other, owner);
-        h.lcode[s] = slot_code_interior(local_of(b, other), owner);
+        h.lcode[s] = (int16_t)slot_code_interior(local_of(b, other), owner);
       } else {
         h.code[s] = boundary_code(f);
-        h.lcode[s] = h.code[s];
+        h.lcode[s] = (int16_t)~(-c2 - 1);   // the kind; the ghost source is in code[s]
       }
       const int32_t fi = (int32_t)(std::lower_bound(fl.begin(), fl.end(), f) - fl.begin());
-      h.sface[s] = (fi << 1) | (owner ? 1 : 0);
+      h.sface[s] = (int16_t)((fi << 1) | (owner ? 1 : 0));
       h.s_nx[s] = h.nx[f]; h.s_ny[s] = h.ny[f]; h.s_area[s] = h.area[f];
       h.s_fcx[s] = h.fcx[f]; h.s_fcy[s] = h.fcy[f];
     }

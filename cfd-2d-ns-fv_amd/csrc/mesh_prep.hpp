@@ -64,15 +64,17 @@ struct HostMesh {
   std::vector<int32_t> chunk_off;     // nchunks + 1, in slots
   std::vector<uint8_t> deg;           // ncp (0 for padding)
   std::vector<int32_t> code;          // per slot, global neighbour index
-  std::vector<int32_t> lcode;         // per slot, block-local neighbour index
-  std::vector<int32_t> sface;         // per slot, (index in the block's face list << 1) | owner
+  // block-local codes are 16-bit: local indices stay below blk + max_halo and
+  // block face indices below max_bfaces, both bounded far below 2^14 by LDS
+  std::vector<int16_t> lcode;         // per slot: (block-local neighbour << 1) | owner, or ~kind
+  std::vector<int16_t> sface;         // per slot, (index in the block's face list << 1) | owner
   std::vector<double> s_nx, s_ny, s_area, s_fcx, s_fcy;   // per slot
   // per cell (new numbering, padded)
   std::vector<double> c_ccx, c_ccy, c_vol, c_ixx, c_iyy, c_ixy;
   // per block
   std::vector<int32_t> halo_ptr, halo_idx;            // halo cells (global new index)
   std::vector<int32_t> bf_ptr;                        // block face lists
-  std::vector<int32_t> bf_a, bf_b;                    // local owner; local neighbour or boundary code
+  std::vector<int16_t> bf_a, bf_b;                    // local owner; local neighbour or ~kind
   std::vector<double> bf_nx, bf_ny, bf_area, bf_fcx, bf_fcy;
   // FP32-mixed only: the single-precision geometry the FP32 kernels use,
   // each value computed in FP64 from the arrays above and rounded once
