@@ -332,6 +332,18 @@ def bench_single(args):
         out["fp64_sensitivity_floor"] = {k: float(f"{v:.3e}") for k, v in floor.items()}
         out["fp64_sensitivity_floor"]["how"] = ("FP64 from Q_inf(1+1e-7 u), u in [-1,1], "
                                                 "against FP64 from Q_inf, same steps")
+        if case.BC1 in ("SLIP_WALL", "NO_SLIP_WALL"):
+            # wall pressure and force coefficients (fv2d.analysis), FP32-mixed vs FP64,
+            # and the same comparison for the perturbed FP64 run
+            from fv2d import analysis
+            _, g32 = r["solver"].state_and_grad()
+            _, g64 = r64["solver"].state_and_grad()
+            _, gfl = fl.state_and_grad()
+            walls = analysis.compare_walls(grid, case, q, g32, q64, g64)
+            wfl = analysis.compare_walls(grid, case, fl.state(), gfl, q64, g64)
+            out["wall_vs_fp64"] = {k: float(f"{v:.4e}") for k, v in walls.items()}
+            out["wall_vs_fp64"]["floor_cd_rel_err"] = float(f"{wfl['cd_rel_err']:.3e}")
+            out["wall_vs_fp64"]["floor_cp_linf_err"] = float(f"{wfl['cp_linf_err']:.3e}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline([(grid, case)])
     return out
@@ -340,11 +352,11 @@ def bench_single(args):
 # ---------------------------------------------------------------------------
 # batch
 # ---------------------------------------------------------------------------
-def run_batch(prec: str, args, cases, nc):
+def run_batch(prec: str, args, cases, nc, opts):
     """Warm up, restart, then time one run to the residual drop (or
     args.steps steps) with retirement; then time each kernel kind per launch
     over all instances."""
-    b = Batch(cases, precision=PRECISIONS[prec], block_size=args.block_size)
+    b = Batch(cases, precision=PRECISIONS[prec], **opts)
     b.init()
     if args.warmup:
         b.step(args.warmup)
@@ -384,9 +396,18 @@ def bench_batch(args):
     nc = W.cells_of(plan)
     nf = np.array([i.grid.num_faces for i in plan], dtype=np.int64)
     log(f"{name}: {len(cases)} instances, {int(nc.sum())} cells")
+    precs = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
+    opts = {"block_size": args.block_size, "xcd_remap": args.xcd_remap}
+    tuned = {}
+    if args.autotune:
+        from fv2d import autotune
+        for prec in precs:
+            best, table = autotune.tune_batch(cases, PRECISIONS[prec], log=log)
+            tuned[prec] = {"choice": best,
+                           "ms_per_step": [{**o, "ms": round(ms, 3)} for o, ms in table]}
     runs = {}
-    for prec in (["fp32", "fp64"] if args.precision == "both" else [args.precision]):
-        runs[prec] = run_batch(prec, args, cases, nc)
+    for prec in precs:
+        runs[prec] = run_batch(prec, args, cases, nc, tuned[prec]["choice"] if tuned else opts)
         r = runs[prec]
         log(f"{prec} batch {r['value']:.4g} cell-stage/s, {r['wall']:.2f} s, "
             f"retired by drop {int((r['r1'] <= args.drop * r['r0']).sum())}")
@@ -394,16 +415,19 @@ def bench_batch(args):
     r = runs[head]
     algo = algorithmic_bytes(cases[0][1], int(nc.sum()), int(nf.sum()),
                              s=4 if head == "fp32" else 8)
-    child = ["--no-cpu", "--no-traffic", "--no-seq", "--steps", "4", "--warmup", "0",
-             "--kernel-reps", "3", "--config", "batch", "--instances", str(args.instances),
-             "--block-size", str(args.block_size), "--precision", head]
+    chosen = tuned[head]["choice"] if tuned else opts
+    child = ["--no-cpu", "--no-traffic", "--no-seq", "--no-autotune", "--steps", "4",
+             "--warmup", "0", "--kernel-reps", "3", "--config", "batch",
+             "--instances", str(args.instances), "--block-size", str(chosen["block_size"]),
+             "--xcd-remap", str(chosen["xcd_remap"]), "--precision", head]
     roof = roofline(r["per_launch"], algo, head, args, child)
     steps = r["steps"]
     cfg = {"workload": name, "instances": len(cases), "cells": int(nc.sum()),
            "faces": int(nf.sum()),
            "precision": "FP32-mixed (FP32 perturbation state)" if head == "fp32" else "FP64 plain",
            "drop": args.drop, "max_steps": args.steps, "check_every": args.check_every,
-           "block_size": args.block_size, "graph": "two-step hipGraph replay",
+           "launch_options": chosen, "autotune": tuned or None,
+           "graph": "two-step hipGraph replay",
            "steps_min_median_max": [int(steps.min()), int(np.median(steps)), int(steps.max())],
            "retired_by_drop": int((r["r1"] <= args.drop * r["r0"]).sum())}
     out = {"metric": METRIC, "value": r["value"], "unit": "cell-stage/s", "n_gpus": 1,
@@ -437,7 +461,7 @@ def bench_batch(args):
             pl2 = np.array([e["p_l2_rel"] for e in errs])
             # the FP64 scheme's own response to an FP32-rounding-sized perturbation
             log("FP64 sensitivity floor batch")
-            fb = Batch(cases, block_size=args.block_size)
+            fb = Batch(cases, **(tuned["fp64"]["choice"] if tuned else opts))
             fb.init([perturbed_freestream(g, c, seed=11 + i) for i, (g, c) in enumerate(cases)])
             fb.run_to_drop(args.drop, args.steps, args.check_every)
             fl_rho = []
@@ -491,6 +515,9 @@ def main():
     ap.add_argument("--variant", type=int, default=0,
                     help="single mesh: 0 default, 1 cell-parallel, 2 slot-parallel, 3 LDS-staged")
     ap.add_argument("--block-size", type=int, default=0, help="cells per mesh block")
+    ap.add_argument("--xcd-remap", type=int, default=1, help="batch: XCD-ordered blocks (1) or not")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="batch: skip the launch-shape autotuner (use --block-size/--xcd-remap)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run device-to-device copies of MB MiB (counter calibration)")
     ap.add_argument("--stream-probe", type=int, default=0, metavar="MB",
@@ -509,7 +536,7 @@ def main():
         sys.exit(2)
     batch = args.config == "batch"
     if args.steps is None:
-        args.steps = 1000 if batch else 500 if args.config == "airfoil" else 2000
+        args.steps = {"batch": 1000, "airfoil": 500, "cylinder": 20000}.get(args.config, 2000)
     if args.warmup is None:
         args.warmup = 10 if batch else 50
     if args.kernel_reps is None:

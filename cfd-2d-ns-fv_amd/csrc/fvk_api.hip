@@ -761,6 +761,95 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   return FVK_OK;
 }
 
+// ---------------------------------------------------------------------------
+// checkpoints: the complete device state, so that a run resumes exactly
+// ---------------------------------------------------------------------------
+struct CheckpointHeader {
+  char magic[8];               // "FVKCKPT1"
+  int32_t precision, instances;
+  int64_t ncp, nblocks, bytes;
+  int32_t flags[6];            // have_state, sentinel, fs_prims, res_valid, smax_cur, pad
+  int64_t steps_sum;
+};
+
+// the device arrays a checkpoint holds, with their byte sizes
+std::vector<std::pair<void*, size_t>> checkpoint_arrays(Engine* c) {
+  const size_t ncp = c->dm.ncp, t = c->tsize;
+  std::vector<std::pair<void*, size_t>> v;
+  for (int i = 0; i < 3; ++i) v.push_back({c->q[i], 4 * ncp * t});
+  for (int i = 0; i < 3; ++i) v.push_back({c->g[i], 8 * ncp * t});
+  v.push_back({c->sigma, ncp * t});
+  v.push_back({c->res, 4 * ncp * t});
+  v.push_back({c->smax, 2 * c->in.size() * sizeof(unsigned long long)});
+  return v;
+}
+
+int64_t checkpoint_bytes(Engine* c) {
+  int64_t n = sizeof(CheckpointHeader);
+  for (auto& a : checkpoint_arrays(c)) n += (int64_t)a.second;
+  n += (int64_t)c->in.size() * (1 + 8);   // active flags, step counters
+  return n;
+}
+
+int checkpoint_save(Engine* c, void* buf, int64_t bytes) {
+  const int64_t need = checkpoint_bytes(c);
+  if (!buf || bytes < need) return set_err(c, FVK_ERR_ARG, "checkpoint: buffer smaller than the checkpoint size");
+  FVK_HIP(c, hipStreamSynchronize(c->stream));
+  CheckpointHeader h{};
+  std::memcpy(h.magic, "FVKCKPT1", 8);
+  h.precision = c->fp32 ? FVK_FP32_MIXED : FVK_FP64;
+  h.instances = (int32_t)c->in.size();
+  h.ncp = c->dm.ncp;
+  h.nblocks = c->dm.nblocks;
+  h.bytes = need;
+  h.flags[0] = c->have_state;
+  h.flags[1] = c->sentinel;
+  h.flags[2] = c->fs_prims;
+  h.flags[3] = c->res_valid;
+  h.flags[4] = c->smax_cur;
+  unsigned char* p = static_cast<unsigned char*>(buf);
+  std::memcpy(p, &h, sizeof(h));
+  p += sizeof(h);
+  for (auto& a : checkpoint_arrays(c)) {
+    FVK_HIP(c, hipMemcpy(p, a.first, a.second, hipMemcpyDeviceToHost));
+    p += a.second;
+  }
+  for (const Instance& in : c->in) *p++ = in.active ? 1 : 0;
+  for (const Instance& in : c->in) {
+    std::memcpy(p, &in.steps, 8);
+    p += 8;
+  }
+  return FVK_OK;
+}
+
+int checkpoint_load(Engine* c, const void* buf, int64_t bytes) {
+  if (!buf || bytes < (int64_t)sizeof(CheckpointHeader)) return set_err(c, FVK_ERR_ARG, "checkpoint: buffer too small");
+  CheckpointHeader h;
+  std::memcpy(&h, buf, sizeof(h));
+  if (std::memcmp(h.magic, "FVKCKPT1", 8) != 0) return set_err(c, FVK_ERR_ARG, "checkpoint: not a checkpoint");
+  if (h.precision != (c->fp32 ? FVK_FP32_MIXED : FVK_FP64) || h.instances != (int32_t)c->in.size() ||
+      h.ncp != c->dm.ncp || h.nblocks != c->dm.nblocks || h.bytes != checkpoint_bytes(c) || bytes < h.bytes)
+    return set_err(c, FVK_ERR_ARG, "checkpoint: written for another mesh layout, instance count or precision");
+  drop_graph(c);
+  const unsigned char* p = static_cast<const unsigned char*>(buf) + sizeof(h);
+  for (auto& a : checkpoint_arrays(c)) {
+    FVK_HIP(c, hipMemcpy(a.first, p, a.second, hipMemcpyHostToDevice));
+    p += a.second;
+  }
+  for (Instance& in : c->in) in.active = *p++ != 0;
+  for (Instance& in : c->in) {
+    std::memcpy(&in.steps, p, 8);
+    p += 8;
+  }
+  c->have_state = h.flags[0] != 0;
+  c->sentinel = h.flags[1] != 0;
+  c->fs_prims = h.flags[2] != 0;
+  c->res_valid = h.flags[3] != 0;
+  c->smax_cur = h.flags[4];
+  if (c->batched) return upload_active(c);
+  return FVK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -944,6 +1033,22 @@ int fvk_time_steps(fvk_ctx* cx, int32_t nsteps, double out_ms[4], int64_t counts
   return rc;
 }
 
+int fvk_checkpoint_size(fvk_ctx* c, int64_t* bytes) {
+  if (!c || !bytes) return FVK_ERR_ARG;
+  *bytes = checkpoint_bytes(&c->e);
+  return FVK_OK;
+}
+
+int fvk_checkpoint_save(fvk_ctx* c, void* buf, int64_t bytes) {
+  if (!c) return FVK_ERR_ARG;
+  return checkpoint_save(&c->e, buf, bytes);
+}
+
+int fvk_checkpoint_load(fvk_ctx* c, const void* buf, int64_t bytes) {
+  if (!c) return FVK_ERR_ARG;
+  return checkpoint_load(&c->e, buf, bytes);
+}
+
 int fvk_time_kernels(fvk_ctx* c, int32_t reps, double avg_us[2]) {
   if (!c) return FVK_ERR_ARG;
   return time_kernels(&c->e, reps, avg_us);
@@ -1088,6 +1193,22 @@ int fvk_batch_use_graph(fvk_batch* b, int32_t on) {
   b->e.use_graph = on != 0;
   if (!on) drop_graph(&b->e);
   return FVK_OK;
+}
+
+int fvk_batch_checkpoint_size(fvk_batch* b, int64_t* bytes) {
+  if (!b || !bytes) return FVK_ERR_ARG;
+  *bytes = checkpoint_bytes(&b->e);
+  return FVK_OK;
+}
+
+int fvk_batch_checkpoint_save(fvk_batch* b, void* buf, int64_t bytes) {
+  if (!b) return FVK_ERR_ARG;
+  return checkpoint_save(&b->e, buf, bytes);
+}
+
+int fvk_batch_checkpoint_load(fvk_batch* b, const void* buf, int64_t bytes) {
+  if (!b) return FVK_ERR_ARG;
+  return checkpoint_load(&b->e, buf, bytes);
 }
 
 int fvk_batch_time_kernels(fvk_batch* b, int32_t reps, double avg_us[2]) {

@@ -1,0 +1,73 @@
+"""Wall quantities of a solution: pressure coefficient along the wall and the
+force coefficients, for comparing the FP32-mixed and FP64 solutions.
+
+The reference's own CL/CD (save_sol.cpp:20-75) truncates centroid and face
+coordinates to int before reconstructing the wall state, which makes its
+value meaningless on meshes whose cells are smaller than one length unit
+(the drop-in CLI reproduces it as it is).  This is the integrator the
+comparison uses instead: the wall state is the owner cell's state
+reconstructed to the face centre with its limited gradient (or the cell
+state at first order), pressure acts along the outward face normal, and at
+no-slip walls the shear is mu * |u_c| / d_n with d_n the centroid's distance
+to the face centre, as reconstruction.cpp:110-123 forms the wall gradient.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .lib import host_geometry
+
+
+def wall_state(grid, case, q, grad=None):
+    """Face index, conserved state at the face centre, and geometry of every
+    face of the BC1 group (file marker 0)."""
+    walls = np.flatnonzero((grid.face_c2 < 0) & (grid.bc_marker == 0))
+    geo = {k: host_geometry(grid, case, k)
+           for k in ("face_nx", "face_ny", "face_area", "face_centerx", "face_centery",
+                     "cell_centerx", "cell_centery")}
+    c = grid.face_c1[walls]
+    qf = q[:, c].copy()
+    dx = geo["face_centerx"][walls] - geo["cell_centerx"][c]
+    dy = geo["face_centery"][walls] - geo["cell_centery"][c]
+    if grad is not None:
+        qf += grad[0:4, c] * dx + grad[4:8, c] * dy
+    return walls, qf, q[:, c], geo, dx, dy
+
+
+def wall_coefficients(grid, case, q, grad=None, length: float = 1.0):
+    """(cp, x_wall, cl, cd): pressure coefficient at each wall face (ordered as
+    the faces), the wall faces' x, and the lift and drag coefficients per
+    unit span referred to `length`."""
+    walls, qf, qc, geo, dx, dy = wall_state(grid, case, q, grad)
+    g = case.gamma
+    rho, mx, my, E = qf
+    P = (E - 0.5 * (mx * mx + my * my) / rho) * (g - 1)
+    vinf2 = case.u ** 2 + case.v ** 2
+    qdyn = 0.5 * case.rho * vinf2
+    nx, ny, A = geo["face_nx"][walls], geo["face_ny"][walls], geo["face_area"][walls]
+    fx = np.sum(P * A * nx)
+    fy = np.sum(P * A * ny)
+    if case.eqn == "NS" and case.BC1 == "NO_SLIP_WALL":
+        dn = np.hypot(dx, dy)
+        u = qc[1] / qc[0]
+        v = qc[2] / qc[0]
+        # shear on the body: mu * (velocity at the centroid) / d_n, along it
+        fx += np.sum(case.mu * u / dn * A)
+        fy += np.sum(case.mu * v / dn * A)
+    vinf = np.sqrt(vinf2)
+    ex, ey = case.u / vinf, case.v / vinf
+    drag = fx * ex + fy * ey
+    lift = -fx * ey + fy * ex
+    cp = (P - case.P) / qdyn
+    return cp, geo["face_centerx"][walls], lift / (qdyn * length), drag / (qdyn * length)
+
+
+def compare_walls(grid, case, q32, g32, q64, g64):
+    """FP32-mixed against FP64 wall quantities."""
+    cp32, _, cl32, cd32 = wall_coefficients(grid, case, q32, g32)
+    cp64, _, cl64, cd64 = wall_coefficients(grid, case, q64, g64)
+    return {"cd_fp64": cd64, "cd_fp32": cd32, "cl_fp64": cl64, "cl_fp32": cl32,
+            "cd_rel_err": abs(cd32 - cd64) / abs(cd64),
+            "cp_l2_err": float(np.linalg.norm(cp32 - cp64) / np.sqrt(cp64.size)),
+            "cp_linf_err": float(np.max(np.abs(cp32 - cp64))),
+            "wall_faces": int(cp64.size)}

@@ -1,0 +1,42 @@
+"""Launch-shape autotuner for batches.
+
+Builds the batch once per candidate option set (cells per block, cell
+ordering, XCD block order), times a few steps of each after a warm-up, and
+returns the fastest.  Every candidate gives the same results bit for bit
+(the layout tests check that), so the choice only moves time.
+"""
+from __future__ import annotations
+
+import itertools
+import time
+
+from .lib import Batch
+
+# block sizes 128/256 (192 leaves 64-cell chunks straddling waves), RCB order
+# (RCM's long ranges give 4-9x larger halos), XCD-ordered blocks on or off
+DEFAULT_SPACE = {"block_size": [256, 128], "renumber": [1], "xcd_remap": [1, 0]}
+
+
+def candidates(space=None):
+    space = space or DEFAULT_SPACE
+    keys = sorted(space)
+    for vals in itertools.product(*(space[k] for k in keys)):
+        yield dict(zip(keys, vals))
+
+
+def tune_batch(cases, precision: int, space=None, warmup: int = 4, steps: int = 20, log=None):
+    """Fastest option set for this batch: (options, [(options, ms per step)])."""
+    table = []
+    for opt in candidates(space):
+        b = Batch(cases, precision=precision, **opt)
+        b.init()
+        b.step(warmup)
+        t0 = time.perf_counter()
+        b.step(steps)
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        b.close()
+        table.append((opt, ms))
+        if log:
+            log(f"autotune {opt}: {ms:.3f} ms/step")
+    best = min(table, key=lambda t: t[1])[0]
+    return best, table

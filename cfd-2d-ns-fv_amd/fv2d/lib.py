@@ -31,7 +31,9 @@ EXPORTS = ["fvk_default_options", "fvk_create", "fvk_destroy", "fvk_last_error",
            "fvk_batch_create", "fvk_batch_destroy", "fvk_batch_last_error", "fvk_batch_init",
            "fvk_batch_step", "fvk_batch_set_active", "fvk_batch_residual_l2",
            "fvk_batch_run_to_drop", "fvk_batch_get_state", "fvk_batch_get_info",
-           "fvk_batch_use_graph", "fvk_batch_time_kernels"]
+           "fvk_batch_use_graph", "fvk_batch_time_kernels",
+           "fvk_checkpoint_size", "fvk_checkpoint_save", "fvk_checkpoint_load",
+           "fvk_batch_checkpoint_size", "fvk_batch_checkpoint_save", "fvk_batch_checkpoint_load"]
 
 
 class FvkError(RuntimeError):
@@ -107,6 +109,10 @@ def lib():
         L.fvk_batch_get_info.argtypes = [vp, P(ctypes.c_int64)]
         L.fvk_batch_use_graph.argtypes = [vp, ctypes.c_int32]
         L.fvk_batch_time_kernels.argtypes = [vp, ctypes.c_int32, dp]
+        for pre in ("fvk_", "fvk_batch_"):
+            getattr(L, pre + "checkpoint_size").argtypes = [vp, P(ctypes.c_int64)]
+            getattr(L, pre + "checkpoint_save").argtypes = [vp, vp, ctypes.c_int64]
+            getattr(L, pre + "checkpoint_load").argtypes = [vp, vp, ctypes.c_int64]
         _lib = L
     return _lib
 
@@ -245,6 +251,19 @@ class Solver:
         self._check(lib().fvk_time_kernels(self.h, reps, _dp(out)))
         return {"pass_a": float(out[0]), "pass_b": float(out[1])}
 
+    def checkpoint(self) -> bytes:
+        """The complete device state (fvk_checkpoint_save)."""
+        n = ctypes.c_int64()
+        self._check(lib().fvk_checkpoint_size(self.h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(lib().fvk_checkpoint_save(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, data: bytes):
+        """Resume from a checkpoint of a solver with the same mesh and options."""
+        buf = ctypes.create_string_buffer(data, len(data))
+        self._check(lib().fvk_checkpoint_load(self.h, buf, len(data)))
+
     def time_steps(self, n: int):
         ms = np.zeros(4)
         cnt = (ctypes.c_int64 * 4)()
@@ -333,6 +352,17 @@ class Batch:
         self._check(lib().fvk_batch_get_info(self.h, a))
         keys = ["instances", "cells", "faces", "blocks", "active_blocks", "active_instances"]
         return dict(zip(keys, list(a)))
+
+    def checkpoint(self) -> bytes:
+        n = ctypes.c_int64()
+        self._check(lib().fvk_batch_checkpoint_size(self.h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(lib().fvk_batch_checkpoint_save(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, data: bytes):
+        buf = ctypes.create_string_buffer(data, len(data))
+        self._check(lib().fvk_batch_checkpoint_load(self.h, buf, len(data)))
 
     def use_graph(self, on: bool):
         self._check(lib().fvk_batch_use_graph(self.h, 1 if on else 0))

@@ -141,6 +141,19 @@ int fvk_get_info(fvk_ctx* ctx, int64_t info[6]);
  * [2] pass B (flux+residual+update), [3] other.  counts[0..3] launches. */
 int fvk_time_steps(fvk_ctx* ctx, int32_t nsteps, double out_ms[4], int64_t counts[4]);
 
+/* Checkpoints (an extension: the reference's restart file, post_process.cpp:13-28,
+ * holds 6-digit conserved variables and a restart re-arms the lagged limiter
+ * and wave speeds, main.cpp:82-118).  A checkpoint holds the complete device
+ * state -- the three stage states, the three limited gradients, the spectral
+ * sums and dt maxima, the residual and the lagged-data flags -- so that
+ * loading it into a context created from the same mesh and options and
+ * stepping on gives exactly the results of never having stopped.  The buffer
+ * is opaque host memory of fvk_checkpoint_size bytes (the device layout,
+ * so it is only valid for the same mesh, options and library build). */
+int fvk_checkpoint_size(fvk_ctx* ctx, int64_t* bytes);
+int fvk_checkpoint_save(fvk_ctx* ctx, void* buf, int64_t bytes);
+int fvk_checkpoint_load(fvk_ctx* ctx, const void* buf, int64_t bytes);
+
 /* Per-kernel timing: `reps` back-to-back launches of the stage-2 pass A
  * kernel (gradient + limiter) and of the stage-2 pass B kernel (flux +
  * residual + update), each between one HIP event pair on the context's
@@ -191,6 +204,10 @@ int fvk_batch_get_state(fvk_batch* b, int32_t i, double* q, double* grad);
 int fvk_batch_get_info(fvk_batch* b, int64_t info[6]);
 /* 0: launch every step's kernels one by one; 1 (default): replay the graph */
 int fvk_batch_use_graph(fvk_batch* b, int32_t on);
+/* checkpoints of a whole batch (with the active set and step counters) */
+int fvk_batch_checkpoint_size(fvk_batch* b, int64_t* bytes);
+int fvk_batch_checkpoint_save(fvk_batch* b, void* buf, int64_t bytes);
+int fvk_batch_checkpoint_load(fvk_batch* b, const void* buf, int64_t bytes);
 /* fvk_time_kernels over the active blocks of the whole batch */
 int fvk_batch_time_kernels(fvk_batch* b, int32_t reps, double avg_us[2]);
 

@@ -144,3 +144,22 @@ def test_batch_rejects_mixed_kernel_options():
         Batch([cases[0], (cases[1][0], inputs.Case(**{**cases[1][1].__dict__, "eqn": "NS"}))])
     with pytest.raises(FvkError, match="OPTION"):
         Batch(cases[:2], kernel_variant=1)
+
+
+def test_batch_checkpoint_resumes_exactly():
+    cases = euler_cases("GLOBAL")
+    a = Batch(cases, precision=FP32_MIXED)
+    a.init()
+    a.step(9)
+    flags = np.ones(len(cases), dtype=np.uint8)
+    flags[2] = 0
+    a.set_active(flags)
+    a.step(4)
+    ck = a.checkpoint()
+    a.step(8)
+    b = Batch(cases, precision=FP32_MIXED)
+    b.restore(ck)
+    assert b.info()["active_instances"] == len(cases) - 1
+    b.step(8)
+    for i in range(len(cases)):
+        assert same(a.state(i), b.state(i)), i

@@ -185,3 +185,22 @@ def test_airfoil_dual_generator():
     sx -= np.bincount(g.face_c2[inner], (nx * area)[inner], g.num_cells)
     sy -= np.bincount(g.face_c2[inner], (ny * area)[inner], g.num_cells)
     assert np.abs(sx).max() < 1e-15 and np.abs(sy).max() < 1e-15
+
+
+def test_wall_coefficients_of_a_uniform_flow():
+    """A uniform freestream exerts no net force on a closed body and has zero
+    pressure coefficient everywhere (fv2d.analysis)."""
+    from fv2d import analysis
+    g = gg.ogrid(64, 32)
+    u, v = inputs.mach_velocity(0.2, angle_deg=3.0)
+    case = inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE", eqn="NS", mu=1.225 * u / 100.0,
+                       BC1="NO_SLIP_WALL")
+    qinf = np.array([case.rho, case.rho * u, case.rho * v,
+                     case.P / (case.gamma - 1) + 0.5 * case.rho * (u * u + v * v)])
+    q = np.repeat(qinf[:, None], g.num_cells, axis=1)
+    cp, xw, cl, cd = analysis.wall_coefficients(g, inputs.Case(**{**case.__dict__, "mu": 0.0}), q)
+    assert cp.size == 64 and np.abs(cp).max() < 1e-9
+    assert abs(cl) < 1e-9 and abs(cd) < 1e-9
+    # with viscosity the no-slip shear of a uniform stream is pure drag, along the flow
+    cp, xw, cl, cd = analysis.wall_coefficients(g, case, q)
+    assert cd > 0 and abs(cl) < 1e-9 * cd + 1e-12
