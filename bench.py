@@ -2,7 +2,8 @@
 the FP32-mixed precision form with the FP64 path timed beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--config batch|ramp|channel|cylinder] [--precision both|fp32|fp64]
+                    [--config batch|ramp|channel|cylinder|airfoil]
+                    [--precision both|fp32|fp64] [--no-autotune]
 
 A "step" is one SSP-RK3 iteration (3 residual evaluations + stage updates)
 over every active cell, with all inputs resident in HBM.  Prints one JSON
@@ -16,8 +17,15 @@ stream has drained).
   FP32-mixed batch: the FP64 batch, and 256 sequential FP64 runs of the plain
   single-instance path (each instance timed alone over a few steps, priced at
   the steps its FP64 run took); per-instance FP32-mixed errors against FP64.
---config ramp: the single 512x256 ramp mesh (196,608 cells), FP32-mixed and
-  FP64, K steps each.
+--config ramp|channel|cylinder|airfoil: one mesh (the 512x256 ramp, the
+  64x32 bump channel, the 1024x512 NS O-grid, the 7.8M-cell Delaunay-dual
+  airfoil), FP32-mixed and FP64, K steps each, with the FP64 sensitivity
+  floor and (wall cases) wall pressure and force coefficients.
+
+Before the timed runs a launch-shape autotuner times a few steps of each
+candidate (cells per block, XCD block order) per precision and keeps the
+fastest; every candidate gives the same bits.  --no-autotune uses
+--block-size / --xcd-remap (default: the library's automatic choice).
 
 `roofline` prices the dominant kernel's algorithmic bytes per launch against
 its average launch time measured with HIP events on the library's stream;
@@ -183,9 +191,9 @@ def _pmc_mean_kib(counter: str, child_args, match, timeout: int = 300):
     cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
            sys.executable, os.path.abspath(__file__)] + child_args
     env = dict(os.environ, TMPDIR="/tmp")
-    try:
-        subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout,
-                       check=True)
+    try:   # the child's log lines pass through to this process's stderr
+        subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, text=True,
+                       timeout=timeout, check=True)
     except (subprocess.SubprocessError, OSError):
         return None
     vals = []
@@ -273,11 +281,10 @@ def roofline(per_launch, algo, head, args, child):
 # ---------------------------------------------------------------------------
 # single mesh
 # ---------------------------------------------------------------------------
-def run_single(prec: str, args, grid, case):
+def run_single(prec: str, args, grid, case, opts):
     """Warm up, time args.steps steps, then time each kernel kind per launch."""
     nc = grid.num_cells
-    s = Solver(grid, case, kernel_variant=args.variant, block_size=args.block_size,
-               precision=PRECISIONS[prec])
+    s = Solver(grid, case, kernel_variant=args.variant, precision=PRECISIONS[prec], **opts)
     s.init_freestream()
     s.step(args.warmup)
     t0 = time.perf_counter()
@@ -290,24 +297,48 @@ def run_single(prec: str, args, grid, case):
 
 def bench_single(args):
     log(f"building the {args.config} mesh")
-    name, grid, case = (W.airfoil(log=log) if args.config == "airfoil" else W.SINGLE[args.config]())
+    if args.config == "airfoil":
+        name, grid, case = W.airfoil(log=log, cache=args.grid_npz)
+    else:
+        name, grid, case = W.SINGLE[args.config]()
     log(f"{grid.num_cells} cells, {grid.num_faces} faces")
     nc, nf = grid.num_cells, grid.num_faces
+    precs = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
+    opts = {"block_size": args.block_size, "xcd_remap": args.xcd_remap}
+    tuned = {}
+    if args.autotune and args.variant in (0, 3):
+        from fv2d import autotune
+        for prec in precs:
+            best, table = autotune.tune_single(grid, case, PRECISIONS[prec], log=log,
+                                               kernel_variant=args.variant)
+            tuned[prec] = {"choice": best,
+                           "ms_per_step": [{**o, "ms": round(ms, 3)} for o, ms in table]}
     runs = {}
-    for prec in (["fp32", "fp64"] if args.precision == "both" else [args.precision]):
-        runs[prec] = run_single(prec, args, grid, case)
+    for prec in precs:
+        runs[prec] = run_single(prec, args, grid, case, tuned[prec]["choice"] if tuned else opts)
         log(f"{prec} {runs[prec]['value']:.4g} cell-stage/s")
     head = "fp32" if "fp32" in runs else "fp64"
     r = runs[head]
     algo = algorithmic_bytes(case, nc, nf, s=4 if head == "fp32" else 8)
-    child = ["--no-cpu", "--no-traffic", "--steps", "5", "--warmup", "2", "--kernel-reps", "5",
-             "--config", args.config, "--variant", str(args.variant),
-             "--block-size", str(args.block_size), "--precision", head]
+    chosen = r["solver"].launch_options()
+    child = ["--no-cpu", "--no-traffic", "--no-autotune", "--steps", "5", "--warmup", "2",
+             "--kernel-reps", "5", "--config", args.config, "--variant", str(args.variant),
+             "--block-size", str(chosen["block_size"]), "--xcd-remap", str(chosen["xcd_remap"]),
+             "--precision", head]
+    tmp = None
+    if args.config == "airfoil" and not args.no_traffic:
+        # the counter runs load this mesh instead of generating it again
+        tmp = tempfile.mkdtemp(dir="/tmp")
+        grid.save_npz(os.path.join(tmp, "grid.npz"))
+        child += ["--grid-npz", os.path.join(tmp, "grid.npz")]
     roof = roofline(r["per_launch"], algo, head, args, child)
+    if tmp:
+        import shutil
+        shutil.rmtree(tmp, ignore_errors=True)
     q = r["solver"].state()
     cfg = {"workload": name, "cells": nc, "faces": nf,
            "precision": "FP32-mixed (FP32 perturbation state)" if head == "fp32" else "FP64 plain",
-           "kernel_variant": args.variant, "block_size": args.block_size,
+           "launch_options": chosen, "autotune": tuned or None,
            "state_finite": bool(np.isfinite(q).all())}
     out = {"metric": METRIC, "value": r["value"], "unit": "cell-stage/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup,
@@ -415,7 +446,7 @@ def bench_batch(args):
     r = runs[head]
     algo = algorithmic_bytes(cases[0][1], int(nc.sum()), int(nf.sum()),
                              s=4 if head == "fp32" else 8)
-    chosen = tuned[head]["choice"] if tuned else opts
+    chosen = r["batch"].launch_options()
     child = ["--no-cpu", "--no-traffic", "--no-seq", "--no-autotune", "--steps", "4",
              "--warmup", "0", "--kernel-reps", "3", "--config", "batch",
              "--instances", str(args.instances), "--block-size", str(chosen["block_size"]),
@@ -514,10 +545,16 @@ def main():
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
     ap.add_argument("--variant", type=int, default=0,
                     help="single mesh: 0 default, 1 cell-parallel, 2 slot-parallel, 3 LDS-staged")
-    ap.add_argument("--block-size", type=int, default=0, help="cells per mesh block")
-    ap.add_argument("--xcd-remap", type=int, default=1, help="batch: XCD-ordered blocks (1) or not")
+    ap.add_argument("--block-size", type=int, default=0,
+                    help="cells per mesh block (0: the library's automatic choice)")
+    ap.add_argument("--xcd-remap", type=int, default=-1,
+                    help="XCD block order: -1 automatic, 0 off, 1 contiguous ranges, "
+                         "G>1 runs of G blocks")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="batch: skip the launch-shape autotuner (use --block-size/--xcd-remap)")
+                    help="skip the launch-shape autotuner (use --block-size/--xcd-remap)")
+    ap.add_argument("--grid-npz", default=None,
+                    help="airfoil: load the generated mesh from this file (Grid.save_npz) "
+                         "instead of generating it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run device-to-device copies of MB MiB (counter calibration)")
     ap.add_argument("--stream-probe", type=int, default=0, metavar="MB",
@@ -531,7 +568,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(instances or mesh partitions would shard across GPUs, see DESIGN.md)",
+              "(instances or mesh partitions would shard across GPUs)",
               file=sys.stderr)
         sys.exit(2)
     batch = args.config == "batch"

@@ -22,7 +22,7 @@ struct Inputs {
   double length = 1.0;
   int BC1 = 0, BC2 = 0;
   // optional trailing [gpu] section (read only after the nine standard ones)
-  int renumber = 1, block_size = 0, xcd_remap = 1, kernel_variant = 0;
+  int renumber = 1, block_size = 0, xcd_remap = -1, kernel_variant = 0;
   int precision = FVK_FP64;    // [gpu] precision = FP64 | FP32_MIXED
 };
 

@@ -46,6 +46,10 @@
 #ifndef FVK_FAST32
 #define FVK_FAST32 1
 #endif
+// FP32-mixed NS: 1 = the viscous face flux in FP64, 0 = in FP32.
+#ifndef FVK_NS64
+#define FVK_NS64 0
+#endif
 
 namespace fvk {
 namespace {
@@ -66,9 +70,17 @@ __device__ __forceinline__ T max2(T a, T b) { return (a < b) ? b : a; }  // std:
 // XCD-aware block order: blocks are dealt to the 8 XCDs round-robin, so block
 // b runs on XCD b%8; give each XCD one contiguous range of the logical blocks
 // so that the neighbour gathers of its blocks hit its own L2.
+// A negative argument -G asks for runs of G consecutive logical blocks per
+// XCD, the runs dealt to the XCDs in turn: neighbour blocks still share an L2
+// while every XCD walks the whole block range at the same pace.
 __device__ __forceinline__ int logical_block(int per_xcd) {
   const int b = blockIdx.x;
-  return per_xcd > 0 ? (b & 7) * per_xcd + (b >> 3) : b;
+  if (per_xcd > 0) return (b & 7) * per_xcd + (b >> 3);
+  if (per_xcd < 0) {
+    const int g = -per_xcd, j = b >> 3;
+    return ((j / g) * 8 + (b & 7)) * g + j % g;
+  }
+  return b;
 }
 
 // Mesh block and instance of logical block lb: through the active-block list
@@ -455,6 +467,89 @@ __device__ __forceinline__ void side_flux_delta(const FlowT<T>& fl, const T d[4]
 #endif
 }
 
+// conserved component k, in type U, of a stored state value d
+template <typename U, typename T>
+__device__ __forceinline__ U conserved(const FlowT<T>& fl, int k, T d) {
+  if constexpr (!kDelta<T>) return d;
+  else if constexpr (std::is_same<U, T>::value) return fl.qinf[k] + d;
+  else return fl.qinf64[k] + U(d);
+}
+
+// Viscous face flux Fv (reconstruction.cpp:83-164, class_q.hpp:163-169,
+// class_f.hpp:52-59) in the arithmetic type U from the face states L, R
+// (stored-state form) and the two cells: U = T is the path's own arithmetic
+// (for FP64, the reference's expressions in its order); the FP32-mixed path
+// may take it in FP64 (FVK_NS64), where the temperature gradient's two
+// products cancel most of each other.
+template <typename U, typename T>
+__device__ __forceinline__ void viscous_flux(const StageArgsT<T>& a, const FlowT<T>& fl,
+                                             const CellQG<T>& c1, const CellQG<T>* c2,
+                                             const FaceGeo<T>& g, const T L[4], const T R[4],
+                                             U Fv[4]) {
+  constexpr bool kSame = std::is_same<U, T>::value;
+  const U gm1 = kSame ? U(fl.gm1) : U(fl.gm1_64);
+  const U nx = U(g.nx), ny = U(g.ny);
+  // face-averaged state, reconstruction.cpp:86-92
+  U fa[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) fa[k] = U(0.5) * (conserved<U>(fl, k, L[k]) + conserved<U>(fl, k, R[k]));
+  const U frho = fa[0], fE = fa[3];
+  const U fu = fa[1] / frho, fv = fa[2] / frho;
+  const U fP = (fE - (U(0.5) * frho * (fu * fu + fv * fv))) * gm1;
+  U ux, uy, vx, vy, Px = 0, Py = 0, rhox = 0, rhoy = 0;
+  if (!c2) {              // no-slip wall, reconstruction.cpp:110-123
+    U cu, cv;
+    if (a.fs_prims) {
+      cu = U(fl.u);
+      cv = U(fl.v);
+    } else {
+      const U crho = conserved<U>(fl, 0, c1.q[0]);
+      cu = conserved<U>(fl, 1, c1.q[1]) / crho;
+      cv = conserved<U>(fl, 2, c1.q[2]) / crho;
+    }
+    const U dn = U(g.dist);
+    const U dudn = -cu / dn;
+    const U dvdn = -cv / dn;
+    ux = dudn * nx;
+    uy = dudn * ny;
+    vx = dvdn * nx;
+    vy = dvdn * ny;
+  } else {                // interior, reconstruction.cpp:124-161
+    const U dist = U(g.dist);                        // face_dist
+    const U lx = U(g.lx), ly = U(g.ly);
+    const U mx = -ny, my = nx;                       // unit p1->p2 (readmesh.cpp:400-406)
+    U ddx[4], ddy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const U dl = (U(c2->q[k]) - U(c1.q[k])) / fabs(dist);
+      const U dm = U(0.5) * (U(c1.gx[k]) * mx + U(c1.gy[k]) * my + U(c2->gx[k]) * mx +
+                             U(c2->gy[k]) * my);
+      ddy[k] = dl * ly + dm * ly;
+      ddx[k] = dl * lx + dm * lx;
+    }
+    rhox = ddx[0];
+    ux = ddx[1] - rhox * fu / frho;
+    vx = ddx[2] - rhox * fv / frho;
+    rhoy = ddy[0];
+    uy = ddy[1] - rhoy * fu / frho;
+    vy = ddy[2] - rhoy * fu / frho;   // the reference uses u here
+    Px = gm1 * (ddx[3] - U(0.5) * rhox * (fu * fu + fv * fv) - frho * (fu * ux + fv * vx));
+    Py = gm1 * (ddy[3] - U(0.5) * rhoy * (fu * fu + fv * fv) - frho * (fu * uy + fv * vy));
+  }
+  const U mu = kSame ? U(fl.mu) : U(fl.mu64);
+  const U Rg = kSame ? U(fl.Rgas) : U(fl.Rgas64);
+  const U kc = kSame ? U(fl.k) : U(fl.k64);
+  const U txx = 2 * mu * (ux - (ux + vy) / 3);
+  const U txy = mu * (uy + vx);
+  const U tyy = 2 * mu * (vy - (ux + vy) / 3);
+  const U dTdx = (Px * frho - rhox * fP) / (Rg * frho * frho);
+  const U dTdy = (Py * frho - rhoy * fP) / (Rg * frho * frho);
+  Fv[0] = 0;
+  Fv[1] = txx * nx + txy * ny;
+  Fv[2] = txy * nx + tyy * ny;
+  Fv[3] = nx * (fu * txx + fv * txy + kc * dTdx) + ny * (fu * txy + fv * tyy + kc * dTdy);
+}
+
 // Pass B: one face with owner c1 and neighbour c2 (interior) or a BC ghost of
 // the given kind.  x = (Fv - F)*A, the owner's residual term (the neighbour's
 // is -x, res.cpp:81-101); sterm = (|Vn_avg| + c_avg) * A (ssp_rk3.cpp:32-34).
@@ -564,54 +659,17 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
   const T c_avg = (cL + cR) / 2;
   T Fv[4] = {0, 0, 0, 0};
   if (NS && ORDER == 2 && (c2 || kind == 3)) {
-    // face-averaged state, reconstruction.cpp:86-92
-    T fa[4];
+#if FVK_NS64
+    if constexpr (kDelta<T>) {
+      double Fd[4];
+      viscous_flux<double>(a, fl, c1, c2, g, L, R, Fd);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) fa[k] = T(0.5) * (absq(fl, k, L[k]) + absq(fl, k, R[k]));
-    const T frho = fa[0], fE = fa[3];
-    const T fu = fa[1] / frho, fv = fa[2] / frho;
-    const T fP = (fE - (T(0.5) * frho * (fu * fu + fv * fv))) * gm1;
-    T ux, uy, vx, vy, Px = 0, Py = 0, rhox = 0, rhoy = 0;
-    if (!c2) {              // no-slip wall, reconstruction.cpp:110-123
-      T crho, cu, cv, cP;
-      cell_prim(c1.q, fl, a.fs_prims, crho, cu, cv, cP);
-      const T dn = g.dist;
-      const T dudn = -cu / dn;
-      const T dvdn = -cv / dn;
-      ux = dudn * nx;
-      uy = dudn * ny;
-      vx = dvdn * nx;
-      vy = dvdn * ny;
-    } else {                // interior, reconstruction.cpp:124-161
-      const T dist = g.dist;                           // face_dist
-      const T lx = g.lx, ly = g.ly;
-      const T mx = -ny, my = nx;                       // unit p1->p2 (readmesh.cpp:400-406)
-      T ddx[4], ddy[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const T dl = (c2->q[k] - c1.q[k]) / fabs(dist);
-        const T dm = T(0.5) * (c1.gx[k] * mx + c1.gy[k] * my + c2->gx[k] * mx + c2->gy[k] * my);
-        ddy[k] = dl * ly + dm * ly;
-        ddx[k] = dl * lx + dm * lx;
-      }
-      rhox = ddx[0];
-      ux = ddx[1] - rhox * fu / frho;
-      vx = ddx[2] - rhox * fv / frho;
-      rhoy = ddy[0];
-      uy = ddy[1] - rhoy * fu / frho;
-      vy = ddy[2] - rhoy * fu / frho;   // the reference uses u here
-      Px = gm1 * (ddx[3] - T(0.5) * rhox * (fu * fu + fv * fv) - frho * (fu * ux + fv * vx));
-      Py = gm1 * (ddy[3] - T(0.5) * rhoy * (fu * fu + fv * fv) - frho * (fu * uy + fv * vy));
+      for (int k = 0; k < 4; ++k) Fv[k] = T(Fd[k]);
+    } else
+#endif
+    {
+      viscous_flux<T>(a, fl, c1, c2, g, L, R, Fv);
     }
-    const T mu = fl.mu;
-    const T txx = 2 * mu * (ux - (ux + vy) / 3);
-    const T txy = mu * (uy + vx);
-    const T tyy = 2 * mu * (vy - (ux + vy) / 3);
-    const T dTdx = (Px * frho - rhox * fP) / (fl.Rgas * frho * frho);
-    const T dTdy = (Py * frho - rhoy * fP) / (fl.Rgas * frho * frho);
-    Fv[1] = txx * nx + txy * ny;
-    Fv[2] = txy * nx + tyy * ny;
-    Fv[3] = nx * (fu * txx + fv * txy + fl.k * dTdx) + ny * (fu * txy + fv * tyy + fl.k * dTdy);
   }
   const T lam = (fabs(vn_avg) + c_avg) / 2;
 #pragma unroll
@@ -1396,9 +1454,13 @@ __global__ void k_fill(T* p, long long n, T v) {
 // grid size and XCD range length of a launch over `nlogical` blocks
 template <typename T>
 void shape(const StageArgsT<T>& a, int nlogical, int* grid, int* per_xcd) {
-  if (a.xcd_remap) {
+  if (a.xcd_remap == 1) {
     *per_xcd = (nlogical + 7) / 8;
     *grid = 8 * *per_xcd;
+  } else if (a.xcd_remap > 1) {
+    const int g = a.xcd_remap;
+    *per_xcd = -g;
+    *grid = (nlogical + 8 * g - 1) / (8 * g) * (8 * g);
   } else {
     *per_xcd = 0;
     *grid = nlogical;

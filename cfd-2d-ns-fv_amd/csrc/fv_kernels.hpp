@@ -75,6 +75,8 @@ struct FlowT {
   T rho, u, v, P;              // freestream primitives
   T vn0, c0;                   // initial face |Vn_avg| and c_avg (class_q.hpp:61-62)
   T sent[4];                   // the -101 initial face state, in stored-state units
+  double qinf64[4];            // FP64 copies for steps an FP32 path takes in FP64
+  double gm1_64, Rgas64, mu64, k64;
   int32_t squeeze;             // limiter SQUEEZE (1) or none (0)
   int32_t first_block;         // first mesh block of the instance
 };
@@ -96,8 +98,8 @@ struct StageArgsT {
   int32_t sentinel;            // lagged face states are the -101 initial values
   int32_t fs_prims;            // stage input's primitives are the freestream ones
   int32_t store_res;           // write the residual (stage 3 of a reported step)
-  int32_t xcd_remap;           // give each XCD a contiguous range of blocks
-  int32_t xcd_blocks;          // blocks per XCD range (set by the launcher)
+  int32_t xcd_remap;           // XCD block order: 0 off, 1 contiguous ranges, G>1 runs of G
+  int32_t xcd_blocks;          // blocks per XCD range, or -G (set by the launcher)
   int32_t variant;             // kVariantCell, kVariantSlot or kVariantLds
   const T* qc;                 // stage input state (4 per cell)
   const T* qp;                 // previous stage's input state (lagged limiter)

@@ -88,6 +88,20 @@ struct fvk_batch {
 
 namespace {
 
+// automatic launch options (measured on MI355X: 2k-200k-cell single meshes,
+// a 7.8M-cell mesh and an 11.3M-cell batch of 256 meshes)
+constexpr int64_t kAutoLargeCells = 2000000;    // FP32-mixed: 128-cell blocks above this
+constexpr int32_t kAutoContiguousBlocks = 4096; // contiguous XCD ranges up to this many blocks
+
+// instance i's options with the engine-wide launch choices resolved
+fvk_options resolved_options(const Engine& c, int32_t i) {
+  fvk_options o = c.in[i].opt;
+  o.block_size = c.dm.blk;
+  o.xcd_remap = c.opt.xcd_remap;
+  o.kernel_variant = c.variant;
+  return o;
+}
+
 int set_err(Engine* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   else g_create_error = msg;
@@ -169,6 +183,7 @@ int check_options(const fvk_options& o, std::string* msg) {
   if (o.precision != FVK_FP64 && o.precision != FVK_FP32_MIXED) { *msg = "precision must be FVK_FP64 or FVK_FP32_MIXED"; return FVK_ERR_OPTION; }
   if (o.kernel_variant < 0 || o.kernel_variant > 3) { *msg = "kernel_variant must be 0, 1, 2 or 3"; return FVK_ERR_OPTION; }
   if (o.renumber < 0 || o.renumber > 2) { *msg = "renumber must be 0, 1 or 2"; return FVK_ERR_OPTION; }
+  if (o.xcd_remap < -1 || o.xcd_remap > 4096) { *msg = "xcd_remap must be -1 (auto), 0, 1 or a run length 2..4096"; return FVK_ERR_OPTION; }
   if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64)) {
     *msg = "block_size must be 0 or a multiple of 64 in [64, 256]";
     return FVK_ERR_OPTION;
@@ -220,6 +235,11 @@ fvk::FlowT<T> make_flow(const fvk_options& o, bool delta, int32_t first_block) {
   fl.P = T(o.P);
   fl.vn0 = T(std::sqrt(o.v * o.v + o.u * o.u));
   fl.c0 = T(std::sqrt(o.gamma * (o.P / o.rho)));
+  for (int k = 0; k < 4; ++k) fl.qinf64[k] = qinf[k];
+  fl.gm1_64 = o.gamma - 1;
+  fl.Rgas64 = o.R;
+  fl.mu64 = o.mu;
+  fl.k64 = (o.mu * Cp) / o.Pr;
   fl.squeeze = o.limiter == 2;
   fl.first_block = first_block;
   return fl;
@@ -592,7 +612,16 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_err(nullptr, FVK_ERR_HIP, "no HIP device");
-  c->opt = opts[0];
+  // automatic launch options, resolved once for the whole solve (a batch
+  // counts all its instances): 128-cell blocks for FP64 and for large solves,
+  // where the smaller LDS footprint keeps more blocks resident per CU
+  std::vector<fvk_options> ro(opts, opts + n);
+  int64_t total_cells = 0;
+  for (int32_t i = 0; i < n; ++i) total_cells += meshes[i].num_cells;
+  for (fvk_options& o : ro)
+    if (o.block_size == 0)
+      o.block_size = (o.precision == FVK_FP64 || total_cells > kAutoLargeCells) ? 128 : 256;
+  c->opt = ro[0];
   c->batched = batched;
   c->in.resize(n);
   {
@@ -601,8 +630,8 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
     std::atomic<int32_t> next{0};
     auto work = [&]() {
       for (int32_t i = next++; i < n; i = next++) {
-        c->in[i].opt = opts[i];
-        rcs[i] = fvk::prepare_mesh(meshes[i], opts[i], &c->in[i].hm, &errs[i]);
+        c->in[i].opt = ro[i];
+        rcs[i] = fvk::prepare_mesh(meshes[i], ro[i], &c->in[i].hm, &errs[i]);
       }
     };
     const int hw = (int)std::thread::hardware_concurrency();
@@ -638,7 +667,7 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   const fvk_options& o0 = opts[0];
   c->fp32 = o0.precision == FVK_FP32_MIXED;
   c->tsize = c->fp32 ? sizeof(float) : sizeof(double);
-  c->block = o0.block_size ? o0.block_size : 256;
+  c->block = o0.block_size ? o0.block_size : 256;   // thread block of the cell-parallel shape
   c->variant = o0.kernel_variant ? o0.kernel_variant : fvk::kVariantLds;
   hipError_t e = hipStreamCreate(&c->stream);
   if (e != hipSuccess) {
@@ -658,6 +687,10 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
     d.max_halo = std::max(d.max_halo, in.hm.max_halo);
     d.max_bfaces = std::max(d.max_bfaces, in.hm.max_bfaces);
   }
+  // XCD order: contiguous ranges while a launch is a few blocks per CU (every
+  // block is resident at once), runs of 8 blocks dealt in turn beyond that
+  // (ranges of unequal work would leave XCDs idle at the end)
+  if (c->opt.xcd_remap < 0) c->opt.xcd_remap = d.nblocks <= kAutoContiguousBlocks ? 1 : 8;
   using HM = fvk::HostMesh;
   int rc;
   if ((rc = upload_cat<int32_t>(c, &HM::chunk_off, &d.chunk_off,
@@ -877,7 +910,7 @@ void fvk_default_options(fvk_options* o) {
   o->precision = FVK_FP64;
   o->renumber = 1;
   o->block_size = 0;
-  o->xcd_remap = 1;
+  o->xcd_remap = -1;
   o->kernel_variant = 0;
 }
 
@@ -991,6 +1024,12 @@ int fvk_get_info(fvk_ctx* c, int64_t info[6]) {
   info[3] = h.chunk_off.empty() ? 0 : h.chunk_off.back();
   info[4] = h.max_deg;
   info[5] = h.num_wall;
+  return FVK_OK;
+}
+
+int fvk_get_options(fvk_ctx* c, fvk_options* out) {
+  if (!c || !out) return FVK_ERR_ARG;
+  *out = resolved_options(c->e, 0);
   return FVK_OK;
 }
 
@@ -1185,6 +1224,13 @@ int fvk_batch_get_info(fvk_batch* b, int64_t info[6]) {
   info[3] = c.dm.nblocks;
   info[4] = c.nactive;
   info[5] = active;
+  return FVK_OK;
+}
+
+int fvk_batch_get_options(fvk_batch* b, int32_t i, fvk_options* out) {
+  if (!b || !out) return FVK_ERR_ARG;
+  if (i < 0 || i >= (int32_t)b->e.in.size()) return set_err(&b->e, FVK_ERR_ARG, "instance out of range");
+  *out = resolved_options(b->e, i);
   return FVK_OK;
 }
 

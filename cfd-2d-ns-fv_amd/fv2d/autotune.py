@@ -1,6 +1,6 @@
-"""Launch-shape autotuner for batches.
+"""Launch-shape autotuner for batches and single meshes.
 
-Builds the batch once per candidate option set (cells per block, cell
+Builds the solve once per candidate option set (cells per block, cell
 ordering, XCD block order), times a few steps of each after a warm-up, and
 returns the fastest.  Every candidate gives the same results bit for bit
 (the layout tests check that), so the choice only moves time.
@@ -10,11 +10,12 @@ from __future__ import annotations
 import itertools
 import time
 
-from .lib import Batch
+from .lib import Batch, Solver
 
 # block sizes 128/256 (192 leaves 64-cell chunks straddling waves), RCB order
-# (RCM's long ranges give 4-9x larger halos), XCD-ordered blocks on or off
-DEFAULT_SPACE = {"block_size": [256, 128], "renumber": [1], "xcd_remap": [1, 0]}
+# (RCM's long ranges give 4-9x larger halos); XCD order: runs of 8 blocks per
+# XCD, none, or one contiguous range per XCD
+DEFAULT_SPACE = {"block_size": [128, 256], "renumber": [1], "xcd_remap": [8, 0, 1]}
 
 
 def candidates(space=None):
@@ -35,6 +36,25 @@ def tune_batch(cases, precision: int, space=None, warmup: int = 4, steps: int = 
         b.step(steps)
         ms = (time.perf_counter() - t0) * 1e3 / steps
         b.close()
+        table.append((opt, ms))
+        if log:
+            log(f"autotune {opt}: {ms:.3f} ms/step")
+    best = min(table, key=lambda t: t[1])[0]
+    return best, table
+
+
+def tune_single(grid, case, precision: int, space=None, warmup: int = 4, steps: int = 20,
+                log=None, **fixed):
+    """Fastest option set for one mesh through fvk_ctx: (options, table)."""
+    table = []
+    for opt in candidates(space):
+        s = Solver(grid, case, precision=precision, **fixed, **opt)
+        s.init_freestream()
+        s.step(warmup)
+        t0 = time.perf_counter()
+        s.step(steps)
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        s.close()
         table.append((opt, ms))
         if log:
             log(f"autotune {opt}: {ms:.3f} ms/step")

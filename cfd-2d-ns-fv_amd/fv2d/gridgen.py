@@ -77,6 +77,19 @@ class Grid:
         are substituted: >=0 neighbour, -1 BC1 group, -2 BC2 group, -3/-4 raw."""
         return np.where(self.face_c2 >= 0, self.face_c2, self.bc_marker - 1).astype(np.int32)
 
+    def save_npz(self, path: str) -> None:
+        """Binary copy of the arrays (a cache for large generated meshes)."""
+        np.savez(path, x=self.x, y=self.y, face_p1=self.face_p1, face_p2=self.face_p2,
+                 face_c1=self.face_c1, face_c2=self.face_c2, bc_marker=self.bc_marker,
+                 num_cells=np.int64(self.num_cells), name=np.array(self.name))
+
+    @classmethod
+    def load_npz(cls, path: str) -> "Grid":
+        with np.load(path, allow_pickle=False) as z:
+            return cls(x=z["x"], y=z["y"], face_p1=z["face_p1"], face_p2=z["face_p2"],
+                       face_c1=z["face_c1"], face_c2=z["face_c2"], bc_marker=z["bc_marker"],
+                       num_cells=int(z["num_cells"]), name=str(z["name"]))
+
     def write(self, path: str) -> None:
         """Write the native text format (readmesh.cpp:31-85), round-trip exact."""
         with open(path, "w") as f:

@@ -27,11 +27,11 @@ FP32_MIXED = 1
 EXPORTS = ["fvk_default_options", "fvk_create", "fvk_destroy", "fvk_last_error",
            "fvk_init_freestream", "fvk_set_state", "fvk_step", "fvk_get_state",
            "fvk_get_residual", "fvk_residual_l2", "fvk_get_geometry", "fvk_geometry",
-           "fvk_get_info", "fvk_time_steps", "fvk_time_kernels",
+           "fvk_get_info", "fvk_get_options", "fvk_time_steps", "fvk_time_kernels",
            "fvk_batch_create", "fvk_batch_destroy", "fvk_batch_last_error", "fvk_batch_init",
            "fvk_batch_step", "fvk_batch_set_active", "fvk_batch_residual_l2",
            "fvk_batch_run_to_drop", "fvk_batch_get_state", "fvk_batch_get_info",
-           "fvk_batch_use_graph", "fvk_batch_time_kernels",
+           "fvk_batch_get_options", "fvk_batch_use_graph", "fvk_batch_time_kernels",
            "fvk_checkpoint_size", "fvk_checkpoint_save", "fvk_checkpoint_load",
            "fvk_batch_checkpoint_size", "fvk_batch_checkpoint_save", "fvk_batch_checkpoint_load"]
 
@@ -92,6 +92,7 @@ def lib():
         L.fvk_get_geometry.argtypes = [vp, ctypes.c_char_p, dp]
         L.fvk_geometry.argtypes = [P(FvkMesh), P(FvkOptions), ctypes.c_char_p, dp]
         L.fvk_get_info.argtypes = [vp, P(ctypes.c_int64)]
+        L.fvk_get_options.argtypes = [vp, P(FvkOptions)]
         L.fvk_time_steps.argtypes = [vp, ctypes.c_int32, dp, P(ctypes.c_int64)]
         L.fvk_time_kernels.argtypes = [vp, ctypes.c_int32, dp]
         ip = P(ctypes.c_int32)
@@ -107,6 +108,7 @@ def lib():
                                             ip, dp, dp]
         L.fvk_batch_get_state.argtypes = [vp, ctypes.c_int32, dp, dp]
         L.fvk_batch_get_info.argtypes = [vp, P(ctypes.c_int64)]
+        L.fvk_batch_get_options.argtypes = [vp, ctypes.c_int32, P(FvkOptions)]
         L.fvk_batch_use_graph.argtypes = [vp, ctypes.c_int32]
         L.fvk_batch_time_kernels.argtypes = [vp, ctypes.c_int32, dp]
         for pre in ("fvk_", "fvk_batch_"):
@@ -245,6 +247,13 @@ class Solver:
                 "num_wall"]
         return dict(zip(keys, list(a)))
 
+    def launch_options(self) -> dict:
+        """Cells per block, XCD order and kernel shape in force."""
+        o = FvkOptions()
+        self._check(lib().fvk_get_options(self.h, ctypes.byref(o)))
+        return {"block_size": o.block_size, "xcd_remap": o.xcd_remap,
+                "kernel_variant": o.kernel_variant}
+
     def time_kernels(self, reps: int):
         """Average microseconds per launch of pass A and pass B (stage 2)."""
         out = np.zeros(2)
@@ -352,6 +361,14 @@ class Batch:
         self._check(lib().fvk_batch_get_info(self.h, a))
         keys = ["instances", "cells", "faces", "blocks", "active_blocks", "active_instances"]
         return dict(zip(keys, list(a)))
+
+    def launch_options(self) -> dict:
+        """Cells per block, XCD order and kernel shape in force (common to all
+        instances)."""
+        o = FvkOptions()
+        self._check(lib().fvk_batch_get_options(self.h, 0, ctypes.byref(o)))
+        return {"block_size": o.block_size, "xcd_remap": o.xcd_remap,
+                "kernel_variant": o.kernel_variant}
 
     def checkpoint(self) -> bytes:
         n = ctypes.c_int64()

@@ -47,9 +47,11 @@ def cylinder():
                         BC1="NO_SLIP_WALL", dt_type="GLOBAL", CFL=0.5))
 
 
-def airfoil(n_points: int = 8_000_000, log=None):
+def airfoil(n_points: int = 8_000_000, log=None, cache: str = None):
+    """The airfoil workload; `cache` names a Grid.save_npz file of the same
+    mesh to load instead of generating it."""
     u, v = inputs.mach_velocity(0.7, angle_deg=1.25)
-    grid = gg.airfoil_dual(n_points, log=log)
+    grid = gg.Grid.load_npz(cache) if cache else gg.airfoil_dual(n_points, log=log)
     return (f"airfoil dual of Delaunay({n_points} points) NACA0012 Euler o2 SQUEEZE LOCAL "
             "CFL0.5 M0.7 AoA1.25", grid,
             inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE", dt_type="LOCAL", CFL=0.5,

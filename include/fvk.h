@@ -79,8 +79,12 @@ typedef struct fvk_options {
   int32_t precision;       /* FVK_FP64 (default) or FVK_FP32_MIXED */
   int32_t renumber;        /* cell blocks: 1 recursive coordinate bisection (default),
                               2 reverse Cuthill-McKee ranges, 0 file-order ranges */
-  int32_t block_size;      /* cells per block: 0 = 256; else 64, 128, 192 or 256 */
-  int32_t xcd_remap;       /* 1: contiguous block ranges per XCD (default), 0: off */
+  int32_t block_size;      /* cells per block: 64, 128, 192 or 256; 0 (default) =
+                              128 for FP64 or above 2M cells in the solve, else 256 */
+  int32_t xcd_remap;       /* XCD block order: 1 contiguous block ranges per XCD;
+                              G in 2..4096 runs of G consecutive blocks dealt to
+                              the XCDs in turn; 0 off; -1 (default) = 1 up to 4096
+                              blocks, else 8 */
   int32_t kernel_variant;  /* 0: default (3 when the blocks fit in LDS, else 1);
                               1: one thread per cell; 2: one thread per face slot;
                               3: LDS-staged blocks */
@@ -134,6 +138,11 @@ int fvk_geometry(const fvk_mesh* mesh, const fvk_options* opt, const char* name,
 /* Sizes: [num_cells, num_faces, num_boundary_faces, num_slots, max_faces_per_cell,
  * num_wall_faces (readmesh.cpp:356-360)]. */
 int fvk_get_info(fvk_ctx* ctx, int64_t info[6]);
+
+/* The options in force, with the automatic choices (block_size 0,
+ * xcd_remap -1) resolved.  No counterpart in the reference, whose options
+ * are fixed by input.in. */
+int fvk_get_options(fvk_ctx* ctx, fvk_options* out);
 
 /* Timing: run nsteps with a HIP event pair around every launch on the
  * context's stream.  out_ms[0] = wall time of the nsteps (event-timed),
@@ -202,6 +211,8 @@ int fvk_batch_run_to_drop(fvk_batch* b, double drop, int32_t max_steps, int32_t 
 int fvk_batch_get_state(fvk_batch* b, int32_t i, double* q, double* grad);
 /* [instances, cells, faces, mesh blocks, active blocks, active instances] */
 int fvk_batch_get_info(fvk_batch* b, int64_t info[6]);
+/* instance i's options in force, as fvk_get_options */
+int fvk_batch_get_options(fvk_batch* b, int32_t i, fvk_options* out);
 /* 0: launch every step's kernels one by one; 1 (default): replay the graph */
 int fvk_batch_use_graph(fvk_batch* b, int32_t on);
 /* checkpoints of a whole batch (with the active set and step counters) */

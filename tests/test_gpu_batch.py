@@ -119,6 +119,28 @@ def test_batch_retirement():
         assert same(b.state(i), q1), f"instance {i} ({n} steps): {mismatch(b.state(i), q1)}"
 
 
+@pytest.mark.parametrize("opts", [dict(block_size=128, xcd_remap=0),
+                                  dict(block_size=128, xcd_remap=4), dict(xcd_remap=2)])
+def test_batch_launch_orders(opts):
+    """Block size and XCD block order change no bit, with retired instances
+    left out of the active-block list."""
+    cases = euler_cases("GLOBAL")
+    print(f"[gpu] batch launch order {opts}", flush=True)
+    flags = np.ones(len(cases), dtype=np.uint8)
+    flags[2] = 0
+    out = []
+    for ov in ({}, opts):
+        b = Batch(cases, precision=FP32_MIXED, **ov)
+        b.init()
+        b.step(6)
+        b.set_active(flags)
+        b.step(9)
+        out.append([b.state_and_grad(i) for i in range(len(cases))])
+        b.close()
+    for i in range(len(cases)):
+        assert same(out[0][i][0], out[1][i][0]) and same(out[0][i][1], out[1][i][1]), i
+
+
 def test_batch_run_to_drop():
     cases = []
     for nx, mach in [(24, 0.4), (32, 0.5), (40, 0.6)]:

@@ -198,7 +198,8 @@ def test_fp32_residual_norm():
 
 @pytest.mark.parametrize("overrides", [{"kernel_variant": 1}, {"kernel_variant": 2},
                                        {"renumber": 0}, {"renumber": 2}, {"block_size": 128},
-                                       {"xcd_remap": 0}])
+                                       {"xcd_remap": 0}, {"xcd_remap": 8},
+                                       {"block_size": 128, "xcd_remap": 5}])
 def test_fp32_layout_invariance(overrides):
     grid = gg.ramp(96, 48)
     u, v = inputs.mach_velocity(2.5)

@@ -15,7 +15,7 @@ import pytest
 import oracle as O
 from fv2d import gridgen as gg
 from fv2d import inputs
-from fv2d.lib import FvkError, Solver
+from fv2d.lib import FP32_MIXED, FvkError, Solver
 from fvtest import Golden, golden_names, mismatch, same
 
 pytestmark = pytest.mark.gpu
@@ -101,9 +101,10 @@ def test_full_size_against_oracle(kind, steps, variant):
 
 @pytest.mark.parametrize("over", [dict(renumber=0), dict(renumber=2), dict(kernel_variant=1),
                                   dict(kernel_variant=2), dict(kernel_variant=1, block_size=64),
-                                  dict(kernel_variant=3, block_size=128),
+                                  dict(kernel_variant=3, block_size=256),
                                   dict(kernel_variant=3, block_size=64, renumber=2),
-                                  dict(xcd_remap=0)])
+                                  dict(xcd_remap=0), dict(xcd_remap=3),
+                                  dict(kernel_variant=2, xcd_remap=16)])
 def test_layout_invariance(over):
     g = Golden("ogrid_ns_squeeze_global")
     a = Solver(g.grid, g.case)
@@ -115,6 +116,20 @@ def test_layout_invariance(over):
     qa, ga = a.state_and_grad()
     qb, gb = b.state_and_grad()
     assert same(qa, qb) and same(ga, gb), over
+
+
+def test_automatic_launch_options():
+    """block_size 0 and xcd_remap -1 resolve by precision and size; explicit
+    values are kept; out-of-range orders are refused."""
+    g = Golden("ramp_o2_squeeze")
+    auto = {"block_size": 128, "xcd_remap": 1, "kernel_variant": 3}
+    assert Solver(g.grid, g.case).launch_options() == auto
+    assert Solver(g.grid, g.case, precision=FP32_MIXED).launch_options() == {**auto,
+                                                                             "block_size": 256}
+    got = Solver(g.grid, g.case, block_size=192, xcd_remap=5).launch_options()
+    assert got == {"block_size": 192, "xcd_remap": 5, "kernel_variant": 3}
+    with pytest.raises(FvkError):
+        Solver(g.grid, g.case, xcd_remap=-2)
 
 
 def test_restart_semantics_match_oracle():
@@ -182,9 +197,9 @@ def test_checkpoint_resumes_exactly(name, precision):
 
 def test_checkpoint_rejects_another_layout():
     g = Golden("ramp_o2_squeeze")
-    a = Solver(g.grid, g.case)
+    a = Solver(g.grid, g.case, block_size=128)
     a.init_freestream()
     ck = a.checkpoint()
-    b = Solver(g.grid, g.case, block_size=128)
+    b = Solver(g.grid, g.case, block_size=256)
     with pytest.raises(FvkError):
         b.restore(ck)

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
         assert re.search(rf"\bT {name}$", out, re.M), name
 
 
-def test_struct_layout_matches_header():
+def test_struct_layout_matches_header(tmp_path):
     # offsets the C compiler gives the header structs
     src = r"""
 #include <stddef.h>
@@ -42,8 +42,7 @@ int main(void) {
   return 0;
 }
 """
-    d = os.path.join(ROOT, "_work", "abi")
-    os.makedirs(d, exist_ok=True)
+    d = str(tmp_path)
     with open(os.path.join(d, "abi.c"), "w") as f:
         f.write(src)
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), "-o",
@@ -163,7 +162,7 @@ def test_reference_grid_file_reads():
     assert np.all(vol > 0)
 
 
-def test_airfoil_dual_generator():
+def test_airfoil_dual_generator(tmp_path):
     """The polygonal Delaunay-dual mesh: 4- to 9-sided cells, a slip wall
     around the section and freestream outside, closed cells."""
     g = gg.airfoil_dual(20000)
@@ -185,6 +184,14 @@ def test_airfoil_dual_generator():
     sx -= np.bincount(g.face_c2[inner], (nx * area)[inner], g.num_cells)
     sy -= np.bincount(g.face_c2[inner], (ny * area)[inner], g.num_cells)
     assert np.abs(sx).max() < 1e-15 and np.abs(sy).max() < 1e-15
+    # the binary cache gives back the same mesh
+    path = os.path.join(str(tmp_path), "g.npz")
+    g.save_npz(path)
+    h = gg.Grid.load_npz(path)
+    for k in ("x", "y", "face_p1", "face_p2", "face_c1", "face_c2", "bc_marker"):
+        a, b = getattr(g, k), getattr(h, k)
+        assert a.dtype == b.dtype and np.array_equal(a, b), k
+    assert (h.num_cells, h.name) == (g.num_cells, g.name)
 
 
 def test_wall_coefficients_of_a_uniform_flow():
