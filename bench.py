@@ -121,7 +121,7 @@ def cpu_baseline(samples, target_s: float = 15.0):
                 return time.perf_counter() - t0
             kind = "port"
         t1 = run(2)
-        n = max(2, min(400, int(per / max(t1 / 2, 1e-6))))
+        n = max(2, min(100000, int(per / max(t1 / 2, 1e-6))))
         t = run(n)
         work += nc * 3 * n
         secs += t
@@ -152,9 +152,10 @@ def copy_calibration(mbytes: int) -> None:
 
 
 def stream_probe(mbytes: int) -> None:
-    """Device-to-device copy bandwidth of the HIP runtime's copy kernel for a
-    buffer of `mbytes` MiB (read + write bytes / time, 20 timed copies): the
-    bandwidth a plain stream reaches on this GPU, beside the 8 TB/s peak."""
+    """Achievable bandwidth on this GPU, beside the 8 TB/s peak: the HIP
+    runtime's device-to-device copy of `mbytes` MiB (read + write bytes /
+    time, 20 timed copies), then `lib/fvk_bw_probe` (read-only, four reads
+    per write and copy kernels over arrays of `mbytes` MiB, best grid)."""
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     n = mbytes << 20
@@ -176,7 +177,12 @@ def stream_probe(mbytes: int) -> None:
     hip.hipFree(src)
     hip.hipFree(dst)
     gbs = 2.0 * n * reps / (ms.value * 1e-3) / 1e9
-    print(json.dumps({"stream_probe_mib": mbytes, "copies": reps, "GB_per_s": round(gbs, 1)}))
+    out = {"stream_probe_mib": mbytes, "copies": reps, "GB_per_s": round(gbs, 1)}
+    probe = os.path.join(ROOT, "cfd-2d-ns-fv_amd", "lib", "fvk_bw_probe")
+    if os.path.exists(probe):
+        r = subprocess.run([probe, str(mbytes)], capture_output=True, text=True, check=True)
+        out["kernels_GB_per_s"] = json.loads(r.stdout)
+    print(json.dumps(out))
 
 
 def _pmc_mean_kib(counter: str, child_args, match, timeout: int = 300):
@@ -367,11 +373,13 @@ def bench_single(args):
             # wall pressure and force coefficients (fv2d.analysis), FP32-mixed vs FP64,
             # and the same comparison for the perturbed FP64 run
             from fv2d import analysis
+            log("wall coefficients")
             _, g32 = r["solver"].state_and_grad()
             _, g64 = r64["solver"].state_and_grad()
             _, gfl = fl.state_and_grad()
-            walls = analysis.compare_walls(grid, case, q, g32, q64, g64)
-            wfl = analysis.compare_walls(grid, case, fl.state(), gfl, q64, g64)
+            geo = analysis.wall_geometry(grid, case, r64["solver"])
+            walls = analysis.compare_walls(grid, case, q, g32, q64, g64, geo)
+            wfl = analysis.compare_walls(grid, case, fl.state(), gfl, q64, g64, geo)
             out["wall_vs_fp64"] = {k: float(f"{v:.4e}") for k, v in walls.items()}
             out["wall_vs_fp64"]["floor_cd_rel_err"] = float(f"{wfl['cd_rel_err']:.3e}")
             out["wall_vs_fp64"]["floor_cp_linf_err"] = float(f"{wfl['cp_linf_err']:.3e}")

@@ -18,13 +18,24 @@ import numpy as np
 from .lib import host_geometry
 
 
-def wall_state(grid, case, q, grad=None):
+GEOMETRY = ("face_nx", "face_ny", "face_area", "face_centerx", "face_centery", "cell_centerx",
+            "cell_centery")
+
+
+def wall_geometry(grid, case, solver=None):
+    """The host geometry arrays the wall quantities use (compute once per mesh
+    and pass as `geo` when comparing several solutions); from an existing
+    solver context of the mesh when given, which holds them already."""
+    if solver is not None:
+        return {k: solver.geometry(k) for k in GEOMETRY}
+    return {k: host_geometry(grid, case, k) for k in GEOMETRY}
+
+
+def wall_state(grid, case, q, grad=None, geo=None):
     """Face index, conserved state at the face centre, and geometry of every
     face of the BC1 group (file marker 0)."""
     walls = np.flatnonzero((grid.face_c2 < 0) & (grid.bc_marker == 0))
-    geo = {k: host_geometry(grid, case, k)
-           for k in ("face_nx", "face_ny", "face_area", "face_centerx", "face_centery",
-                     "cell_centerx", "cell_centery")}
+    geo = geo or wall_geometry(grid, case)
     c = grid.face_c1[walls]
     qf = q[:, c].copy()
     dx = geo["face_centerx"][walls] - geo["cell_centerx"][c]
@@ -34,11 +45,11 @@ def wall_state(grid, case, q, grad=None):
     return walls, qf, q[:, c], geo, dx, dy
 
 
-def wall_coefficients(grid, case, q, grad=None, length: float = 1.0):
+def wall_coefficients(grid, case, q, grad=None, length: float = 1.0, geo=None):
     """(cp, x_wall, cl, cd): pressure coefficient at each wall face (ordered as
     the faces), the wall faces' x, and the lift and drag coefficients per
     unit span referred to `length`."""
-    walls, qf, qc, geo, dx, dy = wall_state(grid, case, q, grad)
+    walls, qf, qc, geo, dx, dy = wall_state(grid, case, q, grad, geo)
     g = case.gamma
     rho, mx, my, E = qf
     P = (E - 0.5 * (mx * mx + my * my) / rho) * (g - 1)
@@ -62,10 +73,11 @@ def wall_coefficients(grid, case, q, grad=None, length: float = 1.0):
     return cp, geo["face_centerx"][walls], lift / (qdyn * length), drag / (qdyn * length)
 
 
-def compare_walls(grid, case, q32, g32, q64, g64):
+def compare_walls(grid, case, q32, g32, q64, g64, geo=None):
     """FP32-mixed against FP64 wall quantities."""
-    cp32, _, cl32, cd32 = wall_coefficients(grid, case, q32, g32)
-    cp64, _, cl64, cd64 = wall_coefficients(grid, case, q64, g64)
+    geo = geo or wall_geometry(grid, case)
+    cp32, _, cl32, cd32 = wall_coefficients(grid, case, q32, g32, geo=geo)
+    cp64, _, cl64, cd64 = wall_coefficients(grid, case, q64, g64, geo=geo)
     return {"cd_fp64": cd64, "cd_fp32": cd32, "cl_fp64": cl64, "cl_fp32": cl32,
             "cd_rel_err": abs(cd32 - cd64) / abs(cd64),
             "cp_l2_err": float(np.linalg.norm(cp32 - cp64) / np.sqrt(cp64.size)),
