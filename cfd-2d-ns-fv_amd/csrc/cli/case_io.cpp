@@ -3,12 +3,17 @@
 // writers, so the files compare line for line.
 #include "case_io.hpp"
 
+#include <sys/stat.h>
+
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <sstream>
+#include <thread>
 
 namespace fv2d {
 
@@ -129,6 +134,7 @@ bool read_inputs(const std::string& path, Inputs* in, std::string* log, std::str
         if (key == "block_size") in->block_size = std::stoi(val);
         if (key == "xcd_remap") in->xcd_remap = std::stoi(val);
         if (key == "kernel_variant") in->kernel_variant = std::stoi(val);
+        if (key == "grid_cache") in->grid_cache = std::stoi(val);
         if (key == "precision") in->precision = (val == "FP32_MIXED") ? FVK_FP32_MIXED : (val == "FP64" ? FVK_FP64 : -1);
       }
     }
@@ -147,7 +153,95 @@ bool read_inputs(const std::string& path, Inputs* in, std::string* log, std::str
   return true;
 }
 
-bool read_grid(const std::string& path, Grid* g, std::string* log, std::string* err) {
+namespace {
+
+inline bool is_space(char c) {   // the characters strtod/strtol skip (isspace in the C locale)
+  return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\v' || c == '\f';
+}
+
+int reader_threads() {
+  const int hw = (int)std::thread::hardware_concurrency();
+  return std::max(1, std::min(16, hw));
+}
+
+// The whitespace-separated token stream of readmesh.cpp:31-85 after the
+// header: 2*np coordinates, then 2*nf point ids, then 2*nf cell ids, then
+// anything (ignored, as the reference ignores trailing tokens).  The text is
+// cut into one range per thread at token boundaries; a first pass counts each
+// range's tokens, so every thread knows the index of its first token and
+// parses its range straight into the arrays.  Each value is the one
+// strtod/strtol gives for its token, as the serial reader's.
+bool parse_body(const char* p, const char* end, Grid* g, std::string* err) {
+  const int nt = (end - p > (1 << 22)) ? reader_threads() : 1;
+  std::vector<const char*> cut(nt + 1);
+  cut[0] = p;
+  cut[nt] = end;
+  for (int t = 1; t < nt; ++t) {
+    const char* q = p + (end - p) * t / nt;
+    while (q < end && !is_space(*q)) ++q;   // move the cut to the end of a token
+    cut[t] = std::max(q, cut[t - 1]);
+  }
+  std::vector<int64_t> count(nt + 1, 0);
+  auto count_range = [&](int t) {
+    int64_t n = 0;
+    bool in_tok = false;
+    for (const char* q = cut[t]; q < cut[t + 1]; ++q) {
+      const bool sp = is_space(*q);
+      if (!sp && !in_tok) ++n;
+      in_tok = !sp;
+    }
+    count[t + 1] = n;
+  };
+  const int64_t need_pts = 2 * (int64_t)g->np, need = need_pts + 4 * (int64_t)g->nf;
+  std::vector<int> bad(nt, 0);
+  auto parse_range = [&](int t, int64_t k) {
+    const char* q = cut[t];
+    const char* e = cut[t + 1];
+    while (k < need) {
+      while (q < e && is_space(*q)) ++q;
+      if (q >= e) break;
+      char* stop = nullptr;
+      if (k < need_pts) {
+        const double v = std::strtod(q, &stop);
+        (k & 1 ? g->y : g->x)[k >> 1] = v;
+      } else {
+        const long v = std::strtol(q, &stop, 10);
+        const int64_t j = k - need_pts;
+        const int64_t i = (j % (2 * (int64_t)g->nf)) >> 1;
+        if (j < 2 * (int64_t)g->nf) (j & 1 ? g->p2 : g->p1)[i] = (int32_t)v - 1;
+        else (j & 1 ? g->c2 : g->c1)[i] = (int32_t)v - 1;
+      }
+      if (stop == q || (stop < e && !is_space(*stop))) { bad[t] = 1; return; }
+      q = stop;
+      ++k;
+    }
+  };
+  {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(count_range, t);
+    count_range(0);
+    for (auto& th : pool) th.join();
+  }
+  for (int t = 0; t < nt; ++t) count[t + 1] += count[t];
+  if (count[nt] < need) {
+    *err = "ERROR: grid file ended early";
+    return false;
+  }
+  {
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(parse_range, t, count[t]);
+    parse_range(0, 0);
+    for (auto& th : pool) th.join();
+  }
+  for (int t = 0; t < nt; ++t)
+    if (bad[t]) {
+      *err = "ERROR: grid file holds a token that is not a number";
+      return false;
+    }
+  return true;
+}
+
+bool read_grid_text(const std::string& path, Grid* g, std::string* err) {
   FILE* f = std::fopen(path.c_str(), "rb");
   if (!f) {
     *err = "ERROR: Cannot Open Grid File";
@@ -169,13 +263,6 @@ bool read_grid(const std::string& path, Grid* g, std::string* log, std::string* 
     p = e;
     return v;
   };
-  auto rd_d = [&]() -> double {
-    char* e = nullptr;
-    double v = std::strtod(p, &e);
-    if (e == p) ok = false;
-    p = e;
-    return v;
-  };
   g->np = (int32_t)rd_l();
   g->nf = (int32_t)rd_l();
   g->nc = (int32_t)rd_l();
@@ -184,13 +271,75 @@ bool read_grid(const std::string& path, Grid* g, std::string* log, std::string* 
     return false;
   }
   g->x.resize(g->np); g->y.resize(g->np);
-  for (int32_t i = 0; i < g->np; ++i) { g->x[i] = rd_d(); g->y[i] = rd_d(); }
   g->p1.resize(g->nf); g->p2.resize(g->nf); g->c1.resize(g->nf); g->c2.resize(g->nf);
-  for (int32_t i = 0; i < g->nf; ++i) { g->p1[i] = (int32_t)rd_l() - 1; g->p2[i] = (int32_t)rd_l() - 1; }
-  for (int32_t i = 0; i < g->nf; ++i) { g->c1[i] = (int32_t)rd_l() - 1; g->c2[i] = (int32_t)rd_l() - 1; }
-  if (!ok) {
-    *err = "ERROR: grid file ended early";
-    return false;
+  return parse_body(p, &buf[0] + got, g, err);
+}
+
+// Binary cache of a parsed grid: header, then x, y, p1, p2, c1, c2 as read.
+// It is used only while the text file keeps the size and modification time
+// recorded in it.
+struct CacheHeader {
+  char magic[8];
+  int64_t text_size;
+  int64_t text_mtime_ns;
+  int32_t np, nf, nc, pad;
+};
+
+bool text_stamp(const std::string& path, int64_t* size, int64_t* mtime_ns) {
+  struct stat st;
+  if (::stat(path.c_str(), &st) != 0) return false;
+  *size = (int64_t)st.st_size;
+  *mtime_ns = (int64_t)st.st_mtim.tv_sec * 1000000000 + st.st_mtim.tv_nsec;
+  return true;
+}
+
+bool load_cache(const std::string& path, const std::string& cpath, Grid* g) {
+  int64_t size, mtime;
+  if (!text_stamp(path, &size, &mtime)) return false;
+  FILE* f = std::fopen(cpath.c_str(), "rb");
+  if (!f) return false;
+  CacheHeader h;
+  bool ok = std::fread(&h, sizeof h, 1, f) == 1 && std::memcmp(h.magic, "FVKGRID1", 8) == 0 &&
+            h.text_size == size && h.text_mtime_ns == mtime && h.np > 0 && h.nf > 0 && h.nc > 0;
+  if (ok) {
+    g->np = h.np; g->nf = h.nf; g->nc = h.nc;
+    g->x.resize(h.np); g->y.resize(h.np);
+    g->p1.resize(h.nf); g->p2.resize(h.nf); g->c1.resize(h.nf); g->c2.resize(h.nf);
+    ok = std::fread(g->x.data(), sizeof(double), h.np, f) == (size_t)h.np &&
+         std::fread(g->y.data(), sizeof(double), h.np, f) == (size_t)h.np;
+    for (std::vector<int32_t>* v : {&g->p1, &g->p2, &g->c1, &g->c2})
+      ok = ok && std::fread(v->data(), sizeof(int32_t), h.nf, f) == (size_t)h.nf;
+  }
+  std::fclose(f);
+  return ok;
+}
+
+void save_cache(const std::string& path, const std::string& cpath, const Grid& g) {
+  CacheHeader h{};
+  std::memcpy(h.magic, "FVKGRID1", 8);
+  if (!text_stamp(path, &h.text_size, &h.text_mtime_ns)) return;
+  h.np = g.np; h.nf = g.nf; h.nc = g.nc;
+  const std::string tmp = cpath + ".tmp";
+  FILE* f = std::fopen(tmp.c_str(), "wb");
+  if (!f) return;
+  bool ok = std::fwrite(&h, sizeof h, 1, f) == 1 &&
+            std::fwrite(g.x.data(), sizeof(double), g.np, f) == (size_t)g.np &&
+            std::fwrite(g.y.data(), sizeof(double), g.np, f) == (size_t)g.np;
+  for (const std::vector<int32_t>* v : {&g.p1, &g.p2, &g.c1, &g.c2})
+    ok = ok && std::fwrite(v->data(), sizeof(int32_t), g.nf, f) == (size_t)g.nf;
+  ok = (std::fclose(f) == 0) && ok;
+  if (ok) ok = std::rename(tmp.c_str(), cpath.c_str()) == 0;
+  if (!ok) std::remove(tmp.c_str());
+}
+
+}  // namespace
+
+bool read_grid(const std::string& path, Grid* g, std::string* log, std::string* err, bool cache) {
+  const std::string cpath = path + ".fvkgrid";
+  const bool cached = cache && load_cache(path, cpath, g);
+  if (!cached) {
+    if (!read_grid_text(path, g, err)) return false;
+    if (cache) save_cache(path, cpath, *g);
   }
   std::vector<int32_t> cnt(g->nc, 0);
   for (int32_t i = 0; i < g->nf; ++i) {

@@ -23,6 +23,7 @@ struct Inputs {
   int BC1 = 0, BC2 = 0;
   // optional trailing [gpu] section (read only after the nine standard ones)
   int renumber = 1, block_size = 0, xcd_remap = -1, kernel_variant = 0;
+  int grid_cache = 0;          // [gpu] grid_cache = 1: binary grid cache beside the grid file
   int precision = FVK_FP64;    // [gpu] precision = FP64 | FP32_MIXED
 };
 
@@ -36,8 +37,11 @@ struct Grid {
   std::vector<int32_t> p1, p2, c1, c2;  // 0-based; c2 = file value - 1 (readmesh.cpp:59-69)
 };
 
-// readmesh.cpp:31-85 token stream
-bool read_grid(const std::string& path, Grid* g, std::string* log, std::string* err);
+// readmesh.cpp:31-85 token stream, parsed by several threads.  With `cache`,
+// a binary copy `<path>.fvkgrid` is read instead while the text file keeps its
+// size and modification time, and written after a text parse.
+bool read_grid(const std::string& path, Grid* g, std::string* log, std::string* err,
+               bool cache = false);
 
 fvk_mesh as_mesh(const Grid& g);
 fvk_options as_options(const Inputs& in);

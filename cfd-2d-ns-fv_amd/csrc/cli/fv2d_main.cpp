@@ -1,5 +1,9 @@
 // fv2d — drop-in command line program: `fv2d [input.in]`.
 //
+// `fv2d --cache-grid <grid file>` only parses the grid file and writes its
+// binary cache `<grid file>.fvkgrid` (used by runs whose input.in sets
+// `grid_cache = 1` in the [gpu] section); it needs no GPU.
+//
 // Same flow as the reference program (src/base/main.cpp:33-140): read
 // input.in and the grid, write the output headers, start from freestream or
 // sol/restart_sol.dat, run nmax SSP-RK3 iterations (here on the GPU through
@@ -19,6 +23,16 @@
 #include "fvk.h"
 
 int main(int argc, char** argv) {
+  if (argc == 3 && std::string(argv[1]) == "--cache-grid") {
+    fv2d::Grid grid;
+    std::string log, err;
+    if (!fv2d::read_grid(argv[2], &grid, &log, &err, true)) {
+      std::cout << err << std::endl;
+      return 1;
+    }
+    std::cout << log;
+    return 0;
+  }
   ::mkdir("sol", 0755);
   ::mkdir("mesh", 0755);
   fv2d::Inputs in;
@@ -30,7 +44,7 @@ int main(int argc, char** argv) {
   }
   std::cout << log;
   fv2d::Grid grid;
-  if (!fv2d::read_grid(in.grid_file, &grid, &log, &err)) {
+  if (!fv2d::read_grid(in.grid_file, &grid, &log, &err, in.grid_cache != 0)) {
     std::cout << err << std::endl;
     return 1;
   }

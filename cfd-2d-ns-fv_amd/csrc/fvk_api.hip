@@ -17,9 +17,11 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "fv_kernels.hpp"
@@ -93,6 +95,17 @@ namespace {
 constexpr int64_t kAutoLargeCells = 2000000;    // FP32-mixed: 128-cell blocks above this
 constexpr int32_t kAutoContiguousBlocks = 4096; // contiguous XCD ranges up to this many blocks
 
+// host threads of the mesh preparation: FVK_PREP_THREADS (1..64) when set,
+// else the hardware threads, at most 16
+int prep_threads() {
+  if (const char* v = std::getenv("FVK_PREP_THREADS")) {
+    const int t = std::atoi(v);
+    if (t >= 1 && t <= 64) return t;
+  }
+  const int hw = (int)std::thread::hardware_concurrency();
+  return std::max(1, std::min(16, hw));
+}
+
 // instance i's options with the engine-wide launch choices resolved
 fvk_options resolved_options(const Engine& c, int32_t i) {
   fvk_options o = c.in[i].opt;
@@ -133,8 +146,8 @@ int dev_alloc(Engine* c, T** out, size_t count) {
 // `adjust(instance, x)` rewrites a value (global indices get the instance's
 // offsets); prefix arrays (`prefix`) drop every instance's last entry except
 // the last instance's.
-template <typename T, typename Adjust>
-int upload_cat(Engine* c, std::vector<T> fvk::HostMesh::*field, const T** out, Adjust adjust,
+template <typename T, typename Vec, typename Adjust>
+int upload_cat(Engine* c, Vec fvk::HostMesh::*field, const T** out, Adjust adjust,
                bool prefix = false) {
   const size_t n = c->in.size();
   auto count = [&](size_t i) {
@@ -149,12 +162,17 @@ int upload_cat(Engine* c, std::vector<T> fvk::HostMesh::*field, const T** out, A
   size_t off = 0;
   std::vector<T> tmp;
   for (size_t i = 0; i < n; ++i) {
-    const std::vector<T>& v = c->in[i].hm.*field;
+    const Vec& v = c->in[i].hm.*field;
     const size_t k = count(i);
-    tmp.assign(v.begin(), v.begin() + k);
-    for (T& x : tmp) x = adjust(c->in[i], x);
+    const T* src = v.data();
+    if (!std::is_same<Adjust, std::nullptr_t>::value) {   // copy, rewriting the values
+      tmp.assign(v.begin(), v.begin() + k);
+      if constexpr (!std::is_same<Adjust, std::nullptr_t>::value)
+        for (T& x : tmp) x = adjust(c->in[i], x);
+      src = tmp.data();
+    }
     if (k) {
-      hipError_t e = hipMemcpy(d + off, tmp.data(), k * sizeof(T), hipMemcpyHostToDevice);
+      hipError_t e = hipMemcpy(d + off, src, k * sizeof(T), hipMemcpyHostToDevice);
       if (e != hipSuccess) return hip_err(c, e, "hipMemcpy H2D");
     }
     off += k;
@@ -163,9 +181,10 @@ int upload_cat(Engine* c, std::vector<T> fvk::HostMesh::*field, const T** out, A
   return FVK_OK;
 }
 
-template <typename T>
-int upload_cat(Engine* c, std::vector<T> fvk::HostMesh::*field, const T** out) {
-  return upload_cat(c, field, out, [](const Instance&, T x) { return x; });
+// values unchanged: straight from the host array
+template <typename T, typename Vec>
+int upload_cat(Engine* c, Vec fvk::HostMesh::*field, const T** out) {
+  return upload_cat(c, field, out, nullptr);
 }
 
 // boundary code ~((src << 2) | kind) with the ghost source cell moved by `off`
@@ -628,14 +647,16 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
     std::vector<int> rcs(n, FVK_OK);
     std::vector<std::string> errs(n);
     std::atomic<int32_t> next{0};
+    const int pool_size = prep_threads();
+    const int nthreads = std::max(1, std::min<int>(n, pool_size));
+    // threads inside each instance's preparation: all of them for one mesh
+    const int inner = std::max(1, pool_size / nthreads);
     auto work = [&]() {
       for (int32_t i = next++; i < n; i = next++) {
         c->in[i].opt = ro[i];
-        rcs[i] = fvk::prepare_mesh(meshes[i], ro[i], &c->in[i].hm, &errs[i]);
+        rcs[i] = fvk::prepare_mesh(meshes[i], ro[i], &c->in[i].hm, &errs[i], inner);
       }
     };
-    const int hw = (int)std::thread::hardware_concurrency();
-    const int nthreads = std::max(1, std::min<int>(n, std::min(16, hw > 0 ? hw : 1)));
     std::vector<std::thread> pool;
     for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
     work();
@@ -984,7 +1005,7 @@ int fvk_residual_l2(fvk_ctx* c, double out[4]) {
 int fvk_get_geometry(fvk_ctx* c, const char* name, double* out) {
   if (!c || !name || !out) return FVK_ERR_ARG;
   const fvk::HostMesh& h = c->e.in[0].hm;
-  const std::vector<double>* v = nullptr;
+  const fvk::hvec<double>* v = nullptr;
   const std::string n = name;
   if (n == "face_area") v = &h.area;
   else if (n == "face_nx") v = &h.nx;
@@ -1008,7 +1029,7 @@ int fvk_geometry(const fvk_mesh* mesh, const fvk_options* opt, const char* name,
   fvk_ctx tmp;
   tmp.e.in.resize(1);
   std::string msg;
-  int rc = fvk::prepare_mesh(*mesh, *opt, &tmp.e.in[0].hm, &msg);
+  int rc = fvk::prepare_mesh(*mesh, *opt, &tmp.e.in[0].hm, &msg, prep_threads());
   if (rc) return set_err(nullptr, rc, msg);
   rc = fvk_get_geometry(&tmp, name, out);
   if (rc) g_create_error = tmp.e.err;

@@ -7,9 +7,27 @@
 #include <cmath>
 #include <cstdio>
 #include <numeric>
+#include <thread>
 
 namespace fvk {
 namespace {
+
+// fn(begin, end) over [0, n) split into `threads` contiguous ranges, one
+// thread each (the calling thread takes the first).  Every loop run through
+// it writes only the elements of its own range, so the result does not
+// depend on the split.
+template <typename Fn>
+void parallel_for(int64_t n, int threads, Fn fn) {
+  const int64_t t = std::max<int64_t>(1, std::min<int64_t>(threads, n / 4096));
+  if (t <= 1) {
+    fn(int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int64_t k = 1; k < t; ++k) pool.emplace_back(fn, n * k / t, n * (k + 1) / t);
+  fn(int64_t(0), n / t);
+  for (auto& th : pool) th.join();
+}
 
 int bc_kind_of(const fvk_options& o, int32_t c2) {
   if (c2 == -1) return o.bc1;  // readmesh.cpp:61-63
@@ -18,10 +36,11 @@ int bc_kind_of(const fvk_options& o, int32_t c2) {
 }
 
 // Face normals/areas/centres (readmesh.cpp:117-139).
-void face_geometry(const fvk_mesh& m, HostMesh& h) {
+void face_geometry(const fvk_mesh& m, HostMesh& h, int threads) {
   h.area.resize(h.nf); h.nx.resize(h.nf); h.ny.resize(h.nf);
   h.fcx.resize(h.nf); h.fcy.resize(h.nf);
-  for (int32_t f = 0; f < h.nf; ++f) {
+  parallel_for(h.nf, threads, [&](int64_t f0, int64_t f1) {
+  for (int32_t f = (int32_t)f0; f < (int32_t)f1; ++f) {
     const double xa = m.x[m.face_point1[f]], ya = m.y[m.face_point1[f]];
     const double xb = m.x[m.face_point2[f]], yb = m.y[m.face_point2[f]];
     const double dx = xb - xa, dy = yb - ya;
@@ -32,15 +51,17 @@ void face_geometry(const fvk_mesh& m, HostMesh& h) {
     h.fcx[f] = dx / 2 + xa;
     h.fcy[f] = dy / 2 + ya;
   }
+  });
 }
 
 // Cell volume and centroid (readmesh.cpp:147-228): triangles fanned about the
 // mean of the cell's distinct vertices.  The distinct-vertex search scans all
 // face-count slots, which start at (0,0), exactly as the reference does.
-void cell_geometry(const fvk_mesh& m, HostMesh& h) {
+void cell_geometry(const fvk_mesh& m, HostMesh& h, int threads) {
   h.vol.resize(h.nc); h.ccx.resize(h.nc); h.ccy.resize(h.nc);
+  parallel_for(h.nc, threads, [&](int64_t c0, int64_t c1) {
   std::vector<double> vx(h.max_deg), vy(h.max_deg);
-  for (int32_t c = 0; c < h.nc; ++c) {
+  for (int32_t c = (int32_t)c0; c < (int32_t)c1; ++c) {
     const int32_t b = h.cf_ptr[c], n = h.cf_ptr[c + 1] - b;
     std::fill(vx.begin(), vx.begin() + n, 0.0);
     std::fill(vy.begin(), vy.begin() + n, 0.0);
@@ -73,6 +94,7 @@ void cell_geometry(const fvk_mesh& m, HostMesh& h) {
     h.ccx[c] = wx / vsum;
     h.ccy[c] = wy / vsum;
   }
+  });
 }
 
 // Mirror image of the owner centroid about a boundary face (readmesh.cpp:238-243).
@@ -87,30 +109,34 @@ inline void ghost_centroid(const HostMesh& h, int32_t f, double* gx, double* gy)
 
 // LSQ normal-matrix inverse including ghost neighbours (readmesh.cpp:268-320),
 // accumulated per cell over its faces in increasing face index.
-void lsq_geometry(HostMesh& h) {
-  std::vector<double> sxx(h.nc, 0.0), syy(h.nc, 0.0), sxy(h.nc, 0.0);
-  for (int32_t f = 0; f < h.nf; ++f) {
-    const int32_t c1 = h.fc1[f], c2 = h.fc2[f];
-    const double xi = h.ccx[c1], yi = h.ccy[c1];
-    double xj, yj;
-    if (c2 >= 0) { xj = h.ccx[c2]; yj = h.ccy[c2]; }
-    else ghost_centroid(h, f, &xj, &yj);
-    sxx[c1] += (xi - xj) * (xi - xj);
-    syy[c1] += (yi - yj) * (yi - yj);
-    sxy[c1] += ((xi - xj) * (yi - yj));
-    if (c2 >= 0) {
-      sxx[c2] += (xj - xi) * (xj - xi);
-      syy[c2] += (yj - yi) * (yj - yi);
-      sxy[c2] += ((xj - xi) * (yj - yi));
-    }
-  }
+// The reference's face loop (readmesh.cpp:246-285) adds each face's terms to
+// its owner and neighbour in increasing face index.  Gathered per cell over
+// its faces in that same order, each sum takes the same additions in the same
+// order: the neighbour's terms (xj-xi)^2 ... are the cell's own
+// (x_self - x_other)^2 ..., as the owner's are.
+void lsq_geometry(HostMesh& h, int threads) {
   h.ixx.resize(h.nc); h.iyy.resize(h.nc); h.ixy.resize(h.nc);
-  for (int32_t c = 0; c < h.nc; ++c) {
-    const double det = sxx[c] * syy[c] - (sxy[c] * sxy[c]);
-    h.ixx[c] = sxx[c] / det;
-    h.iyy[c] = syy[c] / det;
-    h.ixy[c] = sxy[c] / det;
+  parallel_for(h.nc, threads, [&](int64_t c0, int64_t c1) {
+  for (int32_t c = (int32_t)c0; c < (int32_t)c1; ++c) {
+    double sxx = 0.0, syy = 0.0, sxy = 0.0;
+    const double xs = h.ccx[c], ys = h.ccy[c];
+    for (int32_t j = h.cf_ptr[c]; j < h.cf_ptr[c + 1]; ++j) {
+      const int32_t f = h.cf_idx[j];
+      const int32_t a = h.fc1[f], b = h.fc2[f];
+      double xo, yo;
+      if (b < 0) ghost_centroid(h, f, &xo, &yo);
+      else if (a == c) { xo = h.ccx[b]; yo = h.ccy[b]; }
+      else { xo = h.ccx[a]; yo = h.ccy[a]; }
+      sxx += (xs - xo) * (xs - xo);
+      syy += (ys - yo) * (ys - yo);
+      sxy += ((xs - xo) * (ys - yo));
+    }
+    const double det = sxx * syy - (sxy * sxy);
+    h.ixx[c] = sxx / det;
+    h.iyy[c] = syy / det;
+    h.ixy[c] = sxy / det;
   }
+  });
 }
 
 // Reverse Cuthill-McKee over the interior-face graph.  Each connected part
@@ -195,20 +221,21 @@ std::vector<int32_t> rcm_order(const HostMesh& h) {
 
 // Recursive coordinate bisection: split ids[0..n) into k parts of at most
 // cap cells, cutting each range across the longer extent of its centroids.
-// Appends the parts' cells, part after part, to out and the part sizes to
-// *sizes.
-void rcb(int32_t* ids, int64_t n, int64_t k, int64_t cap, const double* cx, const double* cy,
-         std::vector<int32_t>& out, std::vector<int32_t>* sizes) {
-  if (k <= 1 || n <= 1) {
-    out.insert(out.end(), ids, ids + n);
-    if (sizes) sizes->push_back((int32_t)n);
-    for (int64_t i = 1; sizes && i < k; ++i) sizes->push_back(0);
-    return;
-  }
+// In place: the parts end up contiguous, part after part.  The two halves of
+// a cut are independent, so `threads` threads share them; the order of each
+// half depends only on its own cells, so the result is the serial one.
+int64_t rcb_left(int64_t n, int64_t k, int64_t cap) {
   const int64_t kl = k / 2, kr = k - kl;
   int64_t nl = (n * kl + k / 2) / k;
   nl = std::max(nl, n - cap * kr);
-  nl = std::min(nl, cap * kl);
+  return std::min(nl, cap * kl);
+}
+
+void rcb(int32_t* ids, int64_t n, int64_t k, int64_t cap, const double* cx, const double* cy,
+         int threads) {
+  if (k <= 1 || n <= 1) return;
+  const int64_t kl = k / 2, kr = k - kl;
+  const int64_t nl = rcb_left(n, k, cap);
   double x0 = cx[ids[0]], x1 = x0, y0 = cy[ids[0]], y1 = y0;
   for (int64_t i = 1; i < n; ++i) {
     x0 = std::min(x0, cx[ids[i]]); x1 = std::max(x1, cx[ids[i]]);
@@ -218,13 +245,33 @@ void rcb(int32_t* ids, int64_t n, int64_t k, int64_t cap, const double* cx, cons
   std::nth_element(ids, ids + nl, ids + n, [key](int32_t a, int32_t b) {
     return key[a] < key[b] || (key[a] == key[b] && a < b);
   });
-  rcb(ids, nl, kl, cap, cx, cy, out, sizes);
-  rcb(ids + nl, n - nl, kr, cap, cx, cy, out, sizes);
+  if (threads > 1 && n > 65536) {
+    std::thread left(rcb, ids, nl, kl, cap, cx, cy, threads / 2);
+    rcb(ids + nl, n - nl, kr, cap, cx, cy, threads - threads / 2);
+    left.join();
+  } else {
+    rcb(ids, nl, kl, cap, cx, cy, 1);
+    rcb(ids + nl, n - nl, kr, cap, cx, cy, 1);
+  }
+}
+
+// part sizes of rcb(n, k, cap), in part order
+void rcb_sizes(int64_t n, int64_t k, int64_t cap, std::vector<int32_t>& sizes) {
+  if (k <= 1 || n <= 1) {
+    sizes.push_back((int32_t)n);
+    for (int64_t i = 1; i < k; ++i) sizes.push_back(0);
+    return;
+  }
+  const int64_t kl = k / 2, nl = rcb_left(n, k, cap);
+  rcb_sizes(nl, kl, cap, sizes);
+  rcb_sizes(n - nl, k - kl, cap, sizes);
 }
 
 }  // namespace
 
-int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::string* err) {
+int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::string* err,
+                 int threads) {
+  threads = std::max(1, threads);
   HostMesh& h = *out;
   h = HostMesh();
   char buf[256];
@@ -288,9 +335,9 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
   if (o.bc1 == FVK_BC_SLIP_WALL || o.bc1 == FVK_BC_NO_SLIP_WALL) h.num_wall = h.num_bc1;
   else if (o.bc2 == FVK_BC_SLIP_WALL || o.bc2 == FVK_BC_NO_SLIP_WALL) h.num_wall = h.num_bc2;
 
-  face_geometry(m, h);
-  cell_geometry(m, h);
-  lsq_geometry(h);
+  face_geometry(m, h, threads);
+  cell_geometry(m, h, threads);
+  lsq_geometry(h, threads);
 
   // The reference's no-slip gradient ghost reads the state of the cell whose
   // index equals the face index (cell_gradient.cpp:63-66).
@@ -333,20 +380,20 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
   h.ncp = h.nblocks * h.blk;
   std::vector<int32_t> order, sizes;
   if (o.renumber == 1) {
-    std::vector<int32_t> ids(h.nc);
-    std::iota(ids.begin(), ids.end(), 0);
-    rcb(ids.data(), h.nc, h.nblocks, h.blk, h.ccx.data(), h.ccy.data(), order, &sizes);
+    order.resize(h.nc);
+    std::iota(order.begin(), order.end(), 0);
+    rcb(order.data(), h.nc, h.nblocks, h.blk, h.ccx.data(), h.ccy.data(), threads);
+    rcb_sizes(h.nc, h.nblocks, h.blk, sizes);
     // order each block's cells by a further bisection into 64-cell groups
-    std::vector<int32_t> sub;
-    size_t pos = 0;
-    for (int32_t b = 0; b < h.nblocks; ++b) {
-      sub.clear();
-      const int32_t n = sizes[b];
-      rcb(order.data() + pos, n, (n + kChunk - 1) / kChunk, kChunk, h.ccx.data(), h.ccy.data(),
-          sub, nullptr);
-      std::copy(sub.begin(), sub.end(), order.begin() + pos);
-      pos += n;
-    }
+    std::vector<int64_t> start(h.nblocks + 1, 0);
+    for (int32_t b = 0; b < h.nblocks; ++b) start[b + 1] = start[b] + sizes[b];
+    parallel_for(h.nblocks, threads, [&](int64_t b0, int64_t b1) {
+      for (int64_t b = b0; b < b1; ++b) {
+        const int32_t n = sizes[b];
+        rcb(order.data() + start[b], n, (n + kChunk - 1) / kChunk, kChunk, h.ccx.data(),
+            h.ccy.data(), 1);
+      }
+    });
   } else {
     if (o.renumber == 2) order = rcm_order(h);
     else { order.resize(h.nc); std::iota(order.begin(), order.end(), 0); }
@@ -365,7 +412,8 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
   // halos and local indices
   h.halo_ptr.assign(h.nblocks + 1, 0);
   std::vector<std::vector<int32_t>> halos(h.nblocks);
-  for (int32_t b = 0; b < h.nblocks; ++b) {
+  parallel_for(h.nblocks, threads, [&](int64_t b0, int64_t b1) {
+  for (int32_t b = (int32_t)b0; b < (int32_t)b1; ++b) {
     std::vector<int32_t>& hl = halos[b];
     for (int32_t i = 0; i < h.blk; ++i) {
       const int32_t c = h.perm[(size_t)b * h.blk + i];
@@ -379,8 +427,11 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
     }
     std::sort(hl.begin(), hl.end());
     hl.erase(std::unique(hl.begin(), hl.end()), hl.end());
-    h.halo_ptr[b + 1] = h.halo_ptr[b] + (int32_t)hl.size();
-    h.max_halo = std::max<int32_t>(h.max_halo, (int32_t)hl.size());
+  }
+  });
+  for (int32_t b = 0; b < h.nblocks; ++b) {
+    h.halo_ptr[b + 1] = h.halo_ptr[b] + (int32_t)halos[b].size();
+    h.max_halo = std::max<int32_t>(h.max_halo, (int32_t)halos[b].size());
   }
   h.halo_idx.reserve(h.halo_ptr[h.nblocks]);
   for (auto& hl : halos) h.halo_idx.insert(h.halo_idx.end(), hl.begin(), hl.end());
@@ -393,7 +444,8 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
   // block face lists, in increasing original face index
   h.bf_ptr.assign(h.nblocks + 1, 0);
   std::vector<std::vector<int32_t>> bfaces(h.nblocks);
-  for (int32_t b = 0; b < h.nblocks; ++b) {
+  parallel_for(h.nblocks, threads, [&](int64_t b0, int64_t b1) {
+  for (int32_t b = (int32_t)b0; b < (int32_t)b1; ++b) {
     std::vector<int32_t>& fl = bfaces[b];
     for (int32_t i = 0; i < h.blk; ++i) {
       const int32_t c = h.perm[(size_t)b * h.blk + i];
@@ -402,8 +454,11 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
     }
     std::sort(fl.begin(), fl.end());
     fl.erase(std::unique(fl.begin(), fl.end()), fl.end());
-    h.bf_ptr[b + 1] = h.bf_ptr[b] + (int32_t)fl.size();
-    h.max_bfaces = std::max<int32_t>(h.max_bfaces, (int32_t)fl.size());
+  }
+  });
+  for (int32_t b = 0; b < h.nblocks; ++b) {
+    h.bf_ptr[b + 1] = h.bf_ptr[b] + (int32_t)bfaces[b].size();
+    h.max_bfaces = std::max<int32_t>(h.max_bfaces, (int32_t)bfaces[b].size());
   }
   const size_t nbf = h.bf_ptr[h.nblocks];
   h.bf_a.resize(nbf); h.bf_b.resize(nbf);
@@ -414,7 +469,8 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
     const int32_t src = (c2 == FVK_BC_NO_SLIP_WALL && f < h.nc) ? h.iperm[f] : 0;
     return slot_code_boundary(src, -c2 - 1);
   };
-  for (int32_t b = 0; b < h.nblocks; ++b) {
+  parallel_for(h.nblocks, threads, [&](int64_t b0, int64_t b1) {
+  for (int32_t b = (int32_t)b0; b < (int32_t)b1; ++b) {
     size_t e = h.bf_ptr[b];
     for (int32_t f : bfaces[b]) {
       h.bf_a[e] = (int16_t)local_of(b, h.iperm[h.fc1[f]]);
@@ -424,6 +480,7 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
       ++e;
     }
   }
+  });
 
   // sliced-ELL slot table
   const int32_t nchunks = h.ncp / kChunk;
@@ -439,14 +496,24 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
     h.chunk_off[k + 1] = h.chunk_off[k] + mx * kChunk;
   }
   const int64_t nslots = h.chunk_off[nchunks];
-  h.code.assign(nslots, 0); h.lcode.assign(nslots, 0); h.sface.assign(nslots, 0);
-  h.s_nx.assign(nslots, 0.0); h.s_ny.assign(nslots, 0.0); h.s_area.assign(nslots, 0.0);
-  h.s_fcx.assign(nslots, 0.0); h.s_fcy.assign(nslots, 0.0);
-  for (int32_t i = 0; i < h.ncp; ++i) {
+  // every slot is written below: a cell's own slots, then zeros in the
+  // padding slots of its lane (and in every slot of a padding cell's lane)
+  h.code.resize(nslots); h.lcode.resize(nslots); h.sface.resize(nslots);
+  h.s_nx.resize(nslots); h.s_ny.resize(nslots); h.s_area.resize(nslots);
+  h.s_fcx.resize(nslots); h.s_fcy.resize(nslots);
+  parallel_for(h.ncp, threads, [&](int64_t i0, int64_t i1) {
+  for (int32_t i = (int32_t)i0; i < (int32_t)i1; ++i) {
     const int32_t c = h.perm[i];
+    const int32_t k = i / kChunk;
+    const int32_t base = h.chunk_off[k] + (i % kChunk);
+    const int32_t width = (h.chunk_off[k + 1] - h.chunk_off[k]) / kChunk;
+    for (int32_t j = c >= 0 ? h.cf_ptr[c + 1] - h.cf_ptr[c] : 0; j < width; ++j) {
+      const int64_t s = base + (int64_t)j * kChunk;
+      h.code[s] = 0; h.lcode[s] = 0; h.sface[s] = 0;
+      h.s_nx[s] = 0.0; h.s_ny[s] = 0.0; h.s_area[s] = 0.0; h.s_fcx[s] = 0.0; h.s_fcy[s] = 0.0;
+    }
     if (c < 0) continue;
     const int32_t b = i / h.blk;
-    const int32_t base = h.chunk_off[i / kChunk] + (i % kChunk);
     const std::vector<int32_t>& fl = bfaces[b];
     for (int32_t j = h.cf_ptr[c]; j < h.cf_ptr[c + 1]; ++j) {
       const int32_t f = h.cf_idx[j];
@@ -468,14 +535,22 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
       h.s_fcx[s] = h.fcx[f]; h.s_fcy[s] = h.fcy[f];
     }
   }
+  });
   if (o.precision == FVK_FP32_MIXED) {
     // single-precision geometry of the FP32-mixed kernels
-    h.s_dfx32.assign(nslots, 0.0f);
-    h.s_dfy32.assign(nslots, 0.0f);
-    for (int32_t i = 0; i < h.ncp; ++i) {
+    h.s_dfx32.resize(nslots);
+    h.s_dfy32.resize(nslots);
+    parallel_for(h.ncp, threads, [&](int64_t i0, int64_t i1) {
+    for (int32_t i = (int32_t)i0; i < (int32_t)i1; ++i) {
       const int32_t c = h.perm[i];
+      const int32_t k = i / kChunk;
+      const int32_t base = h.chunk_off[k] + (i % kChunk);
+      const int32_t width = (h.chunk_off[k + 1] - h.chunk_off[k]) / kChunk;
+      for (int32_t j = c >= 0 ? h.cf_ptr[c + 1] - h.cf_ptr[c] : 0; j < width; ++j) {
+        h.s_dfx32[base + (int64_t)j * kChunk] = 0.0f;
+        h.s_dfy32[base + (int64_t)j * kChunk] = 0.0f;
+      }
       if (c < 0) continue;
-      const int32_t base = h.chunk_off[i / kChunk] + (i % kChunk);
       for (int32_t j = h.cf_ptr[c]; j < h.cf_ptr[c + 1]; ++j) {
         const int32_t f = h.cf_idx[j];
         const int64_t s = base + (int64_t)(j - h.cf_ptr[c]) * kChunk;
@@ -483,14 +558,16 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
         h.s_dfy32[s] = (float)(h.fcy[f] - h.ccy[c]);
       }
     }
+    });
     const bool ns = o.eqn == 2;
     h.bf32_nx.resize(nbf); h.bf32_ny.resize(nbf); h.bf32_area.resize(nbf);
     h.bf32_d1x.resize(nbf); h.bf32_d1y.resize(nbf);
-    h.bf32_d2x.assign(nbf, 0.0f); h.bf32_d2y.assign(nbf, 0.0f);
+    h.bf32_d2x.resize(nbf); h.bf32_d2y.resize(nbf);
     if (ns) {
-      h.bf32_lx.assign(nbf, 0.0f); h.bf32_ly.assign(nbf, 0.0f); h.bf32_dist.assign(nbf, 0.0f);
+      h.bf32_lx.resize(nbf); h.bf32_ly.resize(nbf); h.bf32_dist.resize(nbf);
     }
-    for (int32_t b = 0; b < h.nblocks; ++b) {
+    parallel_for(h.nblocks, threads, [&](int64_t b0, int64_t b1) {
+    for (int32_t b = (int32_t)b0; b < (int32_t)b1; ++b) {
       size_t e = h.bf_ptr[b];
       for (int32_t f : bfaces[b]) {
         const int32_t c1 = h.fc1[f], c2 = h.fc2[f];
@@ -500,6 +577,13 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
         const double dx1 = h.fcx[f] - h.ccx[c1], dy1 = h.fcy[f] - h.ccy[c1];
         h.bf32_d1x[e] = (float)dx1;
         h.bf32_d1y[e] = (float)dy1;
+        h.bf32_d2x[e] = 0.0f;
+        h.bf32_d2y[e] = 0.0f;
+        if (ns) {
+          h.bf32_lx[e] = 0.0f;
+          h.bf32_ly[e] = 0.0f;
+          h.bf32_dist[e] = 0.0f;
+        }
         if (c2 >= 0) {
           h.bf32_d2x[e] = (float)(h.fcx[f] - h.ccx[c2]);
           h.bf32_d2y[e] = (float)(h.fcy[f] - h.ccy[c2]);
@@ -516,11 +600,13 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
         ++e;
       }
     }
+    });
   }
-  auto permute = [&](const std::vector<double>& src, std::vector<double>& dst, double pad) {
-    dst.assign(h.ncp, pad);
-    for (int32_t i = 0; i < h.ncp; ++i)
-      if (h.perm[i] >= 0) dst[i] = src[h.perm[i]];
+  auto permute = [&](const hvec<double>& src, hvec<double>& dst, double pad) {
+    dst.resize(h.ncp);
+    parallel_for(h.ncp, threads, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) dst[i] = h.perm[i] >= 0 ? src[h.perm[i]] : pad;
+    });
   };
   permute(h.ccx, h.c_ccx, 0.0);
   permute(h.ccy, h.c_ccy, 0.0);

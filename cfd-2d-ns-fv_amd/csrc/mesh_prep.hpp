@@ -24,6 +24,8 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <memory>
+#include <utility>
 #include <vector>
 
 #include "fvk.h"
@@ -43,6 +45,25 @@ constexpr int kChunk = 64;
 inline int32_t slot_code_interior(int32_t nb, bool owner) { return (nb << 1) | (owner ? 1 : 0); }
 inline int32_t slot_code_boundary(int32_t src, int kind) { return ~((src << 2) | kind); }
 
+// Allocator whose resize() leaves new elements default-initialised (for the
+// arithmetic types here: not written), so that multi-gigabyte arrays are
+// not zero-filled by one thread before the parallel loops that fill every
+// element.
+template <typename T>
+struct UninitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind { using other = UninitAlloc<U>; };
+  UninitAlloc() = default;
+  template <typename U>
+  UninitAlloc(const UninitAlloc<U>&) noexcept {}
+  template <typename U>
+  void construct(U* p) noexcept { ::new (static_cast<void*>(p)) U; }
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) { ::new (static_cast<void*>(p)) U(std::forward<A>(a)...); }
+};
+template <typename T>
+using hvec = std::vector<T, UninitAlloc<T>>;
+
 struct HostMesh {
   int32_t np = 0, nf = 0, nc = 0, nb = 0;
   int32_t blk = 256;        // cells per block
@@ -55,27 +76,27 @@ struct HostMesh {
   // original numbering (class_mesh fields)
   std::vector<int32_t> fc1, fc2;      // fc2 after BC-kind substitution
   std::vector<int32_t> cf_ptr, cf_idx;
-  std::vector<double> area, nx, ny, fcx, fcy;
-  std::vector<double> vol, ccx, ccy, ixx, iyy, ixy;
+  hvec<double> area, nx, ny, fcx, fcy;
+  hvec<double> vol, ccx, ccy, ixx, iyy, ixy;
   // renumbering
   std::vector<int32_t> perm;   // ncp: new -> old, -1 for padding
   std::vector<int32_t> iperm;  // nc: old -> new
   // slot table (new numbering, padded)
   std::vector<int32_t> chunk_off;     // nchunks + 1, in slots
   std::vector<uint8_t> deg;           // ncp (0 for padding)
-  std::vector<int32_t> code;          // per slot, global neighbour index
+  hvec<int32_t> code;                 // per slot, global neighbour index
   // block-local codes are 16-bit: local indices stay below blk + max_halo and
   // block face indices below max_bfaces, both bounded far below 2^14 by LDS
-  std::vector<int16_t> lcode;         // per slot: (block-local neighbour << 1) | owner, or ~kind
-  std::vector<int16_t> sface;         // per slot, (index in the block's face list << 1) | owner
-  std::vector<double> s_nx, s_ny, s_area, s_fcx, s_fcy;   // per slot
+  hvec<int16_t> lcode;                // per slot: (block-local neighbour << 1) | owner, or ~kind
+  hvec<int16_t> sface;                // per slot, (index in the block's face list << 1) | owner
+  hvec<double> s_nx, s_ny, s_area, s_fcx, s_fcy;   // per slot (0 in padding slots)
   // per cell (new numbering, padded)
-  std::vector<double> c_ccx, c_ccy, c_vol, c_ixx, c_iyy, c_ixy;
+  hvec<double> c_ccx, c_ccy, c_vol, c_ixx, c_iyy, c_ixy;
   // per block
   std::vector<int32_t> halo_ptr, halo_idx;            // halo cells (global new index)
   std::vector<int32_t> bf_ptr;                        // block face lists
-  std::vector<int16_t> bf_a, bf_b;                    // local owner; local neighbour or ~kind
-  std::vector<double> bf_nx, bf_ny, bf_area, bf_fcx, bf_fcy;
+  hvec<int16_t> bf_a, bf_b;                           // local owner; local neighbour or ~kind
+  hvec<double> bf_nx, bf_ny, bf_area, bf_fcx, bf_fcy;
   // FP32-mixed only: the single-precision geometry the FP32 kernels use,
   // each value computed in FP64 from the arrays above and rounded once
   // (exactly what an FP32 kernel would compute from them):
@@ -83,13 +104,15 @@ struct HostMesh {
   //   per block face  normal, length, face centre - owner / neighbour
   //   centroid; NS: unit owner->neighbour vector and centroid distance
   //   (interior), wall distance of the owner centroid (no-slip walls)
-  std::vector<float> s_dfx32, s_dfy32;
-  std::vector<float> bf32_nx, bf32_ny, bf32_area, bf32_d1x, bf32_d1y, bf32_d2x, bf32_d2y;
-  std::vector<float> bf32_lx, bf32_ly, bf32_dist;
+  hvec<float> s_dfx32, s_dfy32;
+  hvec<float> bf32_nx, bf32_ny, bf32_area, bf32_d1x, bf32_d1y, bf32_d2x, bf32_d2y;
+  hvec<float> bf32_lx, bf32_ly, bf32_dist;
   std::string warnings;
 };
 
-// Returns FVK_OK or an FVK_ERR_* code with *err set.
-int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* hm, std::string* err);
+// Returns FVK_OK or an FVK_ERR_* code with *err set.  `threads` host threads
+// share the work (1: serial); the result does not depend on it, bit for bit.
+int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* hm, std::string* err,
+                 int threads = 1);
 
 }  // namespace fvk

@@ -42,9 +42,7 @@ def _expected_files():
     return sorted(fn for fn in os.listdir(CASE) if "__" in fn)
 
 
-@pytest.mark.parametrize("extra", ["", "\n[gpu]\n    kernel_variant = 1\n    renumber = 0\n"])
-def test_cli_outputs_match_reference_program(tmp_path, extra):
-    out = _run(tmp_path, extra)
+def _check_outputs(tmp_path, out):
     for fn in _expected_files():
         sub, name = fn.split("__", 1)
         got = (tmp_path / sub / name).read_text()
@@ -59,6 +57,27 @@ def test_cli_outputs_match_reference_program(tmp_path, extra):
                  if l.startswith("iter:")]
     got_iters = [l for l in out.splitlines() if l.startswith("iter:")]
     assert got_iters == ref_iters
+
+
+@pytest.mark.parametrize("extra", ["", "\n[gpu]\n    kernel_variant = 1\n    renumber = 0\n"])
+def test_cli_outputs_match_reference_program(tmp_path, extra):
+    _check_outputs(tmp_path, _run(tmp_path, extra))
+
+
+def test_cli_grid_cache(tmp_path):
+    """grid_cache = 1: the first run parses the text and writes
+    grid.in.fvkgrid, the second reads that binary copy; both reproduce the
+    reference program's files."""
+    extra = "\n[gpu]\n    grid_cache = 1\n"
+    _check_outputs(tmp_path, _run(tmp_path, extra))
+    cache = tmp_path / "grid.in.fvkgrid"
+    assert cache.exists()
+    stamp = cache.stat().st_mtime_ns
+    r = subprocess.run([CLI_PATH, "input.in"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert cache.stat().st_mtime_ns == stamp    # read, not rewritten
+    _check_outputs(tmp_path, r.stdout)
 
 
 def test_cli_restart_continues(tmp_path):

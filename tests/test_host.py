@@ -66,6 +66,19 @@ def test_host_geometry_bit_identical_to_reference(name):
         assert np.array_equal(a.view(np.uint64), ref.view(np.uint64)), key
 
 
+def test_parallel_preparation_is_bit_identical(monkeypatch):
+    """The mesh preparation's threads change no bit of the geometry (ramp
+    512x256: large enough for every parallel loop to split)."""
+    g = gg.ramp()
+    case = inputs.Case(poly=1, limiter="SQUEEZE", BC1="SLIP_WALL", BC2="FREESTREAM")
+    out = {}
+    for t in ("1", "8"):
+        monkeypatch.setenv("FVK_PREP_THREADS", t)
+        out[t] = {k: F.host_geometry(g, case, k) for k in GEOM_FACE + GEOM_CELL}
+    for k in GEOM_FACE + GEOM_CELL:
+        assert np.array_equal(out["1"][k].view(np.uint64), out["8"][k].view(np.uint64)), k
+
+
 def _mesh_err(grid, case):
     with pytest.raises(F.FvkError) as e:
         F.host_geometry(grid, case, "cell_vol")
@@ -146,6 +159,50 @@ def test_grid_file_round_trip(tmp_path):
         assert np.array_equal(getattr(g, k), getattr(h, k))
     for k in ("face_p1", "face_p2", "face_c1", "face_c2", "bc_marker"):
         assert np.array_equal(getattr(g, k), getattr(h, k))
+
+
+def _read_grid_cache(path):
+    """The CLI's binary grid cache: 40-byte header, then x, y, p1, p2, c1, c2."""
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"FVKGRID1"
+    npt, nf, nc = np.frombuffer(raw, np.int32, 3, 24)
+    x = np.frombuffer(raw, np.float64, npt, 40)
+    y = np.frombuffer(raw, np.float64, npt, 40 + 8 * npt)
+    ids = np.frombuffer(raw, np.int32, 4 * nf, 40 + 16 * npt).reshape(4, nf)
+    return int(nc), x, y, ids
+
+
+def test_cli_parallel_grid_reader_and_cache(tmp_path):
+    """`fv2d --cache-grid` parses a grid file with several threads (the ramp's
+    14 MB text is split) into exactly the generator's arrays, and a run with
+    the cache reads the binary copy only while the text keeps its size and
+    modification time."""
+    cli = os.path.join(os.path.dirname(F.LIB_PATH), "fv2d")
+    g = gg.ramp()
+    p = str(tmp_path / "ramp.in")
+    g.write(p)
+    assert os.path.getsize(p) > (4 << 20)
+    r = subprocess.run([cli, "--cache-grid", p], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+    assert f"{g.num_points} Points" in r.stdout and f"{g.num_cells} Cells" in r.stdout
+    nc, x, y, ids = _read_grid_cache(p + ".fvkgrid")
+    assert nc == g.num_cells
+    assert np.array_equal(x.view(np.uint64), g.x.view(np.uint64))
+    assert np.array_equal(y.view(np.uint64), g.y.view(np.uint64))
+    assert np.array_equal(ids[0], g.face_p1) and np.array_equal(ids[1], g.face_p2)
+    assert np.array_equal(ids[2], g.face_c1) and np.array_equal(ids[3], g.reader_c2())
+    # same size and time stamp: the cache is used, even over a text that no
+    # longer parses; a new time stamp sends the reader back to the text
+    st = os.stat(p)
+    with open(p, "r+b") as f:
+        f.seek(st.st_size // 2)
+        f.write(b"#")
+    os.utime(p, ns=(st.st_atime_ns, st.st_mtime_ns))
+    r2 = subprocess.run([cli, "--cache-grid", p], capture_output=True, text=True)
+    assert r2.returncode == 0 and r2.stdout == r.stdout
+    os.utime(p, ns=(st.st_atime_ns, st.st_mtime_ns + 1000))
+    r3 = subprocess.run([cli, "--cache-grid", p], capture_output=True, text=True)
+    assert r3.returncode != 0 and "not a number" in r3.stdout
 
 
 def test_reference_grid_file_reads():
