@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <sstream>
 #include <thread>
@@ -546,6 +547,47 @@ void monitor(int ndx, const Grid& g, const Geometry& geo, const Inputs& in, cons
   out.close();
   std::ofstream rh("sol/res_history.dat", std::ios_base::app | std::ios_base::out);
   rh << ndx << " " << res[0] << " " << res[1] << " " << res[2] << " " << res[3] << std::endl;
+  rh.close();
+  double cl = 0, cd = 0;
+  wall_forces(g, geo, in, q, grad, &cl, &cd);
+  std::ofstream fo("sol/forces.dat", std::ios_base::app | std::ios_base::out);
+  fo << std::setprecision(10) << ndx << " " << cl << " " << cd << std::endl;
+}
+
+void wall_forces(const Grid& g, const Geometry& geo, const Inputs& in, const double* q,
+                 const double* grad, double* cl, double* cd) {
+  const int32_t nc = g.nc;
+  // the wall group of readmesh.cpp:356-360: BC1 when its kind is a wall, else BC2
+  int32_t group = 0;
+  if (in.BC1 == FVK_BC_SLIP_WALL || in.BC1 == FVK_BC_NO_SLIP_WALL) group = -1;
+  else if (in.BC2 == FVK_BC_SLIP_WALL || in.BC2 == FVK_BC_NO_SLIP_WALL) group = -2;
+  const bool noslip = (group == -1 ? in.BC1 : in.BC2) == FVK_BC_NO_SLIP_WALL;
+  double fx = 0, fy = 0;
+  for (int32_t f : geo.bc_faces) {
+    if (group == 0 || g.c2[f] != group) continue;
+    const int32_t c = g.c1[f];
+    const double dx = geo.fcx[f] - geo.ccx[c], dy = geo.fcy[f] - geo.ccy[c];
+    double qf[4];
+    for (int k = 0; k < 4; ++k) {
+      qf[k] = q[(size_t)k * nc + c];
+      if (grad)
+        qf[k] += grad[(size_t)k * nc + c] * dx + grad[(size_t)(4 + k) * nc + c] * dy;
+    }
+    const double P = (qf[3] - 0.5 * (qf[1] * qf[1] + qf[2] * qf[2]) / qf[0]) * (in.gamma - 1);
+    fx += P * geo.area[f] * geo.nx[f];
+    fy += P * geo.area[f] * geo.ny[f];
+    if (in.eqn == 2 && noslip) {   // mu * (centroid velocity) / (normal distance), along it
+      const double dn = std::sqrt(dx * dx + dy * dy);
+      const double rho = q[c];
+      fx += in.mu * (q[(size_t)nc + c] / rho) / dn * geo.area[f];
+      fy += in.mu * (q[2 * (size_t)nc + c] / rho) / dn * geo.area[f];
+    }
+  }
+  const double vinf2 = in.u * in.u + in.v * in.v, vinf = std::sqrt(vinf2);
+  const double ex = in.u / vinf, ey = in.v / vinf;
+  const double qdyn = 0.5 * in.rho * vinf2 * in.length;
+  *cd = (fx * ex + fy * ey) / qdyn;
+  *cl = (-fx * ey + fy * ex) / qdyn;
 }
 
 void write_tecplot(int ndx, const Grid& g, const Geometry& geo, const Inputs& in, const double* q,

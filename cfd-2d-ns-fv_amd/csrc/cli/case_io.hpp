@@ -60,9 +60,21 @@ bool load_geometry(fvk_ctx* ctx, const Grid& g, const Inputs& in, Geometry* geo,
 void write_mesh_dumps(const Grid& g, const Geometry& geo);
 
 // save_sol.cpp:16-103 monitor block: CL, CD and the residual norms, appended
-// to sol/output.dat and sol/res_history.dat and echoed to stdout.
+// to sol/output.dat and sol/res_history.dat and echoed to stdout; then this
+// build's own lift and drag coefficients (wall_forces), appended to
+// sol/forces.dat with 10 significant digits.
 void monitor(int ndx, const Grid& g, const Geometry& geo, const Inputs& in, const double* q,
              const double* grad, const double res[4]);
+// Lift and drag coefficients of the wall group (BC1 if it is a wall, else
+// BC2), per unit span referred to `length`.  Unlike save_sol.cpp:20-75,
+// which truncates centroids and face centres to int, the wall state is the
+// owner's state reconstructed to the true face centre with its limited
+// gradient (grad may be null: first order), pressure acts along the outward
+// normal, and no-slip walls add the shear mu * |u_c| / d_n along the
+// centroid velocity (the wall gradient of reconstruction.cpp:110-123).
+// q, grad: component-major as fvk_get_state returns them.
+void wall_forces(const Grid& g, const Geometry& geo, const Inputs& in, const double* q,
+                 const double* grad, double* cl, double* cd);
 // save_sol.cpp:105-234 Tecplot volume and surface files
 void write_tecplot(int ndx, const Grid& g, const Geometry& geo, const Inputs& in, const double* q,
                    const double* grad);

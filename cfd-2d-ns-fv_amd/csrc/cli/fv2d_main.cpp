@@ -54,6 +54,8 @@ int main(int argc, char** argv) {
     if (rh.is_open()) rh << "#iter, res1, res2, res3, res4" << std::endl;
     std::ofstream out("sol/output.dat");
     if (out.is_open()) out << "#iter, CL, CD" << std::endl;
+    std::ofstream fo("sol/forces.dat");
+    if (fo.is_open()) fo << "#iter, CL, CD (wall integral at the true face centres)" << std::endl;
   }
   const fvk_mesh mesh = fv2d::as_mesh(grid);
   const fvk_options opt = fv2d::as_options(in);

@@ -80,6 +80,32 @@ def test_cli_grid_cache(tmp_path):
     _check_outputs(tmp_path, r.stdout)
 
 
+def test_cli_wall_forces_match_the_python_integrator(tmp_path):
+    """sol/forces.dat (this build's wall integral at the true face centres)
+    holds, at every monitor step, the lift and drag coefficients that
+    fv2d.analysis computes from the same state and limited gradients."""
+    import numpy as np
+
+    from fv2d import analysis, gridgen as gg, inputs
+    from fv2d.lib import Solver
+
+    _run(tmp_path)
+    rows = [l.split() for l in (tmp_path / "sol" / "forces.dat").read_text().splitlines()
+            if not l.startswith("#")]
+    assert [int(r[0]) for r in rows] == [10, 20, 30, 40, 50, 60]
+    grid = gg.read_grid(os.path.join(CASE, "grid.in"))
+    case = inputs.Case(poly=1, limiter="SQUEEZE", dt_type="LOCAL", CFL=0.8, P=100000.0,
+                       rho=1.225, u=169.03085094570332, v=0.0, BC1="SLIP_WALL",
+                       BC2="FREESTREAM")
+    s = Solver(grid, case)
+    s.init_freestream()
+    for r in rows:
+        s.step(10)
+        q, g = s.state_and_grad()
+        _, _, cl, cd = analysis.wall_coefficients(grid, case, q, g)
+        assert np.allclose([float(r[1]), float(r[2])], [cl, cd], rtol=2e-9, atol=1e-12), r
+
+
 def test_cli_restart_continues(tmp_path):
     """restart = 1 reads sol/restart_sol.dat and continues from nmax+1
     (main.cpp:82-106); the run must finish and append to the histories."""
