@@ -19,8 +19,11 @@ stream has drained).
   the steps its FP64 run took); per-instance FP32-mixed errors against FP64.
 --config ramp|channel|cylinder|airfoil: one mesh (the 512x256 ramp, the
   64x32 bump channel, the 1024x512 NS O-grid, the 7.8M-cell Delaunay-dual
-  airfoil), FP32-mixed and FP64, K steps each, with the FP64 sensitivity
-  floor and (wall cases) wall pressure and force coefficients.
+  airfoil), FP32-mixed and FP64, K steps each (the ramp: to a 1e-8 density
+  residual drop, at most K = 20000), with the density residual level reached,
+  the FP64 sensitivity floor and (wall cases) wall pressure and force
+  coefficients, including the mean lift and drag over the second half of the
+  run.
 
 Before the timed runs a launch-shape autotuner times a few steps of each
 candidate (cells per block, XCD block order) per precision and keeps the
@@ -287,18 +290,50 @@ def roofline(per_launch, algo, head, args, child):
 # ---------------------------------------------------------------------------
 # single mesh
 # ---------------------------------------------------------------------------
-def run_single(prec: str, args, grid, case, opts):
-    """Warm up, time args.steps steps, then time each kernel kind per launch."""
+def run_single(prec: str, args, grid, case, opts, walls=None):
+    """Step 1, then the rest of the warm-up, then the timed steps in chunks of
+    args.chunk: the clock runs only around the stepping calls.  Between
+    chunks, the density residual is checked against its value after step 1
+    (stop at args.drop_single when set), and with `walls` (the mesh's host
+    geometry for the wall integral; True: take it from this run's solver)
+    the lift and drag coefficients are sampled.  Then each kernel kind is
+    timed per launch."""
+    from fv2d import analysis
     nc = grid.num_cells
     s = Solver(grid, case, kernel_variant=args.variant, precision=PRECISIONS[prec], **opts)
     s.init_freestream()
-    s.step(args.warmup)
-    t0 = time.perf_counter()
-    s.step(args.steps)
-    wall = time.perf_counter() - t0
+    if walls is True:
+        walls = analysis.wall_geometry(grid, case, s)
+    s.step(1)
+    r0 = float(s.residual_l2()[0])
+    s.step(max(0, args.warmup - 1))
+    done, wall, samples, reached = 0, 0.0, [], False
+    while done < args.steps and not reached:
+        n = min(args.chunk, args.steps - done)
+        t0 = time.perf_counter()
+        s.step(n)
+        wall += time.perf_counter() - t0
+        done += n
+        r = float(s.residual_l2()[0])
+        reached = bool(args.drop_single) and r <= args.drop_single * r0
+        if walls:
+            q, g = s.state_and_grad()
+            _, _, cl, cd = analysis.wall_coefficients(grid, case, q, g, geo=walls)
+            samples.append((args.warmup + done, cl, cd))
+    conv = {"steps": args.warmup + done, "res0_rho": r0, "res_end_rho": r,
+            "drop": r / r0 if r0 > 0 else None, "drop_target": args.drop_single or None,
+            "reached": reached}
     per_launch = {k: v for k, v in s.time_kernels(args.kernel_reps).items() if v > 0}
-    return {"solver": s, "wall": wall, "value": nc * 3 * args.steps / wall,
-            "per_launch": per_launch}
+    return {"solver": s, "wall": wall, "value": nc * 3 * done / wall, "steps": done,
+            "per_launch": per_launch, "convergence": conv, "forces": samples,
+            "walls": walls or None}
+
+
+def mean_coefficients(samples):
+    """Mean lift and drag over the second half of the samples."""
+    half = samples[len(samples) // 2:]
+    return (float(np.mean([c[1] for c in half])), float(np.mean([c[2] for c in half])),
+            len(half), half[0][0] if half else None)
 
 
 def bench_single(args):
@@ -319,10 +354,15 @@ def bench_single(args):
                                                kernel_variant=args.variant)
             tuned[prec] = {"choice": best,
                            "ms_per_step": [{**o, "ms": round(ms, 3)} for o, ms in table]}
+    walls = case.BC1 in ("SLIP_WALL", "NO_SLIP_WALL")   # the geometry, from the first run
     runs = {}
     for prec in precs:
-        runs[prec] = run_single(prec, args, grid, case, tuned[prec]["choice"] if tuned else opts)
-        log(f"{prec} {runs[prec]['value']:.4g} cell-stage/s")
+        runs[prec] = run_single(prec, args, grid, case, tuned[prec]["choice"] if tuned else opts,
+                                walls)
+        walls = runs[prec]["walls"]
+        c = runs[prec]["convergence"]
+        log(f"{prec} {runs[prec]['value']:.4g} cell-stage/s, {c['steps']} steps, "
+            f"density residual drop {c['drop']:.3g}")
     head = "fp32" if "fp32" in runs else "fp64"
     r = runs[head]
     algo = algorithmic_bytes(case, nc, nf, s=4 if head == "fp32" else 8)
@@ -347,24 +387,40 @@ def bench_single(args):
            "launch_options": chosen, "autotune": tuned or None,
            "state_finite": bool(np.isfinite(q).all())}
     out = {"metric": METRIC, "value": r["value"], "unit": "cell-stage/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": r["wall"] * 1e3 / args.steps, "higher_is_better": True,
+           "steps": r["steps"], "warmup": args.warmup,
+           "ms_per_step": r["wall"] * 1e3 / r["steps"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32" if head == "fp32" else "f64",
-           "data": "synthetic", "config": cfg, "roofline": roof}
+           "data": "synthetic", "config": cfg, "roofline": roof,
+           "convergence": {p: runs[p]["convergence"] for p in runs}}
+    if walls:
+        out["mean_coefficients"] = {}
+        for p in runs:
+            cl, cd, n, first = mean_coefficients(runs[p]["forces"])
+            out["mean_coefficients"][p] = {"cl": cl, "cd": cd, "samples": n,
+                                           "from_step": first, "every": args.chunk}
+        if "fp64" in runs and head == "fp32":
+            m32, m64 = out["mean_coefficients"]["fp32"], out["mean_coefficients"]["fp64"]
+            out["mean_coefficients"]["cd_rel_err"] = abs(m32["cd"] - m64["cd"]) / abs(m64["cd"])
+    if args.config == "cylinder":
+        out["strouhal"] = None
+        out["strouhal_why"] = ("not defined for this scheme: its GLOBAL time step is "
+                               "2*CFL*V_i/max(sum), proportional to each cell's volume "
+                               "(ssp_rk3.cpp:38-42), so cells advance through different "
+                               "physical times")
     if "fp64" in runs and head == "fp32":
         r64 = runs["fp64"]
-        out["fp64"] = {"value": r64["value"], "ms_per_step": r64["wall"] * 1e3 / args.steps,
+        out["fp64"] = {"value": r64["value"], "ms_per_step": r64["wall"] * 1e3 / r64["steps"],
                        "avg_launch_us": {k: round(v, 2) for k, v in r64["per_launch"].items()}}
         out["speedup_vs_fp64"] = round(r["value"] / r64["value"], 4)
         q64 = r64["solver"].state()
         err = errors_vs_fp64(q, q64, case.gamma)
         out["error_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in err.items()}
-        out["error_vs_fp64"]["after_steps"] = args.warmup + args.steps
+        out["error_vs_fp64"]["after_steps"] = {p: runs[p]["convergence"]["steps"] for p in runs}
         # the FP64 scheme's own response to an FP32-rounding-sized perturbation
         log("FP64 sensitivity floor run")
         fl = Solver(grid, case)
         fl.set_state(perturbed_freestream(grid, case))
-        fl.step(args.warmup + args.steps)
+        fl.step(runs["fp64"]["convergence"]["steps"])
         floor = errors_vs_fp64(fl.state(), q64, case.gamma)
         out["fp64_sensitivity_floor"] = {k: float(f"{v:.3e}") for k, v in floor.items()}
         out["fp64_sensitivity_floor"]["how"] = ("FP64 from Q_inf(1+1e-7 u), u in [-1,1], "
@@ -377,10 +433,9 @@ def bench_single(args):
             _, g32 = r["solver"].state_and_grad()
             _, g64 = r64["solver"].state_and_grad()
             _, gfl = fl.state_and_grad()
-            geo = analysis.wall_geometry(grid, case, r64["solver"])
-            walls = analysis.compare_walls(grid, case, q, g32, q64, g64, geo)
-            wfl = analysis.compare_walls(grid, case, fl.state(), gfl, q64, g64, geo)
-            out["wall_vs_fp64"] = {k: float(f"{v:.4e}") for k, v in walls.items()}
+            cmp = analysis.compare_walls(grid, case, q, g32, q64, g64, walls)
+            wfl = analysis.compare_walls(grid, case, fl.state(), gfl, q64, g64, walls)
+            out["wall_vs_fp64"] = {k: float(f"{v:.4e}") for k, v in cmp.items()}
             out["wall_vs_fp64"]["floor_cd_rel_err"] = float(f"{wfl['cd_rel_err']:.3e}")
             out["wall_vs_fp64"]["floor_cp_linf_err"] = float(f"{wfl['cp_linf_err']:.3e}")
     if not args.no_cpu:
@@ -545,6 +600,11 @@ def main():
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--drop", type=float, default=1e-6)
     ap.add_argument("--check-every", type=int, default=100)
+    ap.add_argument("--chunk", type=int, default=None,
+                    help="single mesh: steps between residual checks and force samples")
+    ap.add_argument("--drop-single", type=float, default=None,
+                    help="single mesh: stop at this density-residual drop (default: ramp "
+                         "1e-8, else none)")
     ap.add_argument("--no-seq", action="store_true", help="batch: skip the sequential FP64 runs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=None,
@@ -581,7 +641,12 @@ def main():
         sys.exit(2)
     batch = args.config == "batch"
     if args.steps is None:
-        args.steps = {"batch": 1000, "airfoil": 500, "cylinder": 20000}.get(args.config, 2000)
+        args.steps = {"batch": 1000, "airfoil": 500, "cylinder": 20000,
+                      "ramp": 20000}.get(args.config, 2000)
+    if args.chunk is None:
+        args.chunk = {"airfoil": 500, "channel": 500}.get(args.config, 1000)
+    if args.drop_single is None:
+        args.drop_single = 1e-8 if args.config == "ramp" else 0.0
     if args.warmup is None:
         args.warmup = 10 if batch else 50
     if args.kernel_reps is None:
