@@ -345,13 +345,15 @@ def bench_single(args):
     log(f"{grid.num_cells} cells, {grid.num_faces} faces")
     nc, nf = grid.num_cells, grid.num_faces
     precs = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
-    opts = {"block_size": args.block_size, "xcd_remap": args.xcd_remap}
+    opts = {"block_size": args.block_size, "xcd_remap": args.xcd_remap,
+            "accumulate": args.accumulate}
     tuned = {}
     if args.autotune and args.variant in (0, 3):
         from fv2d import autotune
         for prec in precs:
             best, table = autotune.tune_single(grid, case, PRECISIONS[prec], log=log,
-                                               kernel_variant=args.variant)
+                                               kernel_variant=args.variant,
+                                               extra=autotune.ATOMIC if prec == "fp32" else None)
             tuned[prec] = {"choice": best,
                            "ms_per_step": [{**o, "ms": round(ms, 3)} for o, ms in table]}
     walls = case.BC1 in ("SLIP_WALL", "NO_SLIP_WALL")   # the geometry, from the first run
@@ -370,7 +372,7 @@ def bench_single(args):
     child = ["--no-cpu", "--no-traffic", "--no-autotune", "--steps", "5", "--warmup", "2",
              "--kernel-reps", "5", "--config", args.config, "--variant", str(args.variant),
              "--block-size", str(chosen["block_size"]), "--xcd-remap", str(chosen["xcd_remap"]),
-             "--precision", head]
+             "--accumulate", str(chosen["accumulate"]), "--precision", head]
     tmp = None
     if args.config == "airfoil" and not args.no_traffic:
         # the counter runs load this mesh instead of generating it again
@@ -491,12 +493,14 @@ def bench_batch(args):
     nf = np.array([i.grid.num_faces for i in plan], dtype=np.int64)
     log(f"{name}: {len(cases)} instances, {int(nc.sum())} cells")
     precs = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
-    opts = {"block_size": args.block_size, "xcd_remap": args.xcd_remap}
+    opts = {"block_size": args.block_size, "xcd_remap": args.xcd_remap,
+            "accumulate": args.accumulate}
     tuned = {}
     if args.autotune:
         from fv2d import autotune
         for prec in precs:
-            best, table = autotune.tune_batch(cases, PRECISIONS[prec], log=log)
+            best, table = autotune.tune_batch(cases, PRECISIONS[prec], log=log,
+                                              extra=autotune.ATOMIC if prec == "fp32" else None)
             tuned[prec] = {"choice": best,
                            "ms_per_step": [{**o, "ms": round(ms, 3)} for o, ms in table]}
     runs = {}
@@ -513,7 +517,8 @@ def bench_batch(args):
     child = ["--no-cpu", "--no-traffic", "--no-seq", "--no-autotune", "--steps", "4",
              "--warmup", "0", "--kernel-reps", "3", "--config", "batch",
              "--instances", str(args.instances), "--block-size", str(chosen["block_size"]),
-             "--xcd-remap", str(chosen["xcd_remap"]), "--precision", head]
+             "--xcd-remap", str(chosen["xcd_remap"]), "--accumulate", str(chosen["accumulate"]),
+             "--precision", head]
     roof = roofline(r["per_launch"], algo, head, args, child)
     steps = r["steps"]
     cfg = {"workload": name, "instances": len(cases), "cells": int(nc.sum()),
@@ -618,6 +623,8 @@ def main():
     ap.add_argument("--xcd-remap", type=int, default=-1,
                     help="XCD block order: -1 automatic, 0 off, 1 contiguous ranges, "
                          "G>1 runs of G blocks")
+    ap.add_argument("--accumulate", type=int, default=0,
+                    help="with --no-autotune: 0 owner gather (bit-exact), 1 atomic scatter")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the launch-shape autotuner (use --block-size/--xcd-remap)")
     ap.add_argument("--grid-npz", default=None,

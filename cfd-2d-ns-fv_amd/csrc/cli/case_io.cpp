@@ -136,6 +136,7 @@ bool read_inputs(const std::string& path, Inputs* in, std::string* log, std::str
         if (key == "xcd_remap") in->xcd_remap = std::stoi(val);
         if (key == "kernel_variant") in->kernel_variant = std::stoi(val);
         if (key == "grid_cache") in->grid_cache = std::stoi(val);
+        if (key == "accumulate") in->accumulate = std::stoi(val);
         if (key == "precision") in->precision = (val == "FP32_MIXED") ? FVK_FP32_MIXED : (val == "FP64" ? FVK_FP64 : -1);
       }
     }
@@ -402,6 +403,7 @@ fvk_options as_options(const Inputs& in) {
   o.renumber = in.renumber;
   o.block_size = in.block_size;
   o.xcd_remap = in.xcd_remap;
+  o.accumulate = in.accumulate;
   o.kernel_variant = in.kernel_variant;
   o.precision = in.precision;
   return o;

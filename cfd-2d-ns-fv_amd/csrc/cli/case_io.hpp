@@ -24,6 +24,7 @@ struct Inputs {
   // optional trailing [gpu] section (read only after the nine standard ones)
   int renumber = 1, block_size = 0, xcd_remap = -1, kernel_variant = 0;
   int grid_cache = 0;          // [gpu] grid_cache = 1: binary grid cache beside the grid file
+  int accumulate = 0;          // [gpu] accumulate = 1: atomic-scatter residual accumulation
   int precision = FVK_FP64;    // [gpu] precision = FP64 | FP32_MIXED
 };
 

@@ -1401,6 +1401,97 @@ void k_pass_b_lds(StageArgsT<T> a) {
   if (live && a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) a.smax_next[inst] = 0ull;
 }
 
+// ---------------------------------------------------------------------------
+// Atomic-scatter accumulation (accumulate = 1), the alternative to the owner
+// gather above.  Pass B computes each face once, in the block that holds the
+// face's owner, and adds its terms to both cells' accumulators with hardware
+// atomics; a cell pass then applies the stage update and clears them.  The
+// sums take their terms in run-time order, so results differ from the
+// gather (and the reference) by summation order only.
+// ---------------------------------------------------------------------------
+template <typename T, int ORDER, bool NS>
+__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
+void k_pass_b_atomic(StageArgsT<T> a) {
+  extern __shared__ unsigned char lds_raw[];
+  constexpr int NQ = ORDER == 2 ? 12 : 4;
+  constexpr bool XY = !kDelta<T>;
+  const DevMesh& m = a.m;
+  int b = 0, inst = 0;
+  if (!block_of(a, logical_block(a.xcd_blocks), b, inst)) return;   // uniform
+  const FlowT<T>& fl = a.flows[inst];
+  LdsCells<T, NQ, XY> L(lds_raw, m.blk + m.max_halo);
+  const int fb = m.bf_ptr[b], nfb = m.bf_ptr[b + 1] - fb;
+  const int hb = m.halo_ptr[b];
+  stage_block(a, b, L);
+  __syncthreads();
+  for (int e = threadIdx.x; e < nfb; e += blockDim.x) {
+    FaceRec<T, NS> f;
+    load_face(m, fb + e, f);
+    if (f.ia >= m.blk) continue;                    // the owner's block takes the face
+    const int g1 = b * m.blk + f.ia;
+    CellQG<T> c1, c2;
+    lds_cell<T, ORDER>(L, f.ia, c1);
+    T x[4], st;
+    if (f.ib >= 0) {
+      lds_cell<T, ORDER>(L, f.ib, c2);
+      FaceGeo<T> g;
+      if constexpr (kDelta<T>) g = f.g;
+      else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, &c2, -1);
+      face_flux<ORDER, NS>(a, fl, c1, &c2, -1, g, x, st);
+      const int g2 = f.ib < m.blk ? b * m.blk + f.ib : m.halo_idx[hb + f.ib - m.blk];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        unsafeAtomicAdd(a.acc + 4 * (size_t)g1 + k, x[k]);
+        unsafeAtomicAdd(a.acc + 4 * (size_t)g2 + k, -x[k]);
+      }
+      if (a.stage == 3) {
+        unsafeAtomicAdd(a.sacc + g1, st);
+        unsafeAtomicAdd(a.sacc + g2, st);
+      }
+    } else {
+      const int kind = (~f.ib) & 3;
+      FaceGeo<T> g;
+      if constexpr (kDelta<T>) g = f.g;
+      else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, nullptr, kind);
+      face_flux<ORDER, NS>(a, fl, c1, (const CellQG<T>*)nullptr, kind, g, x, st);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) unsafeAtomicAdd(a.acc + 4 * (size_t)g1 + k, x[k]);
+      if (a.stage == 3) unsafeAtomicAdd(a.sacc + g1, st);
+    }
+  }
+}
+
+// The stage update from the accumulated sums; clears the accumulators for the
+// next stage.  One block per mesh block (blockDim = blk).
+template <typename T>
+__global__ __launch_bounds__(kMaxBlock) void k_update_atomic(StageArgsT<T> a) {
+  const DevMesh& m = a.m;
+  int b = 0, inst = 0;
+  if (!block_of(a, logical_block(a.xcd_blocks), b, inst)) return;   // uniform
+  const FlowT<T>& fl = a.flows[inst];
+  const int c = b * m.blk + threadIdx.x;
+  double sig_new = 0.0;
+  if (m.deg[c] > 0) {
+    T r[4], q[4], qn[4] = {0, 0, 0, 0};
+    ld4(a.acc + 4 * (size_t)c, r);
+    const T zero[4] = {0, 0, 0, 0};
+    st4(a.acc + 4 * (size_t)c, zero);
+    T sg = 0;
+    if (a.stage == 3) {
+      sg = a.sacc[c];
+      a.sacc[c] = T(0);
+    }
+    load_q(a.qc, c, q);
+    if (a.stage > 1) load_q(a.qn, c, qn);
+    const T sig_old = a.dt_global ? T(1) : a.sigma[c];
+    cell_update(a, fl, inst, c, q, qn, r, sg, m.vol[c], sig_old);
+    sig_new = double(sg);
+  }
+  if (a.stage == 3 && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
+  // the instance's first block clears the maximum stage 3 will build
+  if (a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) a.smax_next[inst] = 0ull;
+}
+
 // Spectral sums of the first step: every face carries |V_inf| and c_inf
 // (class_q.hpp:61-62), accumulated as ssp_rk3.cpp:28-36 does.
 // One thread block per mesh block (blockDim = blk).
@@ -1482,6 +1573,11 @@ size_t lds_bytes_pass_b(const DevMesh& m, int order, int stage, size_t tsize) {
          (stage == 3 ? 5 : 4) * (size_t)m.max_bfaces * tsize;
 }
 
+// cell rows only (the atomic-scatter pass B keeps no face terms in LDS)
+static size_t lds_bytes_cells(const DevMesh& m, int order, size_t tsize) {
+  return lds_cells_bytes(m.blk + m.max_halo, order == 2 ? 12 : 4, tsize, tsize == 8);
+}
+
 template <typename T>
 hipError_t launch_pass_a(const StageArgsT<T>& a0, int squeeze, int block, hipStream_t s) {
   StageArgsT<T> a = a0;
@@ -1511,8 +1607,18 @@ hipError_t launch_pass_b(const StageArgsT<T>& a0, int order, int ns, int block, 
   if (a.variant == kVariantLds) {
     shape(a, a.active ? a.nactive : a.m.nblocks, &grid, &a.xcd_blocks);
     if (grid == 0) return hipSuccess;
-    const size_t lds = lds_bytes_pass_b(a.m, order, a.stage, sizeof(T));
     const dim3 blk(a.m.blk * kPassBThreadsPerCell);
+    if (a.accumulate) {
+      const size_t lds = lds_bytes_cells(a.m, order, sizeof(T));
+      if (order == 1) hipLaunchKernelGGL((k_pass_b_atomic<T, 1, false>), dim3(grid), blk, lds, s, a);
+      else if (ns) hipLaunchKernelGGL((k_pass_b_atomic<T, 2, true>), dim3(grid), blk, lds, s, a);
+      else hipLaunchKernelGGL((k_pass_b_atomic<T, 2, false>), dim3(grid), blk, lds, s, a);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((k_update_atomic<T>), dim3(grid), dim3(a.m.blk), 0, s, a);
+      return hipGetLastError();
+    }
+    const size_t lds = lds_bytes_pass_b(a.m, order, a.stage, sizeof(T));
     if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false>), dim3(grid), blk, lds, s, a);
     else if (ns) hipLaunchKernelGGL((k_pass_b_lds<T, 2, true>), dim3(grid), blk, lds, s, a);
     else hipLaunchKernelGGL((k_pass_b_lds<T, 2, false>), dim3(grid), blk, lds, s, a);

@@ -92,7 +92,7 @@ struct StageArgsT {
   const int32_t* blk_inst;     // per mesh block: instance (batch only, else null)
   const int32_t* active;       // logical block -> mesh block (batch only, else null)
   int32_t nactive;             // active blocks (batch only)
-  int32_t pad_;
+  int32_t accumulate;          // 1: pass B scatters with atomics, a cell pass updates
   int32_t stage;               // 1, 2, 3
   int32_t dt_global;           // time_step_type GLOBAL
   int32_t sentinel;            // lagged face states are the -101 initial values
@@ -112,6 +112,9 @@ struct StageArgsT {
                                   // (GLOBAL dt), FP64 bits
   unsigned long long* smax_next;  // per instance: max spectral sum written by stage 3
   T* res;                      // residual (4 per cell)
+  T* acc;                      // accumulate = 1: face sums per cell (4 per cell), zero
+                               // between stages
+  T* sacc;                     // accumulate = 1: spectral sums per cell at stage 3
 };
 
 // launch shapes: one thread per cell; one thread per (cell, face slot); or

@@ -64,6 +64,8 @@ struct Engine {
   void* g[3] = {nullptr, nullptr, nullptr};   // stage 1..3 limited gradients (8*ncp each)
   void* sigma = nullptr;                      // ncp
   void* res = nullptr;                        // 4*ncp
+  void* acc = nullptr;                        // accumulate = 1: 4*ncp face sums (zero between stages)
+  void* sacc = nullptr;                       // accumulate = 1: ncp spectral sums
   unsigned long long* smax = nullptr;         // [2][n], FP64 bits
   double* partial = nullptr;                  // [4][nblocks]
   int block = 256;
@@ -202,6 +204,11 @@ int check_options(const fvk_options& o, std::string* msg) {
   if (o.precision != FVK_FP64 && o.precision != FVK_FP32_MIXED) { *msg = "precision must be FVK_FP64 or FVK_FP32_MIXED"; return FVK_ERR_OPTION; }
   if (o.kernel_variant < 0 || o.kernel_variant > 3) { *msg = "kernel_variant must be 0, 1, 2 or 3"; return FVK_ERR_OPTION; }
   if (o.renumber < 0 || o.renumber > 2) { *msg = "renumber must be 0, 1 or 2"; return FVK_ERR_OPTION; }
+  if (o.accumulate != 0 && o.accumulate != 1) { *msg = "accumulate must be 0 (gather) or 1 (atomic scatter)"; return FVK_ERR_OPTION; }
+  if (o.accumulate == 1 && o.kernel_variant != 0 && o.kernel_variant != fvk::kVariantLds) {
+    *msg = "accumulate = 1 needs the LDS-staged kernels (kernel_variant 0 or 3)";
+    return FVK_ERR_OPTION;
+  }
   if (o.xcd_remap < -1 || o.xcd_remap > 4096) { *msg = "xcd_remap must be -1 (auto), 0, 1 or a run length 2..4096"; return FVK_ERR_OPTION; }
   if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64)) {
     *msg = "block_size must be 0 or a multiple of 64 in [64, 256]";
@@ -217,10 +224,11 @@ int check_options(const fvk_options& o, std::string* msg) {
 int check_common(const fvk_options& a, const fvk_options& b, std::string* msg) {
   const bool same = a.order == b.order && a.eqn == b.eqn && a.time_step_type == b.time_step_type &&
                     a.precision == b.precision && a.block_size == b.block_size &&
-                    a.kernel_variant == b.kernel_variant && a.xcd_remap == b.xcd_remap;
+                    a.kernel_variant == b.kernel_variant && a.xcd_remap == b.xcd_remap &&
+                    a.accumulate == b.accumulate;
   if (!same) {
     *msg = "batch instances must agree on order, eqn, time_step_type, precision, block_size, "
-           "kernel_variant and xcd_remap";
+           "kernel_variant, xcd_remap and accumulate";
     return FVK_ERR_OPTION;
   }
   if (a.kernel_variant == fvk::kVariantCell || a.kernel_variant == fvk::kVariantSlot) {
@@ -289,6 +297,9 @@ fvk::StageArgsT<T> base_args(Engine* c) {
   a.variant = c->variant;
   a.sigma = static_cast<T*>(c->sigma);
   a.res = static_cast<T*>(c->res);
+  a.accumulate = c->opt.accumulate;
+  a.acc = static_cast<T*>(c->acc);
+  a.sacc = static_cast<T*>(c->sacc);
   const size_t n = c->in.size();
   a.smax_cur = c->smax + c->smax_cur * n;
   a.smax_next = c->smax + (1 - c->smax_cur) * n;
@@ -775,7 +786,7 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
     // above what fits in LDS)
     const bool codes_fit = d.blk + d.max_halo < (1 << 14) && d.max_bfaces < (1 << 14);
     if (need > prop.sharedMemPerBlock || !codes_fit) {
-      if (o0.kernel_variant == fvk::kVariantLds || batched)
+      if (o0.kernel_variant == fvk::kVariantLds || batched || o0.accumulate)
         return set_err(c, FVK_ERR_MESH, "mesh blocks need " + std::to_string(need) +
                                             " bytes of LDS, more than the device allows");
       c->variant = fvk::kVariantCell;
@@ -811,6 +822,12 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   }
   if ((rc = fill(c, c->res, 4 * (int64_t)ncp, 0.0))) return rc;
   if ((rc = fill(c, c->sigma, (int64_t)ncp, 0.0))) return rc;
+  if (c->opt.accumulate) {   // the accumulators start at zero and the cell pass re-zeroes them
+    if ((rc = alloc(&c->acc, 4 * ncp))) return rc;
+    if ((rc = alloc(&c->sacc, ncp))) return rc;
+    if ((rc = fill(c, c->acc, 4 * (int64_t)ncp, 0.0))) return rc;
+    if ((rc = fill(c, c->sacc, (int64_t)ncp, 0.0))) return rc;
+  }
   FVK_HIP(c, hipStreamSynchronize(c->stream));
   return FVK_OK;
 }

@@ -1,9 +1,12 @@
 """Launch-shape autotuner for batches and single meshes.
 
 Builds the solve once per candidate option set (cells per block, cell
-ordering, XCD block order), times a few steps of each after a warm-up, and
-returns the fastest.  Every candidate gives the same results bit for bit
-(the layout tests check that), so the choice only moves time.
+ordering, XCD block order, and optionally the residual accumulation), times a
+few steps of each after a warm-up, and returns the fastest.  The launch-shape
+candidates give the same results bit for bit (the layout tests check that),
+so that choice only moves time; the atomic-scatter accumulation changes the
+summation order, so it is offered only where bit-exactness is not required
+(`extra=ATOMIC`, the FP32-mixed form).
 """
 from __future__ import annotations
 
@@ -16,19 +19,23 @@ from .lib import Batch, Solver
 # (RCM's long ranges give 4-9x larger halos); XCD order: runs of 8 blocks per
 # XCD, none, or one contiguous range per XCD
 DEFAULT_SPACE = {"block_size": [128, 256], "renumber": [1], "xcd_remap": [8, 0, 1]}
+# the atomic-scatter accumulation at the batch's best launch shape
+ATOMIC = [{"block_size": 128, "renumber": 1, "xcd_remap": 8, "accumulate": 1}]
 
 
-def candidates(space=None):
+def candidates(space=None, extra=None):
     space = space or DEFAULT_SPACE
     keys = sorted(space)
     for vals in itertools.product(*(space[k] for k in keys)):
         yield dict(zip(keys, vals))
+    yield from extra or []
 
 
-def tune_batch(cases, precision: int, space=None, warmup: int = 4, steps: int = 20, log=None):
+def tune_batch(cases, precision: int, space=None, warmup: int = 4, steps: int = 20, log=None,
+               extra=None):
     """Fastest option set for this batch: (options, [(options, ms per step)])."""
     table = []
-    for opt in candidates(space):
+    for opt in candidates(space, extra):
         b = Batch(cases, precision=precision, **opt)
         b.init()
         b.step(warmup)
@@ -44,10 +51,10 @@ def tune_batch(cases, precision: int, space=None, warmup: int = 4, steps: int = 
 
 
 def tune_single(grid, case, precision: int, space=None, warmup: int = 4, steps: int = 20,
-                log=None, **fixed):
+                log=None, extra=None, **fixed):
     """Fastest option set for one mesh through fvk_ctx: (options, table)."""
     table = []
-    for opt in candidates(space):
+    for opt in candidates(space, extra):
         s = Solver(grid, case, precision=precision, **fixed, **opt)
         s.init_freestream()
         s.step(warmup)

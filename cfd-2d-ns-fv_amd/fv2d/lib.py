@@ -61,7 +61,7 @@ class FvkOptions(ctypes.Structure):
                 ("rho", ctypes.c_double), ("u", ctypes.c_double), ("v", ctypes.c_double),
                 ("precision", ctypes.c_int32), ("renumber", ctypes.c_int32),
                 ("block_size", ctypes.c_int32), ("xcd_remap", ctypes.c_int32),
-                ("kernel_variant", ctypes.c_int32), ("reserved_", ctypes.c_int32)]
+                ("kernel_variant", ctypes.c_int32), ("accumulate", ctypes.c_int32)]
 
 
 _lib = None
@@ -252,7 +252,7 @@ class Solver:
         o = FvkOptions()
         self._check(lib().fvk_get_options(self.h, ctypes.byref(o)))
         return {"block_size": o.block_size, "xcd_remap": o.xcd_remap,
-                "kernel_variant": o.kernel_variant}
+                "kernel_variant": o.kernel_variant, "accumulate": o.accumulate}
 
     def time_kernels(self, reps: int):
         """Average microseconds per launch of pass A and pass B (stage 2)."""
@@ -368,7 +368,7 @@ class Batch:
         o = FvkOptions()
         self._check(lib().fvk_batch_get_options(self.h, 0, ctypes.byref(o)))
         return {"block_size": o.block_size, "xcd_remap": o.xcd_remap,
-                "kernel_variant": o.kernel_variant}
+                "kernel_variant": o.kernel_variant, "accumulate": o.accumulate}
 
     def checkpoint(self) -> bytes:
         n = ctypes.c_int64()

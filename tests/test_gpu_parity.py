@@ -122,12 +122,12 @@ def test_automatic_launch_options():
     """block_size 0 and xcd_remap -1 resolve by precision and size; explicit
     values are kept; out-of-range orders are refused."""
     g = Golden("ramp_o2_squeeze")
-    auto = {"block_size": 128, "xcd_remap": 1, "kernel_variant": 3}
+    auto = {"block_size": 128, "xcd_remap": 1, "kernel_variant": 3, "accumulate": 0}
     assert Solver(g.grid, g.case).launch_options() == auto
     assert Solver(g.grid, g.case, precision=FP32_MIXED).launch_options() == {**auto,
                                                                              "block_size": 256}
     got = Solver(g.grid, g.case, block_size=192, xcd_remap=5).launch_options()
-    assert got == {"block_size": 192, "xcd_remap": 5, "kernel_variant": 3}
+    assert got == {"block_size": 192, "xcd_remap": 5, "kernel_variant": 3, "accumulate": 0}
     with pytest.raises(FvkError):
         Solver(g.grid, g.case, xcd_remap=-2)
 
