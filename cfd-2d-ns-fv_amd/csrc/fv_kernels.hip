@@ -1510,9 +1510,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_init_sigma(StageArgsT<T> a) {
   block_max_to(double(sg), a.smax_cur + inst);
 }
 
-// One thread block per mesh block (blockDim = blk): partial[k*nblocks + b].
+// One thread block per mesh block (blockDim = blk, grid = ncp / blk, so every
+// thread has a cell): partial[k*nblocks + b].
 template <typename T>
-__global__ __launch_bounds__(kMaxBlock) void k_res_sq(const T* __restrict__ res, int ncp,
+__global__ __launch_bounds__(kMaxBlock) void k_res_sq(const T* __restrict__ res,
                                                       double* __restrict__ partial) {
   __shared__ double ws[4][kMaxBlock / 64];
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1645,7 +1646,7 @@ hipError_t launch_init_sigma(const StageArgsT<T>& a, hipStream_t s) {
 
 template <typename T>
 hipError_t launch_res_sq(const T* res, int32_t ncp, int32_t blk, double* partial, hipStream_t s) {
-  hipLaunchKernelGGL(k_res_sq<T>, dim3(ncp / blk), dim3(blk), 0, s, res, ncp, partial);
+  hipLaunchKernelGGL(k_res_sq<T>, dim3(ncp / blk), dim3(blk), 0, s, res, partial);
   return hipGetLastError();
 }
 
