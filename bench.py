@@ -329,6 +329,61 @@ def run_single(prec: str, args, grid, case, opts, walls=None):
             "walls": walls or None}
 
 
+def wake_kick(grid, case, solver, amplitude: float = 0.1):
+    """Freestream plus a transverse-velocity blob in the near wake,
+    amplitude * U * exp(-((x - 1)^2 + y^2) / 0.25), which breaks the mirror
+    symmetry so that shedding starts within a few shedding periods."""
+    x = solver.geometry("cell_centerx")
+    y = solver.geometry("cell_centery")
+    U = float(np.hypot(case.u, case.v))
+    v = case.v + amplitude * U * np.exp(-((x - 1.0) ** 2 + y ** 2) / 0.25)
+    rho = np.full_like(x, case.rho)
+    E = case.P / (case.gamma - 1) + 0.5 * rho * (case.u ** 2 + v ** 2)
+    return np.stack([rho, rho * case.u, rho * v, E])
+
+
+def strouhal_run(prec: str, args, grid, case, walls):
+    """The cylinder with the uniform (time-accurate) step from a kicked
+    start: lift and drag sampled every args.strouhal_every steps, the
+    shedding frequency from the lift history (fv2d.analysis.strouhal)."""
+    from fv2d import analysis
+    c = inputs.Case(**{**case.__dict__, "dt_type": "UNIFORM"})
+    s = Solver(grid, c, precision=PRECISIONS[prec])
+    s.set_state(wake_kick(grid, c, s))
+    t, cl, cd = [], [], []
+    done, t0 = 0, time.perf_counter()
+    while done < args.strouhal_steps:
+        n = min(args.strouhal_every, args.strouhal_steps - done)
+        s.step(n)
+        done += n
+        q, g = s.state_and_grad()
+        _, _, l, d = analysis.wall_coefficients(grid, c, q, g, geo=walls)
+        t.append(s.time())
+        cl.append(l)
+        cd.append(d)
+        if len(t) % 50 == 0:
+            log(f"strouhal {prec}: {done} steps, t = {t[-1]:.4g} s, CL = {l:.4f}")
+    U = float(np.hypot(case.u, case.v))
+    st = analysis.strouhal(t, cl, case.length, U)
+    half = len(cd) // 2
+    s.close()
+    return {**st, "cd_mean": float(np.mean(cd[half:])), "steps": done,
+            "time_s": t[-1] if t else 0.0, "dt_s": t[-1] / done if done else None,
+            "flow_times": t[-1] * U / case.length if t else 0.0,
+            "wall_s": round(time.perf_counter() - t0, 1), "sample_every": args.strouhal_every}
+
+
+def max_aspect_ratio(grid, area):
+    """Largest ratio of a cell's longest to its shortest face."""
+    lo = np.full(grid.num_cells, np.inf)
+    hi = np.zeros(grid.num_cells)
+    for cc in (grid.face_c1, grid.face_c2):
+        m = cc >= 0
+        np.minimum.at(lo, cc[m], area[m])
+        np.maximum.at(hi, cc[m], area[m])
+    return float((hi / lo).max())
+
+
 def mean_coefficients(samples):
     """Mean lift and drag over the second half of the samples."""
     half = samples[len(samples) // 2:]
@@ -408,7 +463,17 @@ def bench_single(args):
         out["strouhal_why"] = ("not defined for this scheme: its GLOBAL time step is "
                                "2*CFL*V_i/max(sum), proportional to each cell's volume "
                                "(ssp_rk3.cpp:38-42), so cells advance through different "
-                               "physical times")
+                               "physical times; see strouhal_uniform_dt")
+        if args.strouhal_steps:
+            out["strouhal_uniform_dt"] = {
+                "how": ("time_step_type 3 (one dt = 2 CFL / max(Sigma/V), time-accurate), from "
+                        "freestream plus a transverse-velocity blob in the wake; St from "
+                        "upward zero crossings of CL over the last half, FFT peak beside it")}
+            for p in runs:
+                log(f"strouhal run, {p}")
+                out["strouhal_uniform_dt"][p] = strouhal_run(p, args, grid, case, walls)
+        if walls:
+            out["config"]["max_cell_aspect_ratio"] = max_aspect_ratio(grid, walls["face_area"])
     if "fp64" in runs and head == "fp32":
         r64 = runs["fp64"]
         out["fp64"] = {"value": r64["value"], "ms_per_step": r64["wall"] * 1e3 / r64["steps"],
@@ -627,6 +692,11 @@ def main():
                     help="with --no-autotune: 0 owner gather (bit-exact), 1 atomic scatter")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the launch-shape autotuner (use --block-size/--xcd-remap)")
+    ap.add_argument("--strouhal-steps", type=int, default=0,
+                    help="cylinder: also run this many steps with the uniform (time-accurate) "
+                         "time step and report the shedding frequency")
+    ap.add_argument("--strouhal-every", type=int, default=2000,
+                    help="cylinder: steps between lift samples of the Strouhal run")
     ap.add_argument("--grid-npz", default=None,
                     help="airfoil: load the generated mesh from this file (Grid.save_npz) "
                          "instead of generating it")

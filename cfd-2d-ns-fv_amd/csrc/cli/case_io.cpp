@@ -110,6 +110,7 @@ bool read_inputs(const std::string& path, Inputs* in, std::string* log, std::str
         if (key == "time_step_type") {
           if (val == "GLOBAL") in->dt_type = 1;
           if (val == "LOCAL") in->dt_type = 2;
+          if (val == "UNIFORM") in->dt_type = 3;   // extension: one dt for every cell
           o << "  time_step_type = " << val << " \n";
         }
         if (key == "time_scheme") {
@@ -493,7 +494,7 @@ void write_mesh_dumps(const Grid& g, const Geometry& geo) {
 }
 
 void monitor(int ndx, const Grid& g, const Geometry& geo, const Inputs& in, const double* q,
-             const double* grad, const double res[4]) {
+             const double* grad, const double res[4], double t) {
   const int32_t nc = g.nc;
   const double* p[4] = {q, q + nc, q + 2 * (size_t)nc, q + 3 * (size_t)nc};
   const double* qx[4] = {grad, grad + nc, grad + 2 * (size_t)nc, grad + 3 * (size_t)nc};
@@ -553,7 +554,9 @@ void monitor(int ndx, const Grid& g, const Geometry& geo, const Inputs& in, cons
   double cl = 0, cd = 0;
   wall_forces(g, geo, in, q, grad, &cl, &cd);
   std::ofstream fo("sol/forces.dat", std::ios_base::app | std::ios_base::out);
-  fo << std::setprecision(10) << ndx << " " << cl << " " << cd << std::endl;
+  fo << std::setprecision(10) << ndx << " " << cl << " " << cd;
+  if (t >= 0) fo << " " << t;
+  fo << std::endl;
 }
 
 void wall_forces(const Grid& g, const Geometry& geo, const Inputs& in, const double* q,

@@ -63,9 +63,10 @@ void write_mesh_dumps(const Grid& g, const Geometry& geo);
 // save_sol.cpp:16-103 monitor block: CL, CD and the residual norms, appended
 // to sol/output.dat and sol/res_history.dat and echoed to stdout; then this
 // build's own lift and drag coefficients (wall_forces), appended to
-// sol/forces.dat with 10 significant digits.
+// sol/forces.dat with 10 significant digits, followed there by the physical
+// time t when t >= 0 (uniform time step).
 void monitor(int ndx, const Grid& g, const Geometry& geo, const Inputs& in, const double* q,
-             const double* grad, const double res[4]);
+             const double* grad, const double res[4], double t = -1.0);
 // Lift and drag coefficients of the wall group (BC1 if it is a wall, else
 // BC2), per unit span referred to `length`.  Unlike save_sol.cpp:20-75,
 // which truncates centroids and face centres to int, the wall state is the

@@ -121,7 +121,9 @@ int main(int argc, char** argv) {
         for (size_t i = 0; i < nc; ++i) s += r[k * nc + i] * r[k * nc + i];
         res[k] = std::sqrt(s);
       }
-      fv2d::monitor(ndx, grid, geo, in, q.data(), grad.data(), res);
+      double t = -1.0;   // physical time: uniform time step only
+      if (in.dt_type == 3) fvk_get_time(ctx, &t);
+      fv2d::monitor(ndx, grid, geo, in, q.data(), grad.data(), res, t);
     }
     if (outp) fv2d::write_tecplot(ndx, grid, geo, in, q.data(), grad.data());
     ++ndx;

@@ -689,6 +689,29 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
 // Residual / volume, dt, SSP-RK3 stage combination and stores of one cell.
 // The combinations' coefficients sum to one, so they apply unchanged to a
 // perturbation state.
+// Uniform time step (time_step_type 3, an extension: one dt for every cell,
+// dt = 2*CFL / max_i(Sigma_i / V_i), so the run is time-accurate): the dt of
+// the step whose maximum has FP64 bits `bits`.
+template <typename T>
+__device__ __forceinline__ double uniform_dt(const FlowT<T>& fl, unsigned long long bits) {
+  return double(fl.cfl2) / __longlong_as_double((long long)bits);
+}
+
+// the value stage 3 maximises for the next step: Sigma_i (GLOBAL) or
+// Sigma_i / V_i (uniform)
+template <typename T>
+__device__ __forceinline__ double dt_key(const StageArgsT<T>& a, T sg, double V) {
+  return a.dt_uniform ? double(sg) / V : double(sg);
+}
+
+// stage 1, by the instance's first block: clear the maximum stage 3 will
+// build and, with the uniform dt, advance the instance's physical time
+template <typename T>
+__device__ __forceinline__ void step_start(const StageArgsT<T>& a, const FlowT<T>& fl, int inst) {
+  if (a.dt_uniform) a.time[inst] += uniform_dt(fl, a.smax_cur[inst]);
+  a.smax_next[inst] = 0ull;
+}
+
 template <typename T>
 __device__ __forceinline__ void cell_update(const StageArgsT<T>& a, const FlowT<T>& fl, int inst,
                                             int c, const T q[4], const T qn[4], T r[4], T sg,
@@ -696,8 +719,13 @@ __device__ __forceinline__ void cell_update(const StageArgsT<T>& a, const FlowT<
   const T V = T(V_d);
 #pragma unroll
   for (int k = 0; k < 4; ++k) r[k] = r[k] / V;
-  const T denom = a.dt_global ? T(__longlong_as_double((long long)a.smax_cur[inst])) : sig_old;
-  const T dt = (fl.cfl2 * V) / denom;
+  T dt;
+  if (a.dt_uniform) {
+    dt = T(uniform_dt(fl, a.smax_cur[inst]));
+  } else {
+    const T denom = a.dt_global ? T(__longlong_as_double((long long)a.smax_cur[inst])) : sig_old;
+    dt = (fl.cfl2 * V) / denom;
+  }
   T out[4];
   if (a.stage == 1) {
 #pragma unroll
@@ -869,10 +897,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_cell(StageArgsT<T> a) {
       sg += st;
     }
     cell_update_ld(a, fl, c, me.q, r, sg);
-    sig_new = double(sg);
+    sig_new = dt_key(a, sg, m.vol[c]);
   }
   if (a.stage == 3 && a.dt_global) block_max_to(sig_new, a.smax_next);
-  if (a.stage == 1 && blockIdx.x == 0 && threadIdx.x == 0) *a.smax_next = 0ull;
+  if (a.stage == 1 && blockIdx.x == 0 && threadIdx.x == 0) step_start(a, fl, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1044,11 +1072,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
     }
     if (row == 0 && active) {
       cell_update_ld(a, fl, c, me.q, r, sg);
-      sig_new = double(sg);
+      sig_new = dt_key(a, sg, m.vol[c]);
     }
   }
   if (a.stage == 3 && a.dt_global) block_max_to(sig_new, a.smax_next);
-  if (a.stage == 1 && blockIdx.x == 0 && threadIdx.x == 0) *a.smax_next = 0ull;
+  if (a.stage == 1 && blockIdx.x == 0 && threadIdx.x == 0) step_start(a, fl, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1393,12 +1421,11 @@ void k_pass_b_lds(StageArgsT<T> a) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) q[k] = L.q[k * L.stride + i];
       cell_update(a, fl, inst, c, q, qn, r, sg, V, sig_old);
-      sig_new = double(sg);
+      sig_new = dt_key(a, sg, V);
     }
   }
   if (live && a.stage == 3 && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
-  // the instance's first block clears the maximum stage 3 will build
-  if (live && a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) a.smax_next[inst] = 0ull;
+  if (live && a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) step_start(a, fl, inst);
 }
 
 // ---------------------------------------------------------------------------
@@ -1485,11 +1512,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_update_atomic(StageArgsT<T> a) {
     if (a.stage > 1) load_q(a.qn, c, qn);
     const T sig_old = a.dt_global ? T(1) : a.sigma[c];
     cell_update(a, fl, inst, c, q, qn, r, sg, m.vol[c], sig_old);
-    sig_new = double(sg);
+    sig_new = dt_key(a, sg, m.vol[c]);
   }
   if (a.stage == 3 && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
-  // the instance's first block clears the maximum stage 3 will build
-  if (a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) a.smax_next[inst] = 0ull;
+  if (a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) step_start(a, fl, inst);
 }
 
 // Spectral sums of the first step: every face carries |V_inf| and c_inf
@@ -1507,7 +1533,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_init_sigma(StageArgsT<T> a) {
   T sg = 0;
   for (int j = 0; j < deg; ++j) sg += (fabs(fl.vn0) + fl.c0) * T(m.s_area[base + j * 64]);
   a.sigma[c] = sg;
-  block_max_to(double(sg), a.smax_cur + inst);
+  block_max_to(dt_key(a, sg, m.vol[c]), a.smax_cur + inst);
 }
 
 // One thread block per mesh block (blockDim = blk, grid = ncp / blk, so every

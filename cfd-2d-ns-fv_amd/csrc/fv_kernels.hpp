@@ -94,7 +94,8 @@ struct StageArgsT {
   int32_t nactive;             // active blocks (batch only)
   int32_t accumulate;          // 1: pass B scatters with atomics, a cell pass updates
   int32_t stage;               // 1, 2, 3
-  int32_t dt_global;           // time_step_type GLOBAL
+  int32_t dt_global;           // one maximum per step: time_step_type GLOBAL or uniform
+  int32_t dt_uniform;          // time_step_type 3: one dt = 2 CFL / max(Sigma_i / V_i)
   int32_t sentinel;            // lagged face states are the -101 initial values
   int32_t fs_prims;            // stage input's primitives are the freestream ones
   int32_t store_res;           // write the residual (stage 3 of a reported step)
@@ -115,6 +116,7 @@ struct StageArgsT {
   T* acc;                      // accumulate = 1: face sums per cell (4 per cell), zero
                                // between stages
   T* sacc;                     // accumulate = 1: spectral sums per cell at stage 3
+  double* time;                // per instance: physical time of the uniform dt
 };
 
 // launch shapes: one thread per cell; one thread per (cell, face slot); or

@@ -67,6 +67,7 @@ struct Engine {
   void* acc = nullptr;                        // accumulate = 1: 4*ncp face sums (zero between stages)
   void* sacc = nullptr;                       // accumulate = 1: ncp spectral sums
   unsigned long long* smax = nullptr;         // [2][n], FP64 bits
+  double* time = nullptr;                     // [n] physical time of the uniform dt
   double* partial = nullptr;                  // [4][nblocks]
   int block = 256;
   int variant = fvk::kVariantLds;
@@ -199,7 +200,7 @@ int check_options(const fvk_options& o, std::string* msg) {
   if (o.flux_solver == 2) { *msg = "flux_solver ROE is not implemented by the reference solver library"; return FVK_ERR_UNSUPPORTED; }
   if (o.flux_solver != 1) { *msg = "flux_solver must be RUSANOV"; return FVK_ERR_OPTION; }
   if (o.time_scheme != 1 && o.time_scheme != 2) { *msg = "time_scheme must be SSP_RK2 or SSP_RK3"; return FVK_ERR_OPTION; }
-  if (o.time_step_type != 1 && o.time_step_type != 2) { *msg = "time_step_type must be GLOBAL or LOCAL"; return FVK_ERR_OPTION; }
+  if (o.time_step_type < 1 || o.time_step_type > 3) { *msg = "time_step_type must be GLOBAL, LOCAL or 3 (uniform)"; return FVK_ERR_OPTION; }
   if (o.bc1 < -4 || o.bc1 > -1 || o.bc2 < -4 || o.bc2 > -1) { *msg = "BC1/BC2 must be FREESTREAM, SLIP_WALL, EXTRAPOLATION or NO_SLIP_WALL"; return FVK_ERR_OPTION; }
   if (o.precision != FVK_FP64 && o.precision != FVK_FP32_MIXED) { *msg = "precision must be FVK_FP64 or FVK_FP32_MIXED"; return FVK_ERR_OPTION; }
   if (o.kernel_variant < 0 || o.kernel_variant > 3) { *msg = "kernel_variant must be 0, 1, 2 or 3"; return FVK_ERR_OPTION; }
@@ -292,7 +293,9 @@ fvk::StageArgsT<T> base_args(Engine* c) {
   a.blk_inst = c->blk_inst;
   a.active = c->active;
   a.nactive = c->nactive;
-  a.dt_global = c->opt.time_step_type == 1;
+  a.dt_global = c->opt.time_step_type == 1 || c->opt.time_step_type == 3;
+  a.dt_uniform = c->opt.time_step_type == 3;
+  a.time = c->time;
   a.xcd_remap = c->opt.xcd_remap;
   a.variant = c->variant;
   a.sigma = static_cast<T*>(c->sigma);
@@ -345,6 +348,7 @@ int arm_start(Engine* c) {
   c->smax_cur = 0;
   FVK_HIP(c, fvk::launch_fill(reinterpret_cast<double*>(c->smax), 2 * (int64_t)c->in.size(), 0.0,
                               c->stream));
+  FVK_HIP(c, fvk::launch_fill(c->time, (int64_t)c->in.size(), 0.0, c->stream));
   if (c->fp32) FVK_HIP(c, fvk::launch_init_sigma(base_args<float>(c), c->stream));
   else FVK_HIP(c, fvk::launch_init_sigma(base_args<double>(c), c->stream));
   c->sentinel = true;
@@ -815,6 +819,8 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   if ((rc = alloc(&c->sigma, ncp))) return rc;
   if ((rc = alloc(&c->res, 4 * ncp))) return rc;
   if ((rc = dev_alloc(c, &c->smax, 2 * c->in.size()))) return rc;
+  if ((rc = dev_alloc(c, &c->time, c->in.size()))) return rc;
+  if ((rc = fill(c, c->time, (int64_t)c->in.size(), 0.0))) return rc;
   if ((rc = dev_alloc(c, &c->partial, 4 * (size_t)d.nblocks))) return rc;
   for (int i = 0; i < 3; ++i) {
     if ((rc = fill(c, c->q[i], 4 * (int64_t)ncp, 0.0))) return rc;
@@ -852,6 +858,7 @@ std::vector<std::pair<void*, size_t>> checkpoint_arrays(Engine* c) {
   v.push_back({c->sigma, ncp * t});
   v.push_back({c->res, 4 * ncp * t});
   v.push_back({c->smax, 2 * c->in.size() * sizeof(unsigned long long)});
+  v.push_back({c->time, c->in.size() * sizeof(double)});
   return v;
 }
 
@@ -1065,6 +1072,20 @@ int fvk_get_info(fvk_ctx* c, int64_t info[6]) {
   return FVK_OK;
 }
 
+// physical time of every instance (uniform dt only)
+static int get_time(Engine* c, double* out) {
+  if (c->opt.time_step_type != 3)
+    return set_err(c, FVK_ERR_OPTION, "physical time is defined only for the uniform time step "
+                                      "(time_step_type 3): GLOBAL and LOCAL steps differ per cell");
+  FVK_HIP(c, hipMemcpy(out, c->time, c->in.size() * sizeof(double), hipMemcpyDeviceToHost));
+  return FVK_OK;
+}
+
+int fvk_get_time(fvk_ctx* c, double* t) {
+  if (!c || !t) return FVK_ERR_ARG;
+  return get_time(&c->e, t);
+}
+
 int fvk_get_options(fvk_ctx* c, fvk_options* out) {
   if (!c || !out) return FVK_ERR_ARG;
   *out = resolved_options(c->e, 0);
@@ -1263,6 +1284,11 @@ int fvk_batch_get_info(fvk_batch* b, int64_t info[6]) {
   info[4] = c.nactive;
   info[5] = active;
   return FVK_OK;
+}
+
+int fvk_batch_get_time(fvk_batch* b, double* t) {
+  if (!b || !t) return FVK_ERR_ARG;
+  return get_time(&b->e, t);
 }
 
 int fvk_batch_get_options(fvk_batch* b, int32_t i, fvk_options* out) {

@@ -83,3 +83,37 @@ def compare_walls(grid, case, q32, g32, q64, g64, geo=None):
             "cp_l2_err": float(np.linalg.norm(cp32 - cp64) / np.sqrt(cp64.size)),
             "cp_linf_err": float(np.max(np.abs(cp32 - cp64))),
             "wall_faces": int(cp64.size)}
+
+
+def strouhal(t, cl, length: float, speed: float, tail: float = 0.5):
+    """Shedding frequency of a lift history: St = f * length / speed.
+
+    Over the last `tail` fraction of the record (the periodic part), f is
+    the number of whole periods between the first and last upward zero
+    crossings of cl - mean(cl) (crossing times interpolated linearly) over
+    the time between them; the peak of the record's spectrum (parabolic
+    interpolation between bins) is given beside it as a cross-check.
+    Returns {"st", "st_fft", "periods", "cl_amplitude", "cl_mean"}; st is None
+    when fewer than two crossings are found."""
+    t = np.asarray(t, dtype=np.float64)
+    cl = np.asarray(cl, dtype=np.float64)
+    k0 = int(len(t) * (1.0 - tail))
+    t, cl = t[k0:], cl[k0:]
+    x = cl - cl.mean()
+    up = np.flatnonzero((x[:-1] < 0) & (x[1:] >= 0))
+    tc = t[up] - x[up] * (t[up + 1] - t[up]) / (x[up + 1] - x[up])
+    out = {"st": None, "st_fft": None, "periods": max(0, len(tc) - 1),
+           "cl_amplitude": float(0.5 * (cl.max() - cl.min())) if len(cl) else None,
+           "cl_mean": float(cl.mean()) if len(cl) else None}
+    if len(tc) >= 2:
+        f = (len(tc) - 1) / (tc[-1] - tc[0])
+        out["st"] = float(f * length / speed)
+    if len(t) >= 8:
+        dt = (t[-1] - t[0]) / (len(t) - 1)
+        spec = np.abs(np.fft.rfft(x * np.hanning(len(x))))
+        j = int(np.argmax(spec[1:])) + 1
+        if 1 <= j < len(spec) - 1:
+            a, b, c = spec[j - 1], spec[j], spec[j + 1]
+            j = j + 0.5 * (a - c) / (a - 2 * b + c)
+        out["st_fft"] = float(j / (len(x) * dt) * length / speed)
+    return out

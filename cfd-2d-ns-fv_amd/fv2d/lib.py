@@ -27,11 +27,12 @@ FP32_MIXED = 1
 EXPORTS = ["fvk_default_options", "fvk_create", "fvk_destroy", "fvk_last_error",
            "fvk_init_freestream", "fvk_set_state", "fvk_step", "fvk_get_state",
            "fvk_get_residual", "fvk_residual_l2", "fvk_get_geometry", "fvk_geometry",
-           "fvk_get_info", "fvk_get_options", "fvk_time_steps", "fvk_time_kernels",
+           "fvk_get_info", "fvk_get_time", "fvk_get_options", "fvk_time_steps", "fvk_time_kernels",
            "fvk_batch_create", "fvk_batch_destroy", "fvk_batch_last_error", "fvk_batch_init",
            "fvk_batch_step", "fvk_batch_set_active", "fvk_batch_residual_l2",
            "fvk_batch_run_to_drop", "fvk_batch_get_state", "fvk_batch_get_info",
-           "fvk_batch_get_options", "fvk_batch_use_graph", "fvk_batch_time_kernels",
+           "fvk_batch_get_time", "fvk_batch_get_options", "fvk_batch_use_graph",
+           "fvk_batch_time_kernels",
            "fvk_checkpoint_size", "fvk_checkpoint_save", "fvk_checkpoint_load",
            "fvk_batch_checkpoint_size", "fvk_batch_checkpoint_save", "fvk_batch_checkpoint_load"]
 
@@ -93,6 +94,7 @@ def lib():
         L.fvk_geometry.argtypes = [P(FvkMesh), P(FvkOptions), ctypes.c_char_p, dp]
         L.fvk_get_info.argtypes = [vp, P(ctypes.c_int64)]
         L.fvk_get_options.argtypes = [vp, P(FvkOptions)]
+        L.fvk_get_time.argtypes = [vp, dp]
         L.fvk_time_steps.argtypes = [vp, ctypes.c_int32, dp, P(ctypes.c_int64)]
         L.fvk_time_kernels.argtypes = [vp, ctypes.c_int32, dp]
         ip = P(ctypes.c_int32)
@@ -109,6 +111,7 @@ def lib():
         L.fvk_batch_get_state.argtypes = [vp, ctypes.c_int32, dp, dp]
         L.fvk_batch_get_info.argtypes = [vp, P(ctypes.c_int64)]
         L.fvk_batch_get_options.argtypes = [vp, ctypes.c_int32, P(FvkOptions)]
+        L.fvk_batch_get_time.argtypes = [vp, dp]
         L.fvk_batch_use_graph.argtypes = [vp, ctypes.c_int32]
         L.fvk_batch_time_kernels.argtypes = [vp, ctypes.c_int32, dp]
         for pre in ("fvk_", "fvk_batch_"):
@@ -136,7 +139,7 @@ def make_options(case, **overrides) -> FvkOptions:
     o.eqn = {"EULER": 1, "NS": 2}[case.eqn]
     o.flux_solver = {"RUSANOV": 1, "ROE": 2}[case.flux]
     o.time_scheme = {"SSP_RK2": 1, "SSP_RK3": 2}[case.scheme]
-    o.time_step_type = {"GLOBAL": 1, "LOCAL": 2}[case.dt_type]
+    o.time_step_type = {"GLOBAL": 1, "LOCAL": 2, "UNIFORM": 3}[case.dt_type]
     o.bc1 = BC_CODES[case.BC1]
     o.bc2 = BC_CODES[case.BC2]
     o.cfl = case.CFL
@@ -246,6 +249,12 @@ class Solver:
         keys = ["num_cells", "num_faces", "num_boundary_faces", "num_slots", "max_faces",
                 "num_wall"]
         return dict(zip(keys, list(a)))
+
+    def time(self) -> float:
+        """Physical time reached (uniform time step only)."""
+        t = np.zeros(1)
+        self._check(lib().fvk_get_time(self.h, _dp(t)))
+        return float(t[0])
 
     def launch_options(self) -> dict:
         """Cells per block, XCD order and kernel shape in force."""
@@ -361,6 +370,12 @@ class Batch:
         self._check(lib().fvk_batch_get_info(self.h, a))
         keys = ["instances", "cells", "faces", "blocks", "active_blocks", "active_instances"]
         return dict(zip(keys, list(a)))
+
+    def time(self) -> np.ndarray:
+        """Physical time reached by each instance (uniform time step only)."""
+        t = np.zeros(self.n)
+        self._check(lib().fvk_batch_get_time(self.h, _dp(t)))
+        return t
 
     def launch_options(self) -> dict:
         """Cells per block, XCD order and kernel shape in force (common to all

@@ -70,7 +70,10 @@ typedef struct fvk_options {
   int32_t eqn;             /* 1 EULER, 2 NS */
   int32_t flux_solver;     /* 1 RUSANOV (2 ROE -> FVK_ERR_UNSUPPORTED) */
   int32_t time_scheme;     /* 1 SSP_RK2 (runs SSP-RK3, main.cpp:123-126), 2 SSP_RK3 */
-  int32_t time_step_type;  /* 1 GLOBAL, 2 LOCAL */
+  int32_t time_step_type;  /* 1 GLOBAL, 2 LOCAL; 3 UNIFORM, an extension: one dt for
+                              every cell, 2*CFL / max_i(Sigma_i / V_i), so that a run
+                              is time-accurate (GLOBAL's dt_i = 2*CFL*V_i / max Sigma,
+                              ssp_rk3.cpp:38-42, differs per cell) */
   int32_t bc1;             /* kind of the BC1 group (FVK_BC_*) */
   int32_t bc2;             /* kind of the BC2 group (FVK_BC_*) */
   double cfl;
@@ -141,6 +144,11 @@ int fvk_geometry(const fvk_mesh* mesh, const fvk_options* opt, const char* name,
 /* Sizes: [num_cells, num_faces, num_boundary_faces, num_slots, max_faces_per_cell,
  * num_wall_faces (readmesh.cpp:356-360)]. */
 int fvk_get_info(fvk_ctx* ctx, int64_t info[6]);
+
+/* Physical time reached (sum of the steps' dt) with time_step_type 3, reset by
+ * fvk_init_freestream and fvk_set_state; FVK_ERR_OPTION for GLOBAL or LOCAL,
+ * whose steps differ per cell.  No counterpart in the reference. */
+int fvk_get_time(fvk_ctx* ctx, double* t);
 
 /* The options in force, with the automatic choices (block_size 0,
  * xcd_remap -1) resolved.  No counterpart in the reference, whose options
@@ -214,6 +222,8 @@ int fvk_batch_run_to_drop(fvk_batch* b, double drop, int32_t max_steps, int32_t 
 int fvk_batch_get_state(fvk_batch* b, int32_t i, double* q, double* grad);
 /* [instances, cells, faces, mesh blocks, active blocks, active instances] */
 int fvk_batch_get_info(fvk_batch* b, int64_t info[6]);
+/* t[i] = instance i's physical time, as fvk_get_time */
+int fvk_batch_get_time(fvk_batch* b, double* t);
 /* instance i's options in force, as fvk_get_options */
 int fvk_batch_get_options(fvk_batch* b, int32_t i, fvk_options* out);
 /* 0: launch every step's kernels one by one; 1 (default): replay the graph */

@@ -46,6 +46,7 @@ struct or_case {
   double *res[4];
   /* scratch */
   double *dt;
+  double time;                     /* physical time of the uniform dt (extension) */
 };
 
 static char g_err[512];
@@ -602,6 +603,18 @@ static void ssp_rk3(or_case* C) {
     for (int c = 0; c < nc; ++c) dt[c] = (2 * C->o.cfl * C->vol[c]) / m;
   } else if (C->o.dt_type == 2) {  /* LOCAL  :43-47 */
     for (int c = 0; c < nc; ++c) dt[c] = (2 * C->o.cfl * C->vol[c]) / dt[c];
+  } else if (C->o.dt_type == 3) {
+    /* UNIFORM: an extension with no reference counterpart.  One dt for every
+     * cell, 2*CFL / max_c(Sigma_c / V_c), the smallest LOCAL dt, so that the
+     * run is time-accurate. */
+    double m = dt[0] / C->vol[0];
+    for (int c = 1; c < nc; ++c) {
+      const double r = dt[c] / C->vol[c];
+      if (m < r) m = r;
+    }
+    const double d = (2 * C->o.cfl) / m;
+    for (int c = 0; c < nc; ++c) dt[c] = d;
+    C->time += d;
   }
   qset* Q = &C->Qbar;
   res(C, Q);                       /* :50-57 */
@@ -745,6 +758,7 @@ void or_destroy(or_case* C) {
 int or_set_state(or_case* C, const double* p1, const double* p2, const double* p3,
                  const double* p4) {
   /* main.cpp:94-106 then the face re-initialisation of main.cpp:114-118 */
+  C->time = 0.0;
   qset_init_plain(&C->Qbar, C->o.gamma, C->o.R);
   const double* src[4] = {p1, p2, p3, p4};
   for (int k = 0; k < 4; ++k) memcpy(C->Qbar.p[k], src[k], 8 * (size_t)C->nc);
@@ -832,7 +846,7 @@ const void* or_array(const or_case* C, const char* name, int64_t* n, char* kind)
     {"fR1", C->Qc2.p[0], nf}, {"fR2", C->Qc2.p[1], nf}, {"fR3", C->Qc2.p[2], nf}, {"fR4", C->Qc2.p[3], nf},
     {"tau_xx", C->Qface.txx, nf}, {"tau_xy", C->Qface.txy, nf}, {"tau_yy", C->Qface.tyy, nf},
     {"dTdx", C->Qface.dtdx, nf}, {"dTdy", C->Qface.dtdy, nf},
-    {"dt", C->dt, nc},
+    {"dt", C->dt, nc}, {"time", &C->time, 1},
     {"geom/face_area", C->area, nf}, {"geom/face_nx", C->nx, nf}, {"geom/face_ny", C->ny, nf},
     {"geom/face_centerx", C->fcx, nf}, {"geom/face_centery", C->fcy, nf},
     {"geom/cell_vol", C->vol, nc}, {"geom/cell_centerx", C->ccx, nc}, {"geom/cell_centery", C->ccy, nc},

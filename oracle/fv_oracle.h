@@ -40,7 +40,7 @@ typedef struct {
   int32_t limiter;        /* 0 NONE, 1 MINMOD (no-op), 2 SQUEEZE */
   int32_t eqn;            /* 1 EULER, 2 NS */
   int32_t flux;           /* 1 RUSANOV, 2 ROE (flux left at -101, as res.cpp:66-68) */
-  int32_t dt_type;        /* 1 GLOBAL, 2 LOCAL */
+  int32_t dt_type;        /* 1 GLOBAL, 2 LOCAL, 3 UNIFORM (extension) */
   int32_t bc1, bc2;       /* BC kinds of file markers 0 and -1: -1..-4 */
   int32_t pad_;
   double cfl;

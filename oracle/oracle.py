@@ -76,7 +76,7 @@ def options_from_case(case) -> OrOptions:
     lim = {"NONE": 0, "MINMOD": 1, "SQUEEZE": 2}[case.limiter]
     return OrOptions(order=case.poly + 1, limiter=lim, eqn={"EULER": 1, "NS": 2}[case.eqn],
                      flux={"RUSANOV": 1, "ROE": 2}[case.flux],
-                     dt_type={"GLOBAL": 1, "LOCAL": 2}[case.dt_type],
+                     dt_type={"GLOBAL": 1, "LOCAL": 2, "UNIFORM": 3}[case.dt_type],
                      bc1=BC_CODES[case.BC1], bc2=BC_CODES[case.BC2], pad_=0, cfl=case.CFL,
                      gamma=case.gamma, R=case.R, mu=case.mu, Pr=case.Pr, P=case.P, rho=case.rho,
                      u=case.u, v=case.v)

@@ -268,3 +268,19 @@ def test_wall_coefficients_of_a_uniform_flow():
     # with viscosity the no-slip shear of a uniform stream is pure drag, along the flow
     cp, xw, cl, cd = analysis.wall_coefficients(g, case, q)
     assert cd > 0 and abs(cl) < 1e-9 * cd + 1e-12
+
+
+def test_strouhal_estimator_on_a_known_signal():
+    """An evenly sampled lift history with a known frequency (a start-up
+    ramp, then a sinusoid with a harmonic): the estimator returns
+    St = f D / U."""
+    from fv2d import analysis
+    f, D, U = 11.3, 1.0, 67.6
+    t = np.arange(4000) * 1.7e-3
+    cl = 0.3 * np.sin(2 * np.pi * f * t + 0.4) + 0.05 * np.sin(4 * np.pi * f * t) + 0.02
+    cl[:1000] *= np.linspace(0, 1, 1000)
+    r = analysis.strouhal(t, cl, D, U)
+    assert abs(r["st"] - f * D / U) < 2e-3 * f * D / U
+    assert abs(r["st_fft"] - f * D / U) < 1e-2 * f * D / U
+    assert r["periods"] >= 30 and abs(r["cl_amplitude"] - 0.3) < 0.06
+    assert analysis.strouhal(t[:10], np.ones(10), D, U)["st"] is None
