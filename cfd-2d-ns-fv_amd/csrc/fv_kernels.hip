@@ -1090,12 +1090,19 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
 // instead of one per face slot.
 // ---------------------------------------------------------------------------
 // Tuning constants of the LDS-staged kernels (measured on MI355X):
-//   kPF / kFPF  face slots per cell / block faces per thread prefetched
-//               before the staging barrier;
+//   PF / kFPF   face slots per cell / block faces per thread prefetched
+//               before the staging barrier.  PF is a template parameter:
+//               kPF for meshes whose cells have at most kPF faces, kPFDeep
+//               for the rest (polygonal meshes: 12% less time per launch
+//               there, while the deep form costs FP64 pass B 8% on
+//               triangle/quad meshes);
 //   kMinBlocksA / kMinBlocksB  minimum resident blocks per CU the register
 //               allocation must allow, per state type (FP64, FP32).
 #ifndef FVK_PF
 #define FVK_PF 4
+#endif
+#ifndef FVK_PF_DEEP
+#define FVK_PF_DEEP 8
 #endif
 #ifndef FVK_FPF
 #define FVK_FPF 2
@@ -1113,6 +1120,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
 #define FVK_MINBLK_B32 1
 #endif
 constexpr int kPF = FVK_PF;
+constexpr int kPFDeep = FVK_PF_DEEP;
 constexpr int kFPF = FVK_FPF;
 template <typename T>
 constexpr int kMinBlocksA = kDelta<T> ? FVK_MINBLK_A32 : FVK_MINBLK_A64;
@@ -1178,7 +1186,7 @@ __device__ __forceinline__ void slot_pos(const DevMesh& m, int s, SlotPos<T>& px
   }
 }
 
-template <typename T, bool SQUEEZE>
+template <typename T, bool SQUEEZE, int PF>
 __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageArgsT<T> a) {
   extern __shared__ unsigned char lds_raw[];
   const DevMesh& m = a.m;
@@ -1193,10 +1201,10 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
   const int deg = m.deg[c];
   const int base = m.chunk_off[c >> 6] + (c & 63);
   // prefetch: slot codes and face positions, lagged state, LSQ inverse
-  int lc[kPF];
-  SlotPos<T> pfx[kPF], pfy[kPF];
+  int lc[PF];
+  SlotPos<T> pfx[PF], pfy[PF];
 #pragma unroll
-  for (int j = 0; j < kPF; ++j) {
+  for (int j = 0; j < PF; ++j) {
     if (j < deg) {
       const int s = base + j * 64;
       lc[j] = m.lcode[s];
@@ -1234,9 +1242,9 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
     accumulate_lsq(sx, sy, qmin, qmax, px, py, tmin, tmax);
   };
 #pragma unroll
-  for (int j = 0; j < kPF; ++j)
+  for (int j = 0; j < PF; ++j)
     if (j < deg) slot(j, lc[j]);
-  for (int j = kPF; j < deg; ++j) slot(j, m.lcode[base + j * 64]);
+  for (int j = PF; j < deg; ++j) slot(j, m.lcode[base + j * 64]);
   T gx[4], gy[4];
   lsq_solve(ixx, iyy, ixy, sx, sy, gx, gy);
   if (squeeze) {
@@ -1249,9 +1257,9 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
       for (int k = 0; k < 4; ++k) phi[k] = min2(ph[k], phi[k]);
     };
 #pragma unroll
-    for (int j = 0; j < kPF; ++j)
+    for (int j = 0; j < PF; ++j)
       if (j < deg) lim(pfx[j], pfy[j]);
-    for (int j = kPF; j < deg; ++j) {
+    for (int j = PF; j < deg; ++j) {
       SlotPos<T> px, py;
       slot_pos<T>(m, base + j * 64, px, py);
       lim(px, py);
@@ -1322,7 +1330,7 @@ __device__ __forceinline__ void lds_cell(const LdsCells<T, NQ, XY>& L, int l, Ce
   d.y = XY ? L.xy[st + l] : 0.0;
 }
 
-template <typename T, int ORDER, bool NS>
+template <typename T, int ORDER, bool NS, int PF>
 __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
 void k_pass_b_lds(StageArgsT<T> a) {
   // cell rows, then [4 or 5][max_bfaces] face terms (the spectral-sum term
@@ -1354,9 +1362,9 @@ void k_pass_b_lds(StageArgsT<T> a) {
       const int e = i + t * blockDim.x;
       if (e < nfb) load_face(m, fb + e, fr[t]);
     }
-    int sf[kPF];
+    int sf[PF];
 #pragma unroll
-    for (int j = 0; j < kPF; ++j)
+    for (int j = 0; j < PF; ++j)
       if (j < deg) sf[j] = m.sface[base + j * 64];
     T qn[4] = {0, 0, 0, 0}, sig_old = T(1);
     double V = 1.0;
@@ -1414,9 +1422,9 @@ void k_pass_b_lds(StageArgsT<T> a) {
         if (a.stage == 3) sg += lf[4 * fstride + e];
       };
 #pragma unroll
-      for (int j = 0; j < kPF; ++j)
+      for (int j = 0; j < PF; ++j)
         if (j < deg) add(sf[j]);
-      for (int j = kPF; j < deg; ++j) add(m.sface[base + j * 64]);
+      for (int j = PF; j < deg; ++j) add(m.sface[base + j * 64]);
       T q[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) q[k] = L.q[k * L.stride + i];
@@ -1613,8 +1621,14 @@ hipError_t launch_pass_a(const StageArgsT<T>& a0, int squeeze, int block, hipStr
     shape(a, a.active ? a.nactive : a.m.nblocks, &grid, &a.xcd_blocks);
     if (grid == 0) return hipSuccess;
     const size_t lds = lds_bytes_pass_a(a.m, sizeof(T));
-    if (squeeze) hipLaunchKernelGGL((k_pass_a_lds<T, true>), dim3(grid), dim3(a.m.blk), lds, s, a);
-    else hipLaunchKernelGGL((k_pass_a_lds<T, false>), dim3(grid), dim3(a.m.blk), lds, s, a);
+    const dim3 g(grid), blk(a.m.blk);
+    if (a.m.max_deg > kPF) {
+      if (squeeze) hipLaunchKernelGGL((k_pass_a_lds<T, true, kPFDeep>), g, blk, lds, s, a);
+      else hipLaunchKernelGGL((k_pass_a_lds<T, false, kPFDeep>), g, blk, lds, s, a);
+    } else {
+      if (squeeze) hipLaunchKernelGGL((k_pass_a_lds<T, true, kPF>), g, blk, lds, s, a);
+      else hipLaunchKernelGGL((k_pass_a_lds<T, false, kPF>), g, blk, lds, s, a);
+    }
   } else if (a.variant == kVariantSlot) {
     shape(a, a.m.ncp / 64, &grid, &a.xcd_blocks);
     if (squeeze) hipLaunchKernelGGL((k_pass_a_slot<T, true>), dim3(grid), dim3(64 * kRows), 0, s, a);
@@ -1646,9 +1660,15 @@ hipError_t launch_pass_b(const StageArgsT<T>& a0, int order, int ns, int block, 
       return hipGetLastError();
     }
     const size_t lds = lds_bytes_pass_b(a.m, order, a.stage, sizeof(T));
-    if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false>), dim3(grid), blk, lds, s, a);
-    else if (ns) hipLaunchKernelGGL((k_pass_b_lds<T, 2, true>), dim3(grid), blk, lds, s, a);
-    else hipLaunchKernelGGL((k_pass_b_lds<T, 2, false>), dim3(grid), blk, lds, s, a);
+    if (a.m.max_deg > kPF) {
+      if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false, kPFDeep>), dim3(grid), blk, lds, s, a);
+      else if (ns) hipLaunchKernelGGL((k_pass_b_lds<T, 2, true, kPFDeep>), dim3(grid), blk, lds, s, a);
+      else hipLaunchKernelGGL((k_pass_b_lds<T, 2, false, kPFDeep>), dim3(grid), blk, lds, s, a);
+    } else {
+      if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false, kPF>), dim3(grid), blk, lds, s, a);
+      else if (ns) hipLaunchKernelGGL((k_pass_b_lds<T, 2, true, kPF>), dim3(grid), blk, lds, s, a);
+      else hipLaunchKernelGGL((k_pass_b_lds<T, 2, false, kPF>), dim3(grid), blk, lds, s, a);
+    }
   } else if (a.variant == kVariantSlot) {
     shape(a, a.m.ncp / 64, &grid, &a.xcd_blocks);
     const dim3 blk(64 * kRows);

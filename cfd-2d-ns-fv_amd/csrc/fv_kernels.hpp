@@ -22,6 +22,7 @@ struct DevMesh {
   int32_t nblocks;
   int32_t max_halo;            // LDS row length is blk + max_halo
   int32_t max_bfaces;          // LDS face-result row length
+  int32_t max_deg;             // most faces of any cell (the prefetch depth's choice)
   const int32_t* chunk_off;    // slot offset of each 64-cell chunk
   const uint8_t* deg;          // faces per cell (0 = padding)
   const int32_t* code;         // per slot: global neighbour/owner or boundary code

@@ -717,11 +717,13 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   d.nblocks = 0;
   d.max_halo = 0;
   d.max_bfaces = 0;
+  d.max_deg = 0;
   for (const Instance& in : c->in) {
     d.nc += in.hm.nc;
     d.nblocks += in.hm.nblocks;
     d.max_halo = std::max(d.max_halo, in.hm.max_halo);
     d.max_bfaces = std::max(d.max_bfaces, in.hm.max_bfaces);
+    d.max_deg = std::max(d.max_deg, in.hm.max_deg);
   }
   // XCD order: contiguous ranges while a launch is a few blocks per CU (every
   // block is resident at once), runs of 8 blocks dealt in turn beyond that

@@ -29,10 +29,13 @@ def _cases():
     yield "closed box o2 SQUEEZE", gg.box(24, 18, jitter=0.25, split_fraction=0.5,
                                           markers=(0, 0, 0, 0)), inputs.Case(
         u=u, v=v, poly=1, limiter="SQUEEZE", dt_type="UNIFORM", CFL=0.5), 30
+    # the uniform step takes the convective spectral sums only, as the
+    # reference's steps do: at Re 100 with 1e-3 wall spacing the viscous
+    # limit needs CFL 0.2 here (0.3 diverges)
     u, v = inputs.mach_velocity(0.2)
     yield "ogrid NS o2 SQUEEZE", gg.ogrid(128, 64), inputs.Case(
         u=u, v=v, poly=1, limiter="SQUEEZE", eqn="NS", mu=1.225 * u / 100.0,
-        BC1="NO_SLIP_WALL", dt_type="UNIFORM", CFL=0.5), 10
+        BC1="NO_SLIP_WALL", dt_type="UNIFORM", CFL=0.2), 10
 
 
 CASES = list(_cases())
