@@ -347,7 +347,7 @@ def strouhal_run(prec: str, args, grid, case, walls):
     start: lift and drag sampled every args.strouhal_every steps, the
     shedding frequency from the lift history (fv2d.analysis.strouhal)."""
     from fv2d import analysis
-    c = inputs.Case(**{**case.__dict__, "dt_type": "UNIFORM"})
+    c = inputs.Case(**{**case.__dict__, "dt_type": "UNIFORM", "CFL": args.strouhal_cfl})
     s = Solver(grid, c, precision=PRECISIONS[prec])
     s.set_state(wake_kick(grid, c, s))
     t, cl, cd = [], [], []
@@ -370,7 +370,8 @@ def strouhal_run(prec: str, args, grid, case, walls):
     return {**st, "cd_mean": float(np.mean(cd[half:])), "steps": done,
             "time_s": t[-1] if t else 0.0, "dt_s": t[-1] / done if done else None,
             "flow_times": t[-1] * U / case.length if t else 0.0,
-            "wall_s": round(time.perf_counter() - t0, 1), "sample_every": args.strouhal_every}
+            "wall_s": round(time.perf_counter() - t0, 1), "sample_every": args.strouhal_every,
+            "cfl": args.strouhal_cfl}
 
 
 def max_aspect_ratio(grid, area):
@@ -466,9 +467,10 @@ def bench_single(args):
                                "physical times; see strouhal_uniform_dt")
         if args.strouhal_steps:
             out["strouhal_uniform_dt"] = {
-                "how": ("time_step_type 3 (one dt = 2 CFL / max(Sigma/V), time-accurate), from "
-                        "freestream plus a transverse-velocity blob in the wake; St from "
-                        "upward zero crossings of CL over the last half, FFT peak beside it")}
+                "how": ("time_step_type 3 (one dt = 2 CFL / max(Sigma/V), time-accurate) at "
+                        f"CFL {args.strouhal_cfl}, from freestream plus a transverse-velocity "
+                        "blob in the wake; St from upward zero crossings of CL over the last "
+                        "half, FFT peak beside it; mean CD over the last half")}
             for p in runs:
                 log(f"strouhal run, {p}")
                 out["strouhal_uniform_dt"][p] = strouhal_run(p, args, grid, case, walls)
@@ -554,6 +556,15 @@ def sequential_fp64(cases, steps, reps: int = 20):
 def bench_batch(args):
     name, plan = W.batch(args.instances)
     cases = [(i.grid, i.case) for i in plan]
+    if args.order or args.limiter:
+        # a variant of the batch (the configured workload is o2 SQUEEZE)
+        over = {}
+        if args.order:
+            over["poly"] = args.order - 1
+        if args.limiter:
+            over["limiter"] = args.limiter
+        cases = [(g, inputs.Case(**{**c.__dict__, **over})) for g, c in cases]
+        name += " -- variant: " + ", ".join(f"{k}={v}" for k, v in over.items())
     nc = W.cells_of(plan)
     nf = np.array([i.grid.num_faces for i in plan], dtype=np.int64)
     log(f"{name}: {len(cases)} instances, {int(nc.sum())} cells")
@@ -583,7 +594,8 @@ def bench_batch(args):
              "--warmup", "0", "--kernel-reps", "3", "--config", "batch",
              "--instances", str(args.instances), "--block-size", str(chosen["block_size"]),
              "--xcd-remap", str(chosen["xcd_remap"]), "--accumulate", str(chosen["accumulate"]),
-             "--precision", head]
+             "--precision", head, "--order", str(args.order)] + \
+        (["--limiter", args.limiter] if args.limiter else [])
     roof = roofline(r["per_launch"], algo, head, args, child)
     steps = r["steps"]
     cfg = {"workload": name, "instances": len(cases), "cells": int(nc.sum()),
@@ -669,6 +681,10 @@ def main():
                          "is the oracle of the stated error")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--drop", type=float, default=1e-6)
+    ap.add_argument("--order", type=int, default=0, choices=[0, 1, 2],
+                    help="batch: a variant at this order (0: the configured order 2)")
+    ap.add_argument("--limiter", default="", choices=["", "NONE", "SQUEEZE"],
+                    help="batch: a variant with this limiter (default: the configured SQUEEZE)")
     ap.add_argument("--check-every", type=int, default=100)
     ap.add_argument("--chunk", type=int, default=None,
                     help="single mesh: steps between residual checks and force samples")
@@ -695,6 +711,10 @@ def main():
     ap.add_argument("--strouhal-steps", type=int, default=0,
                     help="cylinder: also run this many steps with the uniform (time-accurate) "
                          "time step and report the shedding frequency")
+    ap.add_argument("--strouhal-cfl", type=float, default=0.1,
+                    help="cylinder: CFL of the Strouhal run (the uniform step takes the "
+                         "convective spectral sums only; 0.1 keeps the wall cells' viscous "
+                         "terms stable, 0.25 diverges)")
     ap.add_argument("--strouhal-every", type=int, default=2000,
                     help="cylinder: steps between lift samples of the Strouhal run")
     ap.add_argument("--grid-npz", default=None,
