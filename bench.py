@@ -625,7 +625,9 @@ def bench_batch(args):
                 out["speedup_vs_fp64_sequential"] = round(r["value"] / seq, 4)
         if head == "fp32":
             out["speedup_vs_fp64_batched"] = round(r["value"] / r64["value"], 4)
-            errs, bad32, bad64 = [], 0, 0
+            errs, bad32, bad64, conv = [], 0, 0, []
+            # instances whose density residual reached the drop in both precisions
+            reached = (r["r1"] <= args.drop * r["r0"]) & (r64["r1"] <= args.drop * r64["r0"])
             for i, (g, c) in enumerate(cases):
                 q32, q64 = r["batch"].state(i), r64["batch"].state(i)
                 f32, f64 = bool(np.isfinite(q32).all()), bool(np.isfinite(q64).all())
@@ -633,6 +635,8 @@ def bench_batch(args):
                 bad64 += not f64
                 if f32 and f64:
                     errs.append(errors_vs_fp64(q32, q64, c.gamma))
+                    if reached[i]:
+                        conv.append(errs[-1]["rho_l2_rel"])
             rho = np.array([e["rho_l2_rel"] for e in errs])
             pl2 = np.array([e["p_l2_rel"] for e in errs])
             # the FP64 scheme's own response to an FP32-rounding-sized perturbation
@@ -660,6 +664,14 @@ def bench_batch(args):
                 "instances_rho_l2_le_1e-5": int((rho <= 1e-5).sum()),
                 "instances_compared": len(errs), "nonfinite_fp32": bad32, "nonfinite_fp64": bad64,
                 "compared": "each instance's final FP32-mixed state vs its final FP64 state"}
+            conv = np.array(conv)
+            out["error_at_convergence"] = {
+                "instances": int(conv.size),
+                "rho_l2_rel_median": float(f"{np.median(conv):.3e}") if conv.size else None,
+                "rho_l2_rel_max": float(f"{conv.max():.3e}") if conv.size else None,
+                "instances_rho_l2_le_1e-5": int((conv <= 1e-5).sum()),
+                "compared": f"instances whose density residual fell to {args.drop:g} of its "
+                            "first-step value in both precisions"}
     if not args.no_cpu:
         # a bounded sample: the smallest, the median and the largest instance
         order = np.argsort(nc)
