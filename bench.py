@@ -407,9 +407,9 @@ def bench_single(args):
     if args.autotune and args.variant in (0, 3):
         from fv2d import autotune
         for prec in precs:
-            best, table = autotune.tune_single(grid, case, PRECISIONS[prec], log=log,
-                                               kernel_variant=args.variant,
-                                               extra=autotune.ATOMIC if prec == "fp32" else None)
+            best, table = autotune.tune_single(
+                grid, case, PRECISIONS[prec], log=log, kernel_variant=args.variant,
+                extra=autotune.RCM + (autotune.ATOMIC if prec == "fp32" else []))
             tuned[prec] = {"choice": best,
                            "ms_per_step": [{**o, "ms": round(ms, 3)} for o, ms in table]}
     walls = case.BC1 in ("SLIP_WALL", "NO_SLIP_WALL")   # the geometry, from the first run
@@ -575,8 +575,9 @@ def bench_batch(args):
     if args.autotune:
         from fv2d import autotune
         for prec in precs:
-            best, table = autotune.tune_batch(cases, PRECISIONS[prec], log=log,
-                                              extra=autotune.ATOMIC if prec == "fp32" else None)
+            best, table = autotune.tune_batch(
+                cases, PRECISIONS[prec], log=log,
+                extra=autotune.RCM + (autotune.ATOMIC if prec == "fp32" else []))
             tuned[prec] = {"choice": best,
                            "ms_per_step": [{**o, "ms": round(ms, 3)} for o, ms in table]}
     runs = {}

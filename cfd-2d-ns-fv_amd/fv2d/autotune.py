@@ -21,6 +21,8 @@ from .lib import Batch, Solver
 DEFAULT_SPACE = {"block_size": [128, 256], "renumber": [1], "xcd_remap": [8, 0, 1]}
 # the atomic-scatter accumulation at the batch's best launch shape
 ATOMIC = [{"block_size": 128, "renumber": 1, "xcd_remap": 8, "accumulate": 1}]
+# reverse Cuthill-McKee cell order (longer ranges, larger halos) at the same shape
+RCM = [{"block_size": 128, "renumber": 2, "xcd_remap": 8}]
 
 
 def candidates(space=None, extra=None):
