@@ -387,11 +387,18 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
     // order each block's cells by a further bisection into 64-cell groups
     std::vector<int64_t> start(h.nblocks + 1, 0);
     for (int32_t b = 0; b < h.nblocks; ++b) start[b + 1] = start[b] + sizes[b];
+    // then by face count, most first, keeping that order among equals: a
+    // 64-cell chunk (one wavefront) then holds cells of one face count
+    // wherever the block allows, so its slot rows carry little padding
+    // (a mesh of mixed triangles and quads loses most of its padding)
     parallel_for(h.nblocks, threads, [&](int64_t b0, int64_t b1) {
       for (int64_t b = b0; b < b1; ++b) {
         const int32_t n = sizes[b];
-        rcb(order.data() + start[b], n, (n + kChunk - 1) / kChunk, kChunk, h.ccx.data(),
-            h.ccy.data(), 1);
+        int32_t* first = order.data() + start[b];
+        rcb(first, n, (n + kChunk - 1) / kChunk, kChunk, h.ccx.data(), h.ccy.data(), 1);
+        std::stable_sort(first, first + n, [&](int32_t x, int32_t y) {
+          return h.cf_ptr[x + 1] - h.cf_ptr[x] > h.cf_ptr[y + 1] - h.cf_ptr[y];
+        });
       }
     });
   } else {
