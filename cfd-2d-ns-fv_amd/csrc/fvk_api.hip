@@ -692,12 +692,16 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   }
   if (ncells >= (int64_t(1) << 29)) return set_err(nullptr, FVK_ERR_MESH, "more than 2^29 cells in total");
   {
-    int32_t cell = 0, block = 0, slot = 0, halo = 0, bface = 0;
+    // the concatenated layout's offsets are 32-bit on the device
+    int64_t cell = 0, block = 0, slot = 0, halo = 0, bface = 0;
     for (Instance& in : c->in) {
-      in.cell_off = cell; in.block_off = block; in.slot_off = slot;
-      in.halo_off = halo; in.bface_off = bface;
+      in.cell_off = (int32_t)cell; in.block_off = (int32_t)block; in.slot_off = (int32_t)slot;
+      in.halo_off = (int32_t)halo; in.bface_off = (int32_t)bface;
       cell += in.hm.ncp; block += in.hm.nblocks; slot += in.hm.chunk_off.back();
-      halo += (int32_t)in.hm.halo_idx.size(); bface += (int32_t)in.hm.bf_a.size();
+      halo += (int64_t)in.hm.halo_idx.size(); bface += (int64_t)in.hm.bf_a.size();
+      if (std::max({slot, halo, bface}) >= (int64_t(1) << 31))
+        return set_err(nullptr, FVK_ERR_MESH, "more than 2^31 face slots, halo entries or block "
+                                              "faces in total");
     }
   }
   const fvk_options& o0 = opts[0];

@@ -497,10 +497,16 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
     if (c >= 0) h.deg[i] = (uint8_t)(h.cf_ptr[c + 1] - h.cf_ptr[c]);
   }
   h.chunk_off.assign(nchunks + 1, 0);
+  int64_t slots = 0;   // slot offsets are 32-bit on the device
   for (int32_t k = 0; k < nchunks; ++k) {
     int32_t mx = 0;
     for (int32_t l = 0; l < kChunk; ++l) mx = std::max<int32_t>(mx, h.deg[k * kChunk + l]);
-    h.chunk_off[k + 1] = h.chunk_off[k] + mx * kChunk;
+    slots += (int64_t)mx * kChunk;
+    if (slots >= (int64_t(1) << 31)) {
+      *err = "mesh: more than 2^31 face slots";
+      return FVK_ERR_MESH;
+    }
+    h.chunk_off[k + 1] = (int32_t)slots;
   }
   const int64_t nslots = h.chunk_off[nchunks];
   // every slot is written below: a cell's own slots, then zeros in the
