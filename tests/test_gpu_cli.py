@@ -106,6 +106,24 @@ def test_cli_wall_forces_match_the_python_integrator(tmp_path):
         assert np.allclose([float(r[1]), float(r[2])], [cl, cd], rtol=2e-9, atol=1e-12), r
 
 
+def test_cli_uniform_time_step(tmp_path):
+    """time_step_type = UNIFORM (the time-accurate extension) runs through the
+    CLI, and sol/forces.dat carries the physical time, increasing."""
+    for fn in ("grid.in", "input.in"):
+        shutil.copy(os.path.join(CASE, fn), tmp_path / fn)
+    txt = (tmp_path / "input.in").read_text().replace("time_step_type = LOCAL",
+                                                      "time_step_type = UNIFORM")
+    assert "UNIFORM" in txt
+    (tmp_path / "input.in").write_text(txt)
+    r = subprocess.run([CLI_PATH, "input.in"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = [l.split() for l in (tmp_path / "sol" / "forces.dat").read_text().splitlines()
+            if not l.startswith("#")]
+    t = [float(row[3]) for row in rows]
+    assert len(t) == 6 and all(b > a > 0 for a, b in zip(t, t[1:]))
+
+
 def test_cli_restart_continues(tmp_path):
     """restart = 1 reads sol/restart_sol.dat and continues from nmax+1
     (main.cpp:82-106); the run must finish and append to the histories."""
