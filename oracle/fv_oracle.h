@@ -1,6 +1,6 @@
 /* fv_oracle.h — CPU restatement of the reference FV solver (TEST INFRASTRUCTURE).
  *
- * This library is the checker: only tests/, __graft_entry__.smoke() and the
+ * This library is the test oracle: only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py load it.  The product (libfvk.so and the fv2d
  * CLI) never links or calls it.
  *
