@@ -755,8 +755,25 @@ __device__ __forceinline__ void cell_update_ld(const StageArgsT<T>& a, const Flo
   cell_update(a, fl, 0, c, q, qn, r, sg, a.m.vol[c], sig_old);
 }
 
+// LSQ inverse of cell c: FP64, or the same values rounded once to FP32 where
+// the engine provides them (FP32-mixed on meshes whose every normal matrix is
+// well conditioned: the products below stay FP64, so the gradient's relative
+// error stays within about 6e-8 times the condition number)
+__device__ __forceinline__ void load_lsq_inv(const DevMesh& m, int c, double& ixx, double& iyy,
+                                             double& ixy) {
+  if (m.ixx32) {
+    ixx = m.ixx32[c];
+    iyy = m.iyy32[c];
+    ixy = m.ixy32[c];
+  } else {
+    ixx = m.ixx[c];
+    iyy = m.iyy[c];
+    ixy = m.ixy[c];
+  }
+}
+
 // gradient from the LSQ sums (cell_gradient.cpp:120-130); the FP32-mixed
-// path multiplies by the FP64 inverse in FP64
+// path multiplies by the inverse in FP64
 template <typename T>
 __device__ __forceinline__ void lsq_solve(double ixx, double iyy, double ixy, const T sx[4],
                                           const T sy[4], T gx[4], T gy[4]) {
@@ -844,7 +861,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_cell(StageArgsT<T> a) {
     accumulate_lsq(sx, sy, qmin, qmax, px, py, tmin, tmax);
   }
   T gx[4], gy[4];
-  lsq_solve(m.ixx[c], m.iyy[c], m.ixy[c], sx, sy, gx, gy);
+  double ixx, iyy, ixy;
+  load_lsq_inv(m, c, ixx, iyy, ixy);
+  lsq_solve(ixx, iyy, ixy, sx, sy, gx, gy);
   if (SQUEEZE) {
     T phi[4] = {1, 1, 1, 1}, qp[4], gpx[4], gpy[4];
     load_lagged(a, c, qp, gpx, gpy);
@@ -972,7 +991,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_slot(StageArgsT<T> a) {
     __syncthreads();
   }
   T gx[4], gy[4];
-  if (row == 0 && active) lsq_solve(m.ixx[c], m.iyy[c], m.ixy[c], sx, sy, gx, gy);
+  if (row == 0 && active) {
+    double ixx, iyy, ixy;
+    load_lsq_inv(m, c, ixx, iyy, ixy);
+    lsq_solve(ixx, iyy, ixy, sx, sy, gx, gy);
+  }
   if (SQUEEZE) {
     if (row == 0) {
 #pragma unroll
@@ -1214,7 +1237,7 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
   T qp[4], gpx[4], gpy[4];
   if (squeeze && deg > 0) load_lagged(a, c, qp, gpx, gpy);
   double ixx = 0, iyy = 0, ixy = 0;
-  if (deg > 0) { ixx = m.ixx[c]; iyy = m.iyy[c]; ixy = m.ixy[c]; }
+  if (deg > 0) load_lsq_inv(m, c, ixx, iyy, ixy);
   stage_block(a, b, L);
   __syncthreads();
   if (deg == 0) return;

@@ -39,6 +39,9 @@ struct DevMesh {
   const double* ixx;           // per cell: LSQ inverse (readmesh.cpp:316-320)
   const double* iyy;
   const double* ixy;
+  const float* ixx32;          // the same rounded to FP32 (FP32-mixed on well-conditioned
+  const float* iyy32;          // meshes; null otherwise, and then ixx.. are read)
+  const float* ixy32;
   const int32_t* halo_ptr;     // per block: halo cells
   const int32_t* halo_idx;
   const int32_t* bf_ptr;       // per block: face list

@@ -54,6 +54,7 @@ struct Engine {
   std::vector<void*> bufs;      // device allocations, freed by engine_free
   fvk::DevMesh dm{};
   bool fp32 = false;            // FP32-mixed: state = float(Q - Q_inf)
+  bool lsq32 = false;           // FP32-mixed with FP32 LSQ inverses
   size_t tsize = 8;             // bytes per state value
   void* flows = nullptr;        // FlowT<T>[n]
   int32_t* blk_inst = nullptr;  // batch: per-block instance
@@ -97,6 +98,9 @@ namespace {
 // a 7.8M-cell mesh and an 11.3M-cell batch of 256 meshes)
 constexpr int64_t kAutoLargeCells = 2000000;    // FP32-mixed: 128-cell blocks above this
 constexpr int32_t kAutoContiguousBlocks = 4096; // contiguous XCD ranges up to this many blocks
+// FP32-mixed: FP32 LSQ inverses when no cell's normal matrix has a condition
+// number above this (the gradient's relative error then stays below ~6e-6)
+constexpr double kLsq32MaxCond = 100.0;
 
 // host threads of the mesh preparation: FVK_PREP_THREADS (1..64) when set,
 // else the hardware threads, at most 16
@@ -764,9 +768,23 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   if ((rc = upload_cat(c, &HM::c_ccx, &d.ccx))) return rc;
   if ((rc = upload_cat(c, &HM::c_ccy, &d.ccy))) return rc;
   if ((rc = upload_cat(c, &HM::c_vol, &d.vol))) return rc;
-  if ((rc = upload_cat(c, &HM::c_ixx, &d.ixx))) return rc;
-  if ((rc = upload_cat(c, &HM::c_iyy, &d.iyy))) return rc;
-  if ((rc = upload_cat(c, &HM::c_ixy, &d.ixy))) return rc;
+  // LSQ inverses: FP64, or in the FP32-mixed form the same values rounded
+  // to FP32 when every cell's normal matrix is well conditioned
+  double cond = 0;
+  for (const Instance& in : c->in) cond = std::max(cond, in.hm.max_lsq_cond);
+  const char* force64 = std::getenv("FVK_LSQ_FP64");
+  c->lsq32 = c->fp32 && cond <= kLsq32MaxCond && !(force64 && std::atoi(force64) == 1);
+  d.ixx = d.iyy = d.ixy = nullptr;
+  d.ixx32 = d.iyy32 = d.ixy32 = nullptr;
+  if (c->lsq32) {
+    if ((rc = upload_cat(c, &HM::c_ixx32, &d.ixx32))) return rc;
+    if ((rc = upload_cat(c, &HM::c_iyy32, &d.iyy32))) return rc;
+    if ((rc = upload_cat(c, &HM::c_ixy32, &d.ixy32))) return rc;
+  } else {
+    if ((rc = upload_cat(c, &HM::c_ixx, &d.ixx))) return rc;
+    if ((rc = upload_cat(c, &HM::c_iyy, &d.iyy))) return rc;
+    if ((rc = upload_cat(c, &HM::c_ixy, &d.ixy))) return rc;
+  }
   if (c->fp32) {
     // the FP32 kernels' single-precision geometry (mesh_prep.hpp)
     if ((rc = upload_cat(c, &HM::s_dfx32, &d.s_dfx32))) return rc;

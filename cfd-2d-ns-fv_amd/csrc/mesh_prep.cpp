@@ -116,6 +116,7 @@ inline void ghost_centroid(const HostMesh& h, int32_t f, double* gx, double* gy)
 // (x_self - x_other)^2 ..., as the owner's are.
 void lsq_geometry(HostMesh& h, int threads) {
   h.ixx.resize(h.nc); h.iyy.resize(h.nc); h.ixy.resize(h.nc);
+  std::vector<double> cond(h.nc);
   parallel_for(h.nc, threads, [&](int64_t c0, int64_t c1) {
   for (int32_t c = (int32_t)c0; c < (int32_t)c1; ++c) {
     double sxx = 0.0, syy = 0.0, sxy = 0.0;
@@ -135,8 +136,13 @@ void lsq_geometry(HostMesh& h, int threads) {
     h.ixx[c] = sxx / det;
     h.iyy[c] = syy / det;
     h.ixy[c] = sxy / det;
+    // condition number of [[sxx, sxy], [sxy, syy]] from its eigenvalues
+    const double tr = sxx + syy, disc = std::sqrt(std::max(0.0, tr * tr - 4 * det));
+    const double lo = 0.5 * (tr - disc);
+    cond[c] = lo > 0 ? 0.5 * (tr + disc) / lo : HUGE_VAL;
   }
   });
+  h.max_lsq_cond = *std::max_element(cond.begin(), cond.end());
 }
 
 // Reverse Cuthill-McKee over the interior-face graph.  Each connected part
@@ -627,6 +633,17 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
   permute(h.ixx, h.c_ixx, 0.0);
   permute(h.iyy, h.c_iyy, 0.0);
   permute(h.ixy, h.c_ixy, 0.0);
+  if (o.precision == FVK_FP32_MIXED) {
+    auto round32 = [&](const hvec<double>& src, hvec<float>& dst) {
+      dst.resize(h.ncp);
+      parallel_for(h.ncp, threads, [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) dst[i] = (float)src[i];
+      });
+    };
+    round32(h.c_ixx, h.c_ixx32);
+    round32(h.c_iyy, h.c_iyy32);
+    round32(h.c_ixy, h.c_ixy32);
+  }
   return FVK_OK;
 }
 

@@ -78,6 +78,7 @@ struct HostMesh {
   std::vector<int32_t> cf_ptr, cf_idx;
   hvec<double> area, nx, ny, fcx, fcy;
   hvec<double> vol, ccx, ccy, ixx, iyy, ixy;
+  double max_lsq_cond = 0;  // largest condition number of a cell's LSQ normal matrix
   // renumbering
   std::vector<int32_t> perm;   // ncp: new -> old, -1 for padding
   std::vector<int32_t> iperm;  // nc: old -> new
@@ -92,6 +93,9 @@ struct HostMesh {
   hvec<double> s_nx, s_ny, s_area, s_fcx, s_fcy;   // per slot (0 in padding slots)
   // per cell (new numbering, padded)
   hvec<double> c_ccx, c_ccy, c_vol, c_ixx, c_iyy, c_ixy;
+  // FP32-mixed only: the LSQ inverse entries rounded once to FP32 (used when
+  // every cell's normal matrix is well conditioned)
+  hvec<float> c_ixx32, c_iyy32, c_ixy32;
   // per block
   std::vector<int32_t> halo_ptr, halo_idx;            // halo cells (global new index)
   std::vector<int32_t> bf_ptr;                        // block face lists

@@ -225,3 +225,26 @@ def test_fp32_ns_layout_invariance():
         s.step(30)
         out.append(s.state())
     assert same(out[0], out[1])
+
+
+def test_fp32_lsq_inverses_in_fp32(monkeypatch):
+    """On a well-conditioned mesh the FP32-mixed form reads the LSQ inverses
+    rounded to FP32 (the products stay FP64); FVK_LSQ_FP64=1 keeps them FP64.
+    Both stay within FP32 rounding of each other and of the FP64 path."""
+    u, v = inputs.mach_velocity(0.5, angle_deg=2.0)
+    grid = gg.bump(64, 32)
+    case = inputs.Case(u=u, v=v, poly=1, limiter="NONE", dt_type="LOCAL", CFL=0.8)
+    out = {}
+    for force in ("0", "1"):
+        monkeypatch.setenv("FVK_LSQ_FP64", force)
+        s = Solver(grid, case, precision=FP32_MIXED)
+        s.init_freestream()
+        s.step(200)
+        out[force] = s.state()
+    s = Solver(grid, case)
+    s.init_freestream()
+    s.step(200)
+    q64 = s.state()
+    rel = lambda a, b: float(np.linalg.norm(a[0] - b[0]) / np.linalg.norm(b[0]))
+    assert 0 < rel(out["0"], out["1"]) < 1e-6
+    assert rel(out["0"], q64) < 1e-6 and rel(out["1"], q64) < 1e-6
