@@ -550,6 +550,86 @@ __device__ __forceinline__ void viscous_flux(const StageArgsT<T>& a, const FlowT
   Fv[3] = nx * (fu * txx + fv * txy + kc * dTdx) + ny * (fu * txy + fv * tyy + kc * dTdy);
 }
 
+#if FVK_FAST32
+// The FP32-mixed viscous flux: viscous_flux's expressions with one hardware
+// reciprocal (1 ulp) per divisor, 1/rho_f, 1/|d| (interior) or 1/d_wall and
+// 1/(R rho_f^2), in place of its fourteen correctly rounded quotients, and the
+// face-averaged state formed from the perturbations, Q_inf + (dL + dR)/2.
+template <typename T>
+__device__ __forceinline__ void viscous_flux_fast(const StageArgsT<T>& a, const FlowT<T>& fl,
+                                                  const CellQG<T>& c1, const CellQG<T>* c2,
+                                                  const FaceGeo<T>& g, const T L[4], const T R[4],
+                                                  T Fv[4]) {
+  static_assert(kDelta<T>, "the FP32-mixed path only");
+  const T gm1 = fl.gm1;
+  const T nx = g.nx, ny = g.ny;
+  T fa[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) fa[k] = __builtin_fmaf(T(0.5), L[k] + R[k], fl.qinf[k]);
+  const T frho = fa[0], fE = fa[3];
+  const T irho = __builtin_amdgcn_rcpf(frho);
+  const T fu = fa[1] * irho, fv = fa[2] * irho;
+  const T ke = __builtin_fmaf(fv, fv, fu * fu);
+  const T fP = __builtin_fmaf(T(-0.5) * frho, ke, fE) * gm1;
+  T ux, uy, vx, vy, Px = 0, Py = 0, rhox = 0, rhoy = 0;
+  if (!c2) {              // no-slip wall
+    T cu, cv;
+    if (a.fs_prims) {
+      cu = fl.u;
+      cv = fl.v;
+    } else {
+      const T ic = __builtin_amdgcn_rcpf(fl.qinf[0] + c1.q[0]);
+      cu = (fl.qinf[1] + c1.q[1]) * ic;
+      cv = (fl.qinf[2] + c1.q[2]) * ic;
+    }
+    const T idn = __builtin_amdgcn_rcpf(g.dist);
+    const T dudn = -cu * idn;
+    const T dvdn = -cv * idn;
+    ux = dudn * nx;
+    uy = dudn * ny;
+    vx = dvdn * nx;
+    vy = dvdn * ny;
+  } else {                // interior
+    const T idist = __builtin_amdgcn_rcpf(fabsf(g.dist));
+    const T mx = -ny, my = nx;
+    T ddx[4], ddy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const T dl = (c2->q[k] - c1.q[k]) * idist;
+      const T dm = T(0.5) * __builtin_fmaf(c2->gy[k], my, __builtin_fmaf(c2->gx[k], mx,
+                                           __builtin_fmaf(c1.gy[k], my, c1.gx[k] * mx)));
+      const T s = dl + dm;
+      ddy[k] = s * g.ly;
+      ddx[k] = s * g.lx;
+    }
+    rhox = ddx[0];
+    rhoy = ddy[0];
+    const T rx = rhox * irho, ry = rhoy * irho;
+    ux = __builtin_fmaf(-rx, fu, ddx[1]);
+    vx = __builtin_fmaf(-rx, fv, ddx[2]);
+    uy = __builtin_fmaf(-ry, fu, ddy[1]);
+    vy = __builtin_fmaf(-ry, fu, ddy[2]);   // the reference uses u here
+    Px = gm1 * __builtin_fmaf(-frho, __builtin_fmaf(fv, vx, fu * ux),
+                              __builtin_fmaf(T(-0.5) * rhox, ke, ddx[3]));
+    Py = gm1 * __builtin_fmaf(-frho, __builtin_fmaf(fv, vy, fu * uy),
+                              __builtin_fmaf(T(-0.5) * rhoy, ke, ddy[3]));
+  }
+  const T mu = fl.mu, kc = fl.k;
+  const T dv3 = (ux + vy) * T(1.0 / 3.0);
+  const T txx = T(2) * mu * (ux - dv3);
+  const T txy = mu * (uy + vx);
+  const T tyy = T(2) * mu * (vy - dv3);
+  const T iRT = __builtin_amdgcn_rcpf(fl.Rgas * frho * frho);
+  const T dTdx = __builtin_fmaf(Px, frho, -rhox * fP) * iRT;
+  const T dTdy = __builtin_fmaf(Py, frho, -rhoy * fP) * iRT;
+  Fv[0] = 0;
+  Fv[1] = __builtin_fmaf(txy, ny, txx * nx);
+  Fv[2] = __builtin_fmaf(tyy, ny, txy * nx);
+  Fv[3] = __builtin_fmaf(ny, __builtin_fmaf(kc, dTdy, __builtin_fmaf(fv, tyy, fu * txy)),
+                         nx * __builtin_fmaf(kc, dTdx, __builtin_fmaf(fv, txy, fu * txx)));
+}
+#endif
+
 // Pass B: one face with owner c1 and neighbour c2 (interior) or a BC ghost of
 // the given kind.  x = (Fv - F)*A, the owner's residual term (the neighbour's
 // is -x, res.cpp:81-101); sterm = (|Vn_avg| + c_avg) * A (ssp_rk3.cpp:32-34).
@@ -665,6 +745,10 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
       viscous_flux<double>(a, fl, c1, c2, g, L, R, Fd);
 #pragma unroll
       for (int k = 0; k < 4; ++k) Fv[k] = T(Fd[k]);
+    } else
+#elif FVK_FAST32
+    if constexpr (kDelta<T>) {
+      viscous_flux_fast(a, fl, c1, c2, g, L, R, Fv);
     } else
 #endif
     {
