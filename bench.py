@@ -745,7 +745,7 @@ def main():
         stream_probe(args.stream_probe)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the solver runs on one GPU only; --gpus {args.gpus} is not supported "
               "(instances or mesh partitions would shard across GPUs)",
               file=sys.stderr)
         sys.exit(2)

@@ -1,7 +1,7 @@
 """input.in options: the reference's nine sections (src/base/read_inputs.cpp:20-209).
 
 `Case` holds one run's options with the reference's spelling; `write()` emits a
-file both the reference and this build's `fv2d` CLI read.  Numbers are written
+file both the reference and the drop-in `fv2d` CLI read.  Numbers are written
 with repr() so they round-trip exactly through stod().
 """
 from __future__ import annotations

@@ -81,7 +81,7 @@ def test_cli_grid_cache(tmp_path):
 
 
 def test_cli_wall_forces_match_the_python_integrator(tmp_path):
-    """sol/forces.dat (this build's wall integral at the true face centres)
+    """sol/forces.dat (the CLI's own wall integral at the true face centres)
     holds, at every monitor step, the lift and drag coefficients that
     fv2d.analysis computes from the same state and limited gradients."""
     import numpy as np
