@@ -53,7 +53,8 @@ import numpy as np  # noqa: E402
 
 from fv2d import inputs  # noqa: E402
 from fv2d import workloads as W  # noqa: E402
-from fv2d.lib import FP32_MIXED, FP64, Batch, Solver  # noqa: E402
+from fv2d.lib import (FP32_MIXED, FP64, RETIRED_DROP, RETIRED_NONFINITE,  # noqa: E402
+                      RETIRED_STALL, Batch, Solver)
 
 METRIC = "cell-RK-stage updates/s, FP32-mixed vs FP64 at stated error; % HBM peak"
 HBM_PEAK_GBS = 8000.0
@@ -157,8 +158,7 @@ def copy_calibration(mbytes: int) -> None:
 def stream_probe(mbytes: int) -> None:
     """Achievable bandwidth on this GPU, beside the 8 TB/s peak: the HIP
     runtime's device-to-device copy of `mbytes` MiB (read + write bytes /
-    time, 20 timed copies), then `lib/fvk_bw_probe` (read-only, four reads
-    per write and copy kernels over arrays of `mbytes` MiB, best grid)."""
+    time, 20 timed copies)."""
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     n = mbytes << 20
@@ -181,10 +181,6 @@ def stream_probe(mbytes: int) -> None:
     hip.hipFree(dst)
     gbs = 2.0 * n * reps / (ms.value * 1e-3) / 1e9
     out = {"stream_probe_mib": mbytes, "copies": reps, "GB_per_s": round(gbs, 1)}
-    probe = os.path.join(ROOT, "cfd-2d-ns-fv_amd", "lib", "fvk_bw_probe")
-    if os.path.exists(probe):
-        r = subprocess.run([probe, str(mbytes)], capture_output=True, text=True, check=True)
-        out["kernels_GB_per_s"] = json.loads(r.stdout)
     print(json.dumps(out))
 
 
@@ -520,18 +516,22 @@ def run_batch(prec: str, args, cases, nc, opts):
     args.steps steps) with retirement; then time each kernel kind per launch
     over all instances."""
     b = Batch(cases, precision=PRECISIONS[prec], **opts)
+    if args.stall_checks:
+        b.set_stall(args.stall_checks, args.stall_factor)
     b.init()
     if args.warmup:
+        # the warm-up captures the step graph, which the restart keeps
         b.step(args.warmup)
         b.init()
     t0 = time.perf_counter()
     steps, r0, r1 = b.run_to_drop(args.drop, args.steps, args.check_every)
     wall = time.perf_counter() - t0
+    why = b.retired()
     work = float((nc * 3 * steps).sum())
     b.set_active(np.ones(len(cases), dtype=np.uint8))
     per_launch = {k: v for k, v in b.time_kernels(args.kernel_reps).items() if v > 0}
     return {"batch": b, "wall": wall, "value": work / wall, "steps": steps, "r0": r0, "r1": r1,
-            "per_launch": per_launch, "work": work}
+            "why": why, "per_launch": per_launch, "work": work}
 
 
 def sequential_fp64(cases, steps, reps: int = 20):
@@ -585,7 +585,8 @@ def bench_batch(args):
         runs[prec] = run_batch(prec, args, cases, nc, tuned[prec]["choice"] if tuned else opts)
         r = runs[prec]
         log(f"{prec} batch {r['value']:.4g} cell-stage/s, {r['wall']:.2f} s, "
-            f"retired by drop {int((r['r1'] <= args.drop * r['r0']).sum())}")
+            f"retired by drop {int((r['why'] == RETIRED_DROP).sum())}, "
+            f"by stall {int((r['why'] == RETIRED_STALL).sum())}")
     head = "fp32" if "fp32" in runs else "fp64"
     r = runs[head]
     algo = algorithmic_bytes(cases[0][1], int(nc.sum()), int(nf.sum()),
@@ -606,7 +607,12 @@ def bench_batch(args):
            "launch_options": chosen, "autotune": tuned or None,
            "graph": "two-step hipGraph replay",
            "steps_min_median_max": [int(steps.min()), int(np.median(steps)), int(steps.max())],
-           "retired_by_drop": int((r["r1"] <= args.drop * r["r0"]).sum())}
+           "stall_retirement": ({"checks": args.stall_checks, "factor": args.stall_factor}
+                                if args.stall_checks else None),
+           "retired": {p: {"drop": int((runs[p]["why"] == RETIRED_DROP).sum()),
+                           "stall": int((runs[p]["why"] == RETIRED_STALL).sum()),
+                           "nonfinite": int((runs[p]["why"] == RETIRED_NONFINITE).sum())}
+                       for p in runs}}
     out = {"metric": METRIC, "value": r["value"], "unit": "cell-stage/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": r["wall"] * 1e3 / max(1, int(steps.max())), "higher_is_better": True,
@@ -627,8 +633,10 @@ def bench_batch(args):
         if head == "fp32":
             out["speedup_vs_fp64_batched"] = round(r["value"] / r64["value"], 4)
             errs, bad32, bad64, conv = [], 0, 0, []
-            # instances whose density residual reached the drop in both precisions
-            reached = (r["r1"] <= args.drop * r["r0"]) & (r64["r1"] <= args.drop * r64["r0"])
+            # instances that converged in both precisions: retired by the
+            # residual drop or, with stall retirement, by a stalled residual
+            conv_why = (RETIRED_DROP, RETIRED_STALL)
+            reached = np.isin(r["why"], conv_why) & np.isin(r64["why"], conv_why)
             for i, (g, c) in enumerate(cases):
                 q32, q64 = r["batch"].state(i), r64["batch"].state(i)
                 f32, f64 = bool(np.isfinite(q32).all()), bool(np.isfinite(q64).all())
@@ -643,6 +651,8 @@ def bench_batch(args):
             # the FP64 scheme's own response to an FP32-rounding-sized perturbation
             log("FP64 sensitivity floor batch")
             fb = Batch(cases, **(tuned["fp64"]["choice"] if tuned else opts))
+            if args.stall_checks:
+                fb.set_stall(args.stall_checks, args.stall_factor)
             fb.init([perturbed_freestream(g, c, seed=11 + i) for i, (g, c) in enumerate(cases)])
             fb.run_to_drop(args.drop, args.steps, args.check_every)
             fl_rho = []
@@ -672,7 +682,9 @@ def bench_batch(args):
                 "rho_l2_rel_max": float(f"{conv.max():.3e}") if conv.size else None,
                 "instances_rho_l2_le_1e-5": int((conv <= 1e-5).sum()),
                 "compared": f"instances whose density residual fell to {args.drop:g} of its "
-                            "first-step value in both precisions"}
+                            "first-step value"
+                            + (" or stalled" if args.stall_checks else "")
+                            + " in both precisions"}
     if not args.no_cpu:
         # a bounded sample: the smallest, the median and the largest instance
         order = np.argsort(nc)
@@ -699,6 +711,11 @@ def main():
     ap.add_argument("--limiter", default="", choices=["", "NONE", "SQUEEZE"],
                     help="batch: a variant with this limiter (default: the configured SQUEEZE)")
     ap.add_argument("--check-every", type=int, default=100)
+    ap.add_argument("--stall-checks", type=int, default=0,
+                    help="batch: also retire an instance after this many checks in a row "
+                         "without its density residual falling below --stall-factor times "
+                         "its lowest counted value (0: off)")
+    ap.add_argument("--stall-factor", type=float, default=0.5)
     ap.add_argument("--chunk", type=int, default=None,
                     help="single mesh: steps between residual checks and force samples")
     ap.add_argument("--drop-single", type=float, default=None,

@@ -124,6 +124,14 @@ int main(int argc, char** argv) {
       double t = -1.0;   // physical time: uniform time step only
       if (in.dt_type == 3) fvk_get_time(ctx, &t);
       fv2d::monitor(ndx, grid, geo, in, q.data(), grad.data(), res, t);
+      // the reference carries on through NaNs; the drop-in stops after the
+      // monitor line that shows them
+      if (!(std::isfinite(res[0]) && std::isfinite(res[1]) && std::isfinite(res[2]) &&
+            std::isfinite(res[3]))) {
+        std::cout << "ERROR: non-finite residual norm at iteration " << ndx << std::endl;
+        fvk_destroy(ctx);
+        return 3;
+      }
     }
     if (outp) fv2d::write_tecplot(ndx, grid, geo, in, q.data(), grad.data());
     ++ndx;

@@ -1678,6 +1678,70 @@ __global__ __launch_bounds__(kMaxBlock) void k_res_sq(const T* __restrict__ res,
   }
 }
 
+// Error-free transformations (Knuth's TwoSum; the exact product error by one
+// fused multiply-add): s + e == a + b and p + e == a * b exactly.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Global conservation sums: per mesh block, the compensated sums over its
+// cells of V_i * q_k,i (k = 0..3, the stored state: conserved variables, or
+// the FP32-mixed perturbation) and of V_i, as (hi, lo) pairs whose sum is the
+// block's sum to about twice the working precision:
+// partial[(2j)*nblocks + b] = hi_j, partial[(2j+1)*nblocks + b] = lo_j, j = 0..4.
+// One thread block per mesh block (blockDim = blk, grid = ncp / blk).
+template <typename T>
+__global__ __launch_bounds__(kMaxBlock) void k_cons_sum(const T* __restrict__ q,
+                                                        const double* __restrict__ vol,
+                                                        const uint8_t* __restrict__ deg,
+                                                        double* __restrict__ partial) {
+  __shared__ double ws[10][kMaxBlock / 64];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double hi[5], lo[5];
+  const bool live = deg[c] > 0;
+  const double V = live ? vol[c] : 0.0;
+  T qc[4];
+  ld4(q + 4 * (size_t)c, qc);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double x = live ? double(qc[k]) : 0.0;
+    hi[k] = V * x;
+    lo[k] = __builtin_fma(V, x, -hi[k]);
+  }
+  hi[4] = V;
+  lo[4] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 5; ++j)
+    for (int off = 32; off > 0; off >>= 1) {
+      const double oh = __shfl_xor(hi[j], off, 64), ol = __shfl_xor(lo[j], off, 64);
+      double s, e;
+      two_sum(hi[j], oh, s, e);
+      hi[j] = s;
+      lo[j] = (lo[j] + ol) + e;
+    }
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      ws[2 * j][threadIdx.x >> 6] = hi[j];
+      ws[2 * j + 1][threadIdx.x >> 6] = lo[j];
+    }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const int j = threadIdx.x;
+    double h = ws[2 * j][0], l = ws[2 * j + 1][0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      double s, e;
+      two_sum(h, ws[2 * j][w], s, e);
+      h = s;
+      l = (l + ws[2 * j + 1][w]) + e;
+    }
+    partial[(2 * j) * gridDim.x + blockIdx.x] = h;
+    partial[(2 * j + 1) * gridDim.x + blockIdx.x] = l;
+  }
+}
+
 template <typename T>
 __global__ void k_fill(T* p, long long n, T v) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1804,6 +1868,12 @@ hipError_t launch_res_sq(const T* res, int32_t ncp, int32_t blk, double* partial
 }
 
 template <typename T>
+hipError_t launch_cons_sum(const T* q, const DevMesh& m, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_cons_sum<T>, dim3(m.ncp / m.blk), dim3(m.blk), 0, s, q, m.vol, m.deg, partial);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int block = 256;
@@ -1817,6 +1887,7 @@ hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s) {
   template hipError_t launch_pass_b<T>(const StageArgsT<T>&, int, int, int, hipStream_t);       \
   template hipError_t launch_init_sigma<T>(const StageArgsT<T>&, hipStream_t);                  \
   template hipError_t launch_res_sq<T>(const T*, int32_t, int32_t, double*, hipStream_t);       \
+  template hipError_t launch_cons_sum<T>(const T*, const DevMesh&, double*, hipStream_t);        \
   template hipError_t launch_fill<T>(T*, int64_t, T, hipStream_t);
 FVK_INSTANTIATE(double)
 FVK_INSTANTIATE(float)

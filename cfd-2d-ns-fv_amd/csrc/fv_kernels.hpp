@@ -143,6 +143,10 @@ hipError_t launch_init_sigma(const StageArgsT<T>& a, hipStream_t s);
 // per-mesh-block partial sums of R_k^2: partial[k*nblocks + b]
 template <typename T>
 hipError_t launch_res_sq(const T* res, int32_t ncp, int32_t blk, double* partial, hipStream_t s);
+// per-mesh-block compensated sums of V*q_k (k = 0..3) and of V:
+// partial[(2j)*nblocks + b] = hi, partial[(2j+1)*nblocks + b] = lo, j = 0..4
+template <typename T>
+hipError_t launch_cons_sum(const T* q, const DevMesh& m, double* partial, hipStream_t s);
 template <typename T>
 hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s);
 

@@ -43,6 +43,9 @@ struct Instance {
   int32_t bface_off = 0;        // first block-face entry
   bool active = true;
   int64_t steps = 0;            // steps run since the last (re)start
+  int32_t why = 0;              // FVK_RETIRED_*: why the instance is not active
+  double best = 0.0;            // run_to_drop: lowest density residual norm that counted
+  int32_t since_best = 0;       // run_to_drop: checks since `best` was set
 };
 
 struct Engine {
@@ -69,7 +72,8 @@ struct Engine {
   void* sacc = nullptr;                       // accumulate = 1: ncp spectral sums
   unsigned long long* smax = nullptr;         // [2][n], FP64 bits
   double* time = nullptr;                     // [n] physical time of the uniform dt
-  double* partial = nullptr;                  // [4][nblocks]
+  double* partial = nullptr;                  // [10][nblocks]: residual and conservation sums
+  int device = 0;                             // the HIP device the context was created on
   int block = 256;
   int variant = fvk::kVariantLds;
   int smax_cur = 0;
@@ -77,8 +81,11 @@ struct Engine {
   bool sentinel = true;
   bool fs_prims = false;
   bool res_valid = false;
+  int32_t stall_checks = 0;                   // run_to_drop stall retirement (0: off)
+  double stall_factor = 0.5;
   bool use_graph = true;
-  hipGraphExec_t graph = nullptr;             // two steps, captured for `graph_nactive`
+  int graph_steps = 2;                        // steps per captured graph (even)
+  hipGraphExec_t graph = nullptr;             // graph_steps steps, captured for `graph_nactive`
   int32_t graph_nactive = -1;
 };
 
@@ -98,6 +105,10 @@ namespace {
 // a 7.8M-cell mesh and an 11.3M-cell batch of 256 meshes)
 constexpr int64_t kAutoLargeCells = 2000000;    // FP32-mixed: 128-cell blocks above this
 constexpr int32_t kAutoContiguousBlocks = 4096; // contiguous XCD ranges up to this many blocks
+// steps per captured graph by padded cells in the solve: 2 from this size up,
+constexpr int64_t kGraphTwoStepCells = 262144;
+// 8 from this size up, 32 below
+constexpr int64_t kGraphEightStepCells = 16384;
 // FP32-mixed: FP32 LSQ inverses when no cell's normal matrix has a condition
 // number above this (the gradient's relative error then stays below ~6e-6)
 constexpr double kLsq32MaxCond = 100.0;
@@ -415,16 +426,18 @@ void drop_graph(Engine* c) {
   c->graph_nactive = -1;
 }
 
-// Capture two steps (both dt-maximum parities) of a batch into a graph.  The
-// launches read everything that changes from device memory, so the graph
-// stays valid until the active block list changes.
+// Capture graph_steps steps (an even count, so the graph starts and ends on
+// dt-maximum parity 0) into a graph.  The launches read everything that
+// changes from device memory (states, spectral sums, dt maxima, the active
+// block list), so the graph stays valid across restarts and checkpoint loads
+// until the number of active blocks changes.
 int capture_graph(Engine* c) {
   drop_graph(c);
   hipGraph_t g = nullptr;
   FVK_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   auto plain = [](int, auto&& fn) -> int { return fn() == hipSuccess ? FVK_OK : FVK_ERR_HIP; };
-  int rc = run_step(c, false, plain);
-  if (!rc) rc = run_step(c, false, plain);
+  int rc = FVK_OK;
+  for (int k = 0; k < c->graph_steps && !rc; ++k) rc = run_step(c, false, plain);
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
@@ -441,8 +454,9 @@ int capture_graph(Engine* c) {
   return FVK_OK;
 }
 
-// nsteps steps of every active instance.  A batch replays its two-step graph
-// wherever a step is not the first after a start and starts on parity 0.
+// nsteps steps of every active instance.  Steps are replayed from the
+// captured graph wherever a step is not the first after a start, starts on
+// parity 0 and leaves at least one step for the end of the call.
 int engine_step(Engine* c, int32_t nsteps) {
   if (!c->have_state) return set_err(c, FVK_ERR_ARG, "step: no state (initialise it first)");
   if (nsteps < 0) return set_err(c, FVK_ERR_ARG, "step: negative step count");
@@ -450,8 +464,8 @@ int engine_step(Engine* c, int32_t nsteps) {
   int32_t left = nsteps;
   while (left > 0) {
     // the graph's steps keep no residual: the call's last step (eager) does
-    const bool graphable = c->batched && c->use_graph && !c->sentinel && !c->fs_prims &&
-                           c->smax_cur == 0 && left >= 3;
+    const bool graphable = c->use_graph && !c->sentinel && !c->fs_prims && c->smax_cur == 0 &&
+                           left > c->graph_steps;
     if (!graphable) {
       int rc = plain_step(c, left == 1);
       if (rc) return rc;
@@ -464,8 +478,8 @@ int engine_step(Engine* c, int32_t nsteps) {
     }
     FVK_HIP(c, hipGraphLaunch(c->graph, c->stream));
     for (Instance& i : c->in)
-      if (i.active) i.steps += 2;
-    left -= 2;
+      if (i.active) i.steps += c->graph_steps;
+    left -= c->graph_steps;
   }
   FVK_HIP(c, hipStreamSynchronize(c->stream));
   return FVK_OK;
@@ -547,6 +561,18 @@ int put_state(Engine* c, size_t i, const double* q) {
   return FVK_OK;
 }
 
+// Neumaier-compensated sum of n values
+struct Neumaier {
+  double s = 0.0, comp = 0.0;
+  void add(double x) {
+    const double t = s + x;
+    if (std::fabs(s) >= std::fabs(x)) comp += (s - t) + x;
+    else comp += (x - t) + s;
+    s = t;
+  }
+  double value() const { return s + comp; }
+};
+
 // sqrt(sum R_k^2) of every instance's last stored residual: out[4*i + k]
 int residual_l2(Engine* c, double* out) {
   const int32_t nb = c->dm.nblocks;
@@ -562,19 +588,46 @@ int residual_l2(Engine* c, double* out) {
     const Instance& in = c->in[i];
     for (int k = 0; k < 4; ++k) {
       // Neumaier-compensated sum of the instance's block partials
-      double s = 0.0, comp = 0.0;
-      for (int32_t b = in.block_off; b < in.block_off + in.hm.nblocks; ++b) {
-        const double x = part[k * (size_t)nb + b];
-        const double t = s + x;
-        if (std::fabs(s) >= std::fabs(x)) comp += (s - t) + x;
-        else comp += (x - t) + s;
-        s = t;
-      }
-      out[4 * i + k] = std::sqrt(s + comp);
+      Neumaier acc;
+      for (int32_t b = in.block_off; b < in.block_off + in.hm.nblocks; ++b)
+        acc.add(part[k * (size_t)nb + b]);
+      out[4 * i + k] = std::sqrt(acc.value());
       if (!std::isfinite(out[4 * i + k])) finite = false;
     }
   }
   if (!finite) return set_err(c, FVK_ERR_NONFINITE, "residual norm is not finite");
+  return FVK_OK;
+}
+
+// Global conservation sums of every instance's current state:
+// out[4*i + k] = sum over its cells of V * Q_k, compensated (per-block
+// TwoSum pairs on the device, a Neumaier sum of the pairs here).  FP32-mixed:
+// Q_inf,k * sum(V) + sum(V * dQ_k), the product and the final sum in
+// extended precision.
+int conservation_sums(Engine* c, double* out) {
+  if (!c->have_state) return set_err(c, FVK_ERR_ARG, "conservation sums: no state");
+  const int32_t nb = c->dm.nblocks;
+  if (c->fp32)
+    FVK_HIP(c, fvk::launch_cons_sum(static_cast<const float*>(c->q[0]), c->dm, c->partial, c->stream));
+  else
+    FVK_HIP(c, fvk::launch_cons_sum(static_cast<const double*>(c->q[0]), c->dm, c->partial, c->stream));
+  std::vector<double> part(10 * (size_t)nb);
+  FVK_HIP(c, hipMemcpyAsync(part.data(), c->partial, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  FVK_HIP(c, hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < c->in.size(); ++i) {
+    const Instance& in = c->in[i];
+    double sum[5];
+    for (int j = 0; j < 5; ++j) {
+      Neumaier acc;
+      for (int32_t b = in.block_off; b < in.block_off + in.hm.nblocks; ++b) {
+        acc.add(part[(2 * j) * (size_t)nb + b]);
+        acc.add(part[(2 * j + 1) * (size_t)nb + b]);
+      }
+      sum[j] = acc.value();
+    }
+    for (int k = 0; k < 4; ++k)
+      out[4 * i + k] = c->fp32 ? double((long double)in.qinf[k] * sum[4] + (long double)sum[k]) : sum[k];
+  }
   return FVK_OK;
 }
 
@@ -713,7 +766,9 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   c->tsize = c->fp32 ? sizeof(float) : sizeof(double);
   c->block = o0.block_size ? o0.block_size : 256;   // thread block of the cell-parallel shape
   c->variant = o0.kernel_variant ? o0.kernel_variant : fvk::kVariantLds;
-  hipError_t e = hipStreamCreate(&c->stream);
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) return hip_err(nullptr, e, "hipGetDevice");
+  e = hipStreamCreate(&c->stream);
   if (e != hipSuccess) {
     c->stream = nullptr;
     return hip_err(nullptr, e, "hipStreamCreate");
@@ -733,6 +788,11 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
     d.max_bfaces = std::max(d.max_bfaces, in.hm.max_bfaces);
     d.max_deg = std::max(d.max_deg, in.hm.max_deg);
   }
+  // Steps per captured graph: a graph launch costs the host a few
+  // microseconds, which a step of a small mesh (a few blocks per kernel, a few
+  // microseconds of device time) does not hide, so small solves capture more
+  // steps per graph.
+  c->graph_steps = d.ncp >= kGraphTwoStepCells ? 2 : d.ncp >= kGraphEightStepCells ? 8 : 32;
   // XCD order: contiguous ranges while a launch is a few blocks per CU (every
   // block is resident at once), runs of 8 blocks dealt in turn beyond that
   // (ranges of unequal work would leave XCDs idle at the end)
@@ -845,7 +905,7 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   if ((rc = dev_alloc(c, &c->smax, 2 * c->in.size()))) return rc;
   if ((rc = dev_alloc(c, &c->time, c->in.size()))) return rc;
   if ((rc = fill(c, c->time, (int64_t)c->in.size(), 0.0))) return rc;
-  if ((rc = dev_alloc(c, &c->partial, 4 * (size_t)d.nblocks))) return rc;
+  if ((rc = dev_alloc(c, &c->partial, 10 * (size_t)d.nblocks))) return rc;
   for (int i = 0; i < 3; ++i) {
     if ((rc = fill(c, c->q[i], 4 * (int64_t)ncp, 0.0))) return rc;
     if ((rc = fill(c, c->g[i], 8 * (int64_t)ncp, 0.0))) return rc;
@@ -866,12 +926,37 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
 // checkpoints: the complete device state, so that a run resumes exactly
 // ---------------------------------------------------------------------------
 struct CheckpointHeader {
-  char magic[8];               // "FVKCKPT1"
+  char magic[8];               // "FVKCKPT2"
   int32_t precision, instances;
   int64_t ncp, nblocks, bytes;
   int32_t flags[6];            // have_state, sentinel, fs_prims, res_valid, smax_cur, pad
   int64_t steps_sum;
+  uint64_t flow_hash;          // the instances' flow options (options_hash)
 };
+
+// FNV-1a hash of every instance's options that the stored state depends on:
+// the freestream (the FP32-mixed state is stored relative to it), the gas
+// constants, CFL and the scheme switches.  A checkpoint is refused by a
+// context whose hash differs.
+uint64_t options_hash(const Engine* c) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&h](const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) {
+      h ^= b[i];
+      h *= 1099511628211ull;
+    }
+  };
+  for (const Instance& in : c->in) {
+    const fvk_options& o = in.opt;
+    const double d[10] = {in.qinf[0], in.qinf[1], in.qinf[2], in.qinf[3], o.cfl, o.gamma,
+                          o.R, o.mu, o.Pr, o.P};
+    const int32_t k[7] = {o.order, o.limiter, o.eqn, o.time_step_type, o.bc1, o.bc2, o.precision};
+    mix(d, sizeof(d));
+    mix(k, sizeof(k));
+  }
+  return h;
+}
 
 // the device arrays a checkpoint holds, with their byte sizes
 std::vector<std::pair<void*, size_t>> checkpoint_arrays(Engine* c) {
@@ -898,7 +983,7 @@ int checkpoint_save(Engine* c, void* buf, int64_t bytes) {
   if (!buf || bytes < need) return set_err(c, FVK_ERR_ARG, "checkpoint: buffer smaller than the checkpoint size");
   FVK_HIP(c, hipStreamSynchronize(c->stream));
   CheckpointHeader h{};
-  std::memcpy(h.magic, "FVKCKPT1", 8);
+  std::memcpy(h.magic, "FVKCKPT2", 8);
   h.precision = c->fp32 ? FVK_FP32_MIXED : FVK_FP64;
   h.instances = (int32_t)c->in.size();
   h.ncp = c->dm.ncp;
@@ -909,6 +994,7 @@ int checkpoint_save(Engine* c, void* buf, int64_t bytes) {
   h.flags[2] = c->fs_prims;
   h.flags[3] = c->res_valid;
   h.flags[4] = c->smax_cur;
+  h.flow_hash = options_hash(c);
   unsigned char* p = static_cast<unsigned char*>(buf);
   std::memcpy(p, &h, sizeof(h));
   p += sizeof(h);
@@ -928,11 +1014,13 @@ int checkpoint_load(Engine* c, const void* buf, int64_t bytes) {
   if (!buf || bytes < (int64_t)sizeof(CheckpointHeader)) return set_err(c, FVK_ERR_ARG, "checkpoint: buffer too small");
   CheckpointHeader h;
   std::memcpy(&h, buf, sizeof(h));
-  if (std::memcmp(h.magic, "FVKCKPT1", 8) != 0) return set_err(c, FVK_ERR_ARG, "checkpoint: not a checkpoint");
+  if (std::memcmp(h.magic, "FVKCKPT2", 8) != 0) return set_err(c, FVK_ERR_ARG, "checkpoint: not a checkpoint");
   if (h.precision != (c->fp32 ? FVK_FP32_MIXED : FVK_FP64) || h.instances != (int32_t)c->in.size() ||
       h.ncp != c->dm.ncp || h.nblocks != c->dm.nblocks || h.bytes != checkpoint_bytes(c) || bytes < h.bytes)
     return set_err(c, FVK_ERR_ARG, "checkpoint: written for another mesh layout, instance count or precision");
-  drop_graph(c);
+  if (h.flow_hash != options_hash(c))
+    return set_err(c, FVK_ERR_ARG, "checkpoint: written under other flow options (freestream, gas "
+                                   "constants, CFL or scheme switches)");
   const unsigned char* p = static_cast<const unsigned char*>(buf) + sizeof(h);
   for (auto& a : checkpoint_arrays(c)) {
     FVK_HIP(c, hipMemcpy(a.first, p, a.second, hipMemcpyHostToDevice));
@@ -942,6 +1030,7 @@ int checkpoint_load(Engine* c, const void* buf, int64_t bytes) {
   for (Instance& in : c->in) {
     std::memcpy(&in.steps, p, 8);
     p += 8;
+    in.why = in.active ? FVK_RETIRED_NONE : FVK_RETIRED_CALLER;
   }
   c->have_state = h.flags[0] != 0;
   c->sentinel = h.flags[1] != 0;
@@ -1048,6 +1137,18 @@ int fvk_get_residual(fvk_ctx* c, double* r) {
 int fvk_residual_l2(fvk_ctx* c, double out[4]) {
   if (!c || !out) return FVK_ERR_ARG;
   return residual_l2(&c->e, out);
+}
+
+int fvk_conservation_sums(fvk_ctx* c, double out[4]) {
+  if (!c || !out) return FVK_ERR_ARG;
+  return conservation_sums(&c->e, out);
+}
+
+int fvk_use_graph(fvk_ctx* c, int32_t on) {
+  if (!c) return FVK_ERR_ARG;
+  c->e.use_graph = on != 0;
+  if (!on) drop_graph(&c->e);
+  return FVK_OK;
 }
 
 int fvk_get_geometry(fvk_ctx* c, const char* name, double* out) {
@@ -1218,11 +1319,11 @@ int fvk_batch_init(fvk_batch* b, const double* const* q) {
     int rc = put_state(c, i, given ? q[i] : nullptr);
     if (rc) return rc;
     c->in[i].active = true;
+    c->in[i].why = FVK_RETIRED_NONE;
   }
   int rc = upload_active(c);
   if (!rc) rc = arm_start(c);
   c->fs_prims = given == 0;
-  drop_graph(c);
   return rc;
 }
 
@@ -1234,8 +1335,26 @@ int fvk_batch_step(fvk_batch* b, int32_t nsteps) {
 int fvk_batch_set_active(fvk_batch* b, const uint8_t* active) {
   if (!b || !active) return FVK_ERR_ARG;
   Engine* c = &b->e;
-  for (size_t i = 0; i < c->in.size(); ++i) c->in[i].active = active[i] != 0;
+  for (size_t i = 0; i < c->in.size(); ++i) {
+    c->in[i].active = active[i] != 0;
+    c->in[i].why = c->in[i].active ? FVK_RETIRED_NONE : FVK_RETIRED_CALLER;
+  }
   return upload_active(c);
+}
+
+int fvk_batch_set_stall(fvk_batch* b, int32_t checks, double factor) {
+  if (!b) return FVK_ERR_ARG;
+  if (checks < 0 || !(factor > 0.0 && factor < 1.0))
+    return set_err(&b->e, FVK_ERR_ARG, "fvk_batch_set_stall: checks >= 0 and 0 < factor < 1");
+  b->e.stall_checks = checks;
+  b->e.stall_factor = factor;
+  return FVK_OK;
+}
+
+int fvk_batch_retired(fvk_batch* b, int32_t* why) {
+  if (!b || !why) return FVK_ERR_ARG;
+  for (size_t i = 0; i < b->e.in.size(); ++i) why[i] = b->e.in[i].why;
+  return FVK_OK;
 }
 
 int fvk_batch_residual_l2(fvk_batch* b, double* out) {
@@ -1252,7 +1371,10 @@ int fvk_batch_run_to_drop(fvk_batch* b, double drop, int32_t max_steps, int32_t 
   const size_t n = c->in.size();
   std::vector<double> r(4 * n), r0(n, -1.0);
   std::vector<int64_t> start(n);
-  for (size_t i = 0; i < n; ++i) start[i] = c->in[i].steps;
+  for (size_t i = 0; i < n; ++i) {
+    start[i] = c->in[i].steps;
+    c->in[i].since_best = 0;
+  }
   int32_t done = 0;
   bool any = c->nactive > 0;
   while (any && done < max_steps) {
@@ -1269,11 +1391,30 @@ int fvk_batch_run_to_drop(fvk_batch* b, double drop, int32_t max_steps, int32_t 
       Instance& in = c->in[i];
       if (!in.active) continue;
       const double rho_res = r[4 * i];
-      if (r0[i] < 0) r0[i] = rho_res;
+      if (r0[i] < 0) {
+        r0[i] = rho_res;
+        in.best = rho_res;
+      }
       if (res_end) res_end[i] = rho_res;
-      // retire on convergence, and on a non-finite residual (state frozen)
-      if (!(rho_res > drop * r0[i]) || !std::isfinite(rho_res)) {
+      // retire on a non-finite residual (state frozen), on convergence and,
+      // when enabled, on a stall: stall_checks checks in a row without the
+      // norm falling below stall_factor times the lowest norm that counted
+      int why = FVK_RETIRED_NONE;
+      if (!std::isfinite(rho_res)) {
+        why = FVK_RETIRED_NONFINITE;
+      } else if (!(rho_res > drop * r0[i])) {
+        why = FVK_RETIRED_DROP;
+      } else if (c->stall_checks > 0) {
+        if (rho_res < c->stall_factor * in.best) {
+          in.best = rho_res;
+          in.since_best = 0;
+        } else if (++in.since_best >= c->stall_checks) {
+          why = FVK_RETIRED_STALL;
+        }
+      }
+      if (why != FVK_RETIRED_NONE) {
         in.active = false;
+        in.why = why;
         changed = true;
       } else {
         any = true;
@@ -1286,6 +1427,11 @@ int fvk_batch_run_to_drop(fvk_batch* b, double drop, int32_t max_steps, int32_t 
     if (res0) res0[i] = r0[i];
   }
   return FVK_OK;
+}
+
+int fvk_batch_conservation_sums(fvk_batch* b, double* out) {
+  if (!b || !out) return FVK_ERR_ARG;
+  return conservation_sums(&b->e, out);
 }
 
 int fvk_batch_get_state(fvk_batch* b, int32_t i, double* q, double* grad) {

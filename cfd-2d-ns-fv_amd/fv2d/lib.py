@@ -22,14 +22,18 @@ BC_CODES = {"FREESTREAM": -1, "SLIP_WALL": -2, "EXTRAPOLATION": -3, "NO_SLIP_WAL
 # precision forms (FVK_FP64, FVK_FP32_MIXED)
 FP64 = 0
 FP32_MIXED = 1
+# why a batch instance is not active (FVK_RETIRED_*)
+RETIRED_NONE, RETIRED_DROP, RETIRED_NONFINITE, RETIRED_STALL, RETIRED_CALLER = range(5)
 
 # every symbol include/fvk.h declares
 EXPORTS = ["fvk_default_options", "fvk_create", "fvk_destroy", "fvk_last_error",
            "fvk_init_freestream", "fvk_set_state", "fvk_step", "fvk_get_state",
-           "fvk_get_residual", "fvk_residual_l2", "fvk_get_geometry", "fvk_geometry",
+           "fvk_use_graph", "fvk_get_residual", "fvk_residual_l2", "fvk_conservation_sums",
+           "fvk_get_geometry", "fvk_geometry",
            "fvk_get_info", "fvk_get_time", "fvk_get_options", "fvk_time_steps", "fvk_time_kernels",
            "fvk_batch_create", "fvk_batch_destroy", "fvk_batch_last_error", "fvk_batch_init",
            "fvk_batch_step", "fvk_batch_set_active", "fvk_batch_residual_l2",
+           "fvk_batch_conservation_sums", "fvk_batch_set_stall", "fvk_batch_retired",
            "fvk_batch_run_to_drop", "fvk_batch_get_state", "fvk_batch_get_info",
            "fvk_batch_get_time", "fvk_batch_get_options", "fvk_batch_use_graph",
            "fvk_batch_time_kernels",
@@ -90,6 +94,8 @@ def lib():
         L.fvk_get_state.argtypes = [vp, dp, dp]
         L.fvk_get_residual.argtypes = [vp, dp]
         L.fvk_residual_l2.argtypes = [vp, dp]
+        L.fvk_use_graph.argtypes = [vp, ctypes.c_int32]
+        L.fvk_conservation_sums.argtypes = [vp, dp]
         L.fvk_get_geometry.argtypes = [vp, ctypes.c_char_p, dp]
         L.fvk_geometry.argtypes = [P(FvkMesh), P(FvkOptions), ctypes.c_char_p, dp]
         L.fvk_get_info.argtypes = [vp, P(ctypes.c_int64)]
@@ -106,6 +112,9 @@ def lib():
         L.fvk_batch_step.argtypes = [vp, ctypes.c_int32]
         L.fvk_batch_set_active.argtypes = [vp, P(ctypes.c_uint8)]
         L.fvk_batch_residual_l2.argtypes = [vp, dp]
+        L.fvk_batch_conservation_sums.argtypes = [vp, dp]
+        L.fvk_batch_set_stall.argtypes = [vp, ctypes.c_int32, ctypes.c_double]
+        L.fvk_batch_retired.argtypes = [vp, ip]
         L.fvk_batch_run_to_drop.argtypes = [vp, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                                             ip, dp, dp]
         L.fvk_batch_get_state.argtypes = [vp, ctypes.c_int32, dp, dp]
@@ -236,6 +245,16 @@ class Solver:
             self._check(rc)
         return out
 
+    def conservation_sums(self) -> np.ndarray:
+        """sum_i V_i Q_k,i of the current state (compensated), k = 1..4."""
+        out = np.empty(4)
+        self._check(lib().fvk_conservation_sums(self.h, _dp(out)))
+        return out
+
+    def use_graph(self, on: bool):
+        """Replay the captured step graph (default) or launch step by step."""
+        self._check(lib().fvk_use_graph(self.h, 1 if on else 0))
+
     def geometry(self, name: str) -> np.ndarray:
         n = self.nc if name in ("cell_vol", "cell_centerx", "cell_centery", "Ixx", "Iyy", "Ixy") \
             else self.nf
@@ -335,7 +354,9 @@ class Batch:
         self._check(lib().fvk_batch_step(self.h, n))
 
     def set_active(self, flags):
-        a = np.ascontiguousarray(np.asarray(flags, dtype=np.uint8))
+        a = np.ascontiguousarray(np.asarray(flags, dtype=np.uint8)).reshape(-1)
+        if a.size != self.n:
+            raise ValueError(f"set_active: {a.size} flags for {self.n} instances")
         self._check(lib().fvk_batch_set_active(
             self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
 
@@ -344,6 +365,23 @@ class Batch:
         rc = lib().fvk_batch_residual_l2(self.h, _dp(out))
         if rc not in (FVK_OK, -5):
             self._check(rc)
+        return out.reshape(self.n, 4)
+
+    def set_stall(self, checks: int, factor: float = 0.5):
+        """Stall retirement in run_to_drop (checks = 0: off)."""
+        self._check(lib().fvk_batch_set_stall(self.h, checks, factor))
+
+    def retired(self) -> np.ndarray:
+        """Per instance: RETIRED_* code (0 active, 1 drop, 2 non-finite,
+        3 stall, 4 by the caller)."""
+        why = np.zeros(self.n, dtype=np.int32)
+        self._check(lib().fvk_batch_retired(self.h, _ip(why)))
+        return why
+
+    def conservation_sums(self) -> np.ndarray:
+        """Per instance: sum_i V_i Q_k,i of the current state, shape (n, 4)."""
+        out = np.empty(4 * self.n)
+        self._check(lib().fvk_batch_conservation_sums(self.h, _dp(out)))
         return out.reshape(self.n, 4)
 
     def run_to_drop(self, drop: float, max_steps: int, check_every: int):

@@ -118,8 +118,17 @@ int fvk_init_freestream(fvk_ctx* ctx);
 int fvk_set_state(fvk_ctx* ctx, const double* q);
 
 /* nsteps calls of ssp_rk3 (ssp_rk3.cpp:18-77).  The stage-3 residual of the
- * last step is kept for fvk_get_residual / fvk_residual_l2. */
+ * last step is kept for fvk_get_residual / fvk_residual_l2.  Steps that are
+ * not the first after a start, begin on an even step count and leave a step
+ * for the end of the call are replayed from a captured HIP graph of G steps
+ * (G = 2 from 262,144 padded cells up, 8 from 16,384, else 32); the first
+ * call long enough to use it captures and instantiates it, later calls (also
+ * after fvk_init_freestream, fvk_set_state or a checkpoint load) replay it. */
 int fvk_step(fvk_ctx* ctx, int32_t nsteps);
+/* 1 (default): replay the step graph as above; 0: launch every step's
+ * kernels one by one (the graph is released).  Both give the same bits.
+ * No counterpart in the reference. */
+int fvk_use_graph(fvk_ctx* ctx, int32_t on);
 
 /* Conserved state q[4*N_c]; grad[8*N_c] (nullable) = the stage-1 limited
  * gradients Qxp1..4, Qyp1..4 that Qbar holds after ssp_rk3 and that
@@ -131,6 +140,13 @@ int fvk_get_residual(fvk_ctx* ctx, double* r);
 /* sqrt(sum_i R_k,i^2), k = 1..4, of that residual (save_sol.cpp:79-89);
  * FVK_ERR_NONFINITE if any is not finite (out[] is still written). */
 int fvk_residual_l2(fvk_ctx* ctx, double out[4]);
+
+/* Global conservation sums of the current state: out[k] = sum_i V_i Q_k,i,
+ * k = 1..4 (mass, x and y momentum, energy), with compensated summation
+ * (per-block TwoSum pairs on the device, a Neumaier sum of the pairs on the
+ * host; the FP32-mixed form adds Q_inf,k * sum(V) to sum(V dQ_k)).  No
+ * counterpart in the reference, which checks no conservation. */
+int fvk_conservation_sums(fvk_ctx* ctx, double out[4]);
 
 /* Geometry arrays of class_mesh, original numbering, for checks:
  * "face_area", "face_nx", "face_ny", "face_centerx", "face_centery" (N_f);
@@ -189,8 +205,10 @@ int fvk_time_kernels(fvk_ctx* ctx, int32_t reps, double avg_us[2]);
  * process; each instance's results are those of a single fvk_ctx run of the
  * same mesh and options, bit for bit.  Instances must agree on order, eqn,
  * time_step_type, precision, block_size, kernel_variant (0 or 3) and
- * xcd_remap.  Steps after the first two of a start replay a captured
- * two-step HIP graph (fvk_batch_use_graph).  Every call is synchronous.
+ * xcd_remap.  Steps are replayed from a captured HIP graph as fvk_step
+ * describes, counting the cells of all instances; the graph is captured
+ * again when the number of active blocks changes (fvk_batch_use_graph).
+ * Every call is synchronous.
  * ------------------------------------------------------------------------ */
 typedef struct fvk_batch fvk_batch;
 
@@ -210,14 +228,33 @@ int fvk_batch_step(fvk_batch* b, int32_t nsteps);
 int fvk_batch_set_active(fvk_batch* b, const uint8_t* active);
 /* out[4*i + k] = sqrt(sum R_k^2) of instance i's last stage-3 residual */
 int fvk_batch_residual_l2(fvk_batch* b, double* out);
+/* out[4*i + k] = instance i's global conservation sums, as fvk_conservation_sums */
+int fvk_batch_conservation_sums(fvk_batch* b, double* out);
 /* Step the active instances, checking every check_every steps (and after
  * the first step, whose density residual norm is instance i's reference
  * res0[i]), and retire each instance whose density residual norm has fallen
- * to drop * res0[i] or is not finite; stop when none is active or after
- * max_steps steps.  steps[i] = steps instance i ran; res_end[i] = its last
- * checked density residual norm (res0, res_end nullable). */
+ * to drop * res0[i] or is not finite, or (fvk_batch_set_stall) has stalled;
+ * stop when none is active or after max_steps steps.  steps[i] = steps
+ * instance i ran; res_end[i] = its last checked density residual norm (res0,
+ * res_end nullable). */
 int fvk_batch_run_to_drop(fvk_batch* b, double drop, int32_t max_steps, int32_t check_every,
                           int32_t* steps, double* res0, double* res_end);
+/* Stall retirement in fvk_batch_run_to_drop (off by default, checks = 0): an
+ * instance also retires after `checks` checks in a row in which its density
+ * residual norm has not fallen below factor (0 < factor < 1) times the lowest
+ * norm that counted (res0, then each norm that fell below factor times the
+ * previous one).  A run stalls where its residual stops falling: the SQUEEZE
+ * limiter's cycling, or the FP32-mixed state's resolution (a converged FP32
+ * perturbation state carries a residual of about its rounding).  No
+ * counterpart in the reference, which runs a fixed nmax. */
+int fvk_batch_set_stall(fvk_batch* b, int32_t checks, double factor);
+/* why[i] = why instance i is not active: one of FVK_RETIRED_* */
+#define FVK_RETIRED_NONE 0       /* active */
+#define FVK_RETIRED_DROP 1       /* residual norm fell to the drop */
+#define FVK_RETIRED_NONFINITE 2  /* residual norm not finite */
+#define FVK_RETIRED_STALL 3      /* residual norm stalled (fvk_batch_set_stall) */
+#define FVK_RETIRED_CALLER 4     /* fvk_batch_set_active */
+int fvk_batch_retired(fvk_batch* b, int32_t* why);
 /* instance i's state and stage-1 limited gradients, as fvk_get_state */
 int fvk_batch_get_state(fvk_batch* b, int32_t i, double* q, double* grad);
 /* [instances, cells, faces, mesh blocks, active blocks, active instances] */
@@ -228,7 +265,10 @@ int fvk_batch_get_time(fvk_batch* b, double* t);
 int fvk_batch_get_options(fvk_batch* b, int32_t i, fvk_options* out);
 /* 0: launch every step's kernels one by one; 1 (default): replay the graph */
 int fvk_batch_use_graph(fvk_batch* b, int32_t on);
-/* checkpoints of a whole batch (with the active set and step counters) */
+/* checkpoints of a whole batch (with the active set and step counters); a
+ * checkpoint records a hash of every instance's flow options (freestream, gas
+ * constants, CFL, scheme switches) and is refused by a batch, or context,
+ * whose options differ */
 int fvk_batch_checkpoint_size(fvk_batch* b, int64_t* bytes);
 int fvk_batch_checkpoint_save(fvk_batch* b, void* buf, int64_t bytes);
 int fvk_batch_checkpoint_load(fvk_batch* b, const void* buf, int64_t bytes);

@@ -5,12 +5,18 @@ Run in the development container (where the reference tree exists):
     make -C oracle ref
     python tests/golden/make_golden.py
 
+    python tests/golden/make_golden.py NAME...   (only these fixtures)
+
 Each case is a small canonical grid from fv2d.gridgen plus an input.in; the
 reference harness (oracle/_ref/ref_harness, the unmodified reference solver
 with a full-precision dump in place of its 6-digit writers) runs it and the
-dumped arrays are stored with the grid in tests/golden/<case>.npz.  The
-reference program itself (oracle/_ref/fv2d_ref) is also run on one case and
-its output files are kept under tests/golden/cli_bump/ for the CLI test.
+dumped arrays are stored with the grid in tests/golden/<case>.npz.  One case,
+reference_input_grid_file, is the reference's own shipped case: its
+input.in options on its grids/grid_file.in (data read from the reference
+tree), 100 steps.  The reference program itself (oracle/_ref/fv2d_ref) is
+also run on a bump (Euler) and an O-grid (NS, no-slip wall) case and its
+output files are kept under tests/golden/cli_bump/ and cli_ogrid_ns/ for
+the CLI test.
 """
 from __future__ import annotations
 
@@ -78,13 +84,34 @@ def cases():
     out.append(dict(name="bump_restart_state", grid=("bump", dict(nx=16, ny=8)),
                     case=dict(u=u05, v=v05, poly=1, limiter="SQUEEZE", dt_type="GLOBAL", CFL=0.5),
                     steps=10, dumps=[1, 10], state=True))
+    # the reference's own case: input.in (NS, poly 1, no limiter, LOCAL CFL 1,
+    # P 101325, u 170.147, mu 0.04168601, NO_SLIP_WALL / FREESTREAM) on
+    # grids/grid_file.in (8196 cells, pentagons and hexagons among them); the
+    # fixture keeps the state, gradients and residual (not every face array)
+    out.append(dict(name="reference_input_grid_file", grid=("file", dict(path=REF_GRID)),
+                    case=dict(u=170.147, v=0.0, P=101325.0, rho=1.225, gamma=1.4, R=287.0,
+                              mu=0.04168601, Pr=0.72, poly=1, limiter="NONE", eqn="NS",
+                              BC1="NO_SLIP_WALL", BC2="FREESTREAM", dt_type="LOCAL", CFL=1.0),
+                    steps=100, dumps=[1, 2, 100], keep=KEEP_LEAN))
     out.append(dict(name="ramp_res_only", grid=("ramp", dict(nx=12, ny=6)),
                     case=dict(u=u25, v=v25, poly=1, limiter="SQUEEZE", dt_type="LOCAL", CFL=0.8),
                     steps=0, dumps=[], state=True, res_only=True))
     return out
 
 
+REF_GRID = "/root/reference/grids/grid_file.in"
+# arrays a lean fixture keeps: the cell and face geometry the host test checks,
+# and per dumped step the state, limited gradients, residual and face wave speeds
+KEEP_LEAN = ("geom/face_area", "geom/face_nx", "geom/face_ny", "geom/face_centerx",
+             "geom/face_centery", "geom/cell_vol", "geom/cell_centerx", "geom/cell_centery",
+             "geom/Ixx", "geom/Iyy", "geom/Ixy", "geom/face_cell2",
+             "p1", "p2", "p3", "p4", "Qxp1", "Qxp2", "Qxp3", "Qxp4", "Qyp1", "Qyp2", "Qyp3",
+             "Qyp4", "R1", "R2", "R3", "R4", "Vn_avg", "c_avg")
+
+
 def make_grid(kind, kw):
+    if kind == "file":
+        return gg.read_grid(kw["path"])
     return getattr(gg, kind)(**kw)
 
 
@@ -117,6 +144,9 @@ def run_case(c, workdir):
     O.run_reference(os.path.join(workdir, "input.in"), out, c["steps"], c["dumps"],
                     state_path=state_path, res_only=c.get("res_only", False))
     dump = O.read_dump(out)
+    if c.get("keep"):
+        dump = {k: v for k, v in dump.items()
+                if k in c["keep"] or k.split("/", 1)[-1] in c["keep"]}
     arrays = {"grid/x": grid.x, "grid/y": grid.y, "grid/face_p1": grid.face_p1,
               "grid/face_p2": grid.face_p2, "grid/face_c1": grid.face_c1,
               "grid/face_c2": grid.face_c2, "grid/bc_marker": grid.bc_marker,
@@ -131,19 +161,34 @@ def run_case(c, workdir):
     return grid
 
 
-def cli_fixture():
-    """Outputs of the reference program on a small bump case."""
+def cli_cases():
+    u05, v05 = mach(0.5)
+    u02, v02 = mach(0.2)
+    return {
+        # Euler, slip walls
+        "cli_bump": (gg.bump(16, 8), dict(u=u05, v=v05, poly=1, limiter="SQUEEZE",
+                                          dt_type="LOCAL", CFL=0.8, nmax=60, monitor_step=10,
+                                          output_step=30)),
+        # NS with a no-slip wall at second order: the wall-stress branches of the
+        # monitor and the surface file (save_sol.cpp:46-62,164-177)
+        "cli_ogrid_ns": (gg.ogrid(32, 16), dict(u=u02, v=v02, poly=1, limiter="SQUEEZE",
+                                                eqn="NS", mu=1.225 * u02 / 100.0,
+                                                BC1="NO_SLIP_WALL", BC2="FREESTREAM",
+                                                dt_type="GLOBAL", CFL=0.5, nmax=40,
+                                                monitor_step=10, output_step=20)),
+    }
+
+
+def cli_fixture(name):
+    """Outputs of the reference program on one of cli_cases()."""
+    grid, kw = cli_cases()[name]
     d = tempfile.mkdtemp()
     try:
-        grid = gg.bump(16, 8)
-        u, v = mach(0.5)
         grid.write(os.path.join(d, "grid.in"))
-        inputs.Case(grid_file="grid.in", u=u, v=v, poly=1, limiter="SQUEEZE", dt_type="LOCAL",
-                    CFL=0.8, nmax=60, monitor_step=10, output_step=30).write(
-            os.path.join(d, "input.in"))
+        inputs.Case(grid_file="grid.in", **kw).write(os.path.join(d, "input.in"))
         r = subprocess.run([os.path.join(ROOT, "oracle", "_ref", "fv2d_ref"), "input.in"], cwd=d,
                            capture_output=True, text=True, check=True)
-        dst = os.path.join(HERE, "cli_bump")
+        dst = os.path.join(HERE, name)
         if os.path.isdir(dst):
             shutil.rmtree(dst)
         os.makedirs(dst)
@@ -161,15 +206,21 @@ def cli_fixture():
 def main():
     if not os.path.exists(O.REF_HARNESS):
         raise SystemExit("build the reference first: make -C oracle ref")
+    only = set(sys.argv[1:])
     for c in cases():
+        if only and c["name"] not in only:
+            continue
         d = tempfile.mkdtemp()
         try:
             run_case(c, d)
             print("wrote", c["name"])
         finally:
             shutil.rmtree(d)
-    cli_fixture()
-    print("wrote cli_bump")
+    for name in cli_cases():
+        if only and name not in only:
+            continue
+        cli_fixture(name)
+        print("wrote", name)
 
 
 if __name__ == "__main__":

@@ -185,3 +185,67 @@ def test_batch_checkpoint_resumes_exactly():
     b.step(8)
     for i in range(len(cases)):
         assert same(a.state(i), b.state(i)), i
+
+
+def test_batch_conservation_sums_equal_single():
+    cases = euler_cases("LOCAL")
+    b = Batch(cases)
+    b.init()
+    b.step(9)
+    sums = b.conservation_sums()
+    for i, (g, c) in enumerate(cases):
+        s = single(g, c, 9)
+        assert np.array_equal(sums[i], s.conservation_sums()), i
+
+
+def test_batch_set_active_checks_length():
+    cases = euler_cases("LOCAL")
+    b = Batch(cases)
+    with pytest.raises(ValueError):
+        b.set_active(np.ones(len(cases) - 1, dtype=np.uint8))
+
+
+def test_batch_graph_survives_restart():
+    """fvk_batch_init keeps the captured graph (it reads everything from
+    device memory); a restarted batch replays it and gives the results of a
+    fresh batch."""
+    cases = euler_cases("GLOBAL")
+    b = Batch(cases)
+    b.init()
+    b.step(9)
+    b.init()
+    b.step(13)
+    f = Batch(cases)
+    f.init()
+    f.step(13)
+    for i in range(len(cases)):
+        assert same(b.state(i), f.state(i)), i
+
+
+def test_batch_stall_retirement_and_error_at_convergence():
+    """First-order channel and ramp instances run to an unreachable drop with
+    stall retirement on: both precisions retire every instance by stall (FP64
+    near 1e-14 of the first-step residual, FP32-mixed where its state's
+    rounding stops the residual), and the converged FP32-mixed density is
+    within 1e-5 (relative L2) of the converged FP64 density."""
+    from fv2d.lib import RETIRED_STALL
+    u, v = inputs.mach_velocity(0.5, angle_deg=1.0)
+    u2, v2 = inputs.mach_velocity(2.2, angle_deg=2.0)
+    cases = [(gg.bump(64, 32), inputs.Case(u=u, v=v, poly=0, dt_type="LOCAL", CFL=0.5)),
+             (gg.ramp(48, 24), inputs.Case(u=u2, v=v2, poly=0, dt_type="LOCAL", CFL=0.5))]
+    final = {}
+    for prec in (0, FP32_MIXED):
+        b = Batch(cases, precision=prec)
+        b.set_stall(5, 0.5)
+        b.init()
+        steps, r0, r1 = b.run_to_drop(1e-30, 40000, 200)
+        why = b.retired()
+        print(f"[gpu] stall precision={prec}: steps {steps.tolist()}, drop {(r1 / r0).tolist()}",
+              flush=True)
+        assert (why == RETIRED_STALL).all(), why
+        assert (steps < 40000).all(), steps
+        final[prec] = [b.state(i) for i in range(len(cases))]
+    for i in range(len(cases)):
+        q32, q64 = final[FP32_MIXED][i], final[0][i]
+        err = float(np.linalg.norm(q32[0] - q64[0]) / np.linalg.norm(q64[0]))
+        assert err <= 1e-5, (i, err)

@@ -1,9 +1,11 @@
 """The drop-in CLI (cfd-2d-ns-fv_amd/lib/fv2d) against the reference program.
 
-tests/golden/cli_bump holds the input.in, the grid and every file the
-reference program wrote for it (sol/*.dat, mesh/*.dat) and its stdout.  The
-CLI runs the same case on the GPU; every output file must match line for
-line (same 6-digit formatting, same bits underneath).
+tests/golden/cli_bump (Euler, slip walls) and tests/golden/cli_ogrid_ns (NS,
+second order, a no-slip wall: the wall-stress terms of the monitor and the
+surface file) hold the input.in, the grid and every file the reference
+program wrote for them (sol/*.dat, mesh/*.dat) and its stdout.  The CLI runs
+the same case on the GPU; every output file must match line for line (same
+6-digit formatting, same bits underneath).
 """
 import os
 import shutil
@@ -19,16 +21,16 @@ pytestmark = pytest.mark.gpu
 CASE = os.path.join(GOLDEN, "cli_bump")
 
 
-def _run(tmp_path, extra_input=""):
+def _run(tmp_path, extra_input="", case=CASE, expect_rc=0):
     for fn in ("grid.in", "input.in"):
-        shutil.copy(os.path.join(CASE, fn), tmp_path / fn)
+        shutil.copy(os.path.join(case, fn), tmp_path / fn)
     if extra_input:
         with open(tmp_path / "input.in", "a") as f:
             f.write(extra_input)
     print(f"[gpu] fv2d CLI in {tmp_path}", flush=True)
     r = subprocess.run([CLI_PATH, "input.in"], cwd=tmp_path, capture_output=True, text=True,
                        timeout=120)
-    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.returncode == expect_rc, r.stdout + r.stderr
     return r.stdout
 
 
@@ -38,22 +40,22 @@ def _drop_last_two_fields(txt):
     return "\n".join(lines)
 
 
-def _expected_files():
-    return sorted(fn for fn in os.listdir(CASE) if "__" in fn)
+def _expected_files(case=CASE):
+    return sorted(fn for fn in os.listdir(case) if "__" in fn)
 
 
-def _check_outputs(tmp_path, out):
-    for fn in _expected_files():
+def _check_outputs(tmp_path, out, case=CASE):
+    for fn in _expected_files(case):
         sub, name = fn.split("__", 1)
         got = (tmp_path / sub / name).read_text()
-        want = open(os.path.join(CASE, fn)).read()
+        want = open(os.path.join(case, fn)).read()
         if name.startswith("tec_surf"):
             # the reference's last surface line reads one past its wall-stress
             # arrays (save_sol.cpp:164-165,227-229): those two fields are
             # whatever memory held, so they are left out of the comparison
             got, want = _drop_last_two_fields(got), _drop_last_two_fields(want)
         assert got == want, f"{sub}/{name} differs"
-    ref_iters = [l for l in open(os.path.join(CASE, "stdout.txt")).read().splitlines()
+    ref_iters = [l for l in open(os.path.join(case, "stdout.txt")).read().splitlines()
                  if l.startswith("iter:")]
     got_iters = [l for l in out.splitlines() if l.startswith("iter:")]
     assert got_iters == ref_iters
@@ -62,6 +64,27 @@ def _check_outputs(tmp_path, out):
 @pytest.mark.parametrize("extra", ["", "\n[gpu]\n    kernel_variant = 1\n    renumber = 0\n"])
 def test_cli_outputs_match_reference_program(tmp_path, extra):
     _check_outputs(tmp_path, _run(tmp_path, extra))
+
+
+@pytest.mark.parametrize("extra", ["", "\n[gpu]\n    kernel_variant = 1\n    renumber = 0\n"])
+def test_cli_ns_noslip_outputs_match_reference_program(tmp_path, extra):
+    case = os.path.join(GOLDEN, "cli_ogrid_ns")
+    _check_outputs(tmp_path, _run(tmp_path, extra, case=case), case=case)
+
+
+def test_cli_stops_on_nonfinite_norm(tmp_path):
+    """A run that blows up (CFL 40) prints the monitor line with the
+    non-finite norms, then an error, and exits with status 3; the reference
+    would carry on through the NaNs."""
+    _run(tmp_path, "")
+    txt = (tmp_path / "input.in").read_text().replace("CFL = 0.8", "CFL = 40.0")
+    assert "CFL = 40.0" in txt
+    (tmp_path / "input.in").write_text(txt)
+    r = subprocess.run([CLI_PATH, "input.in"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 3, r.stdout + r.stderr
+    assert "ERROR: non-finite residual norm at iteration" in r.stdout
+    assert any(l.startswith("iter:") and ("nan" in l or "inf" in l) for l in r.stdout.splitlines())
 
 
 def test_cli_grid_cache(tmp_path):

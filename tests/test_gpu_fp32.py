@@ -130,14 +130,20 @@ def _bench_case(kind):
         return gg.ogrid(1024, 512), inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE", eqn="NS",
                                                 mu=1.225 * u / 100.0, BC1="NO_SLIP_WALL",
                                                 dt_type="GLOBAL", CFL=0.5)
+    if kind == "airfoil":
+        # the polygonal Delaunay-dual workload at 1M points (979k cells of 4 to
+        # 9 sides), its options unchanged
+        u, v = inputs.mach_velocity(0.7, angle_deg=1.25)
+        return gg.airfoil_dual(1000000), inputs.Case(u=u, v=v, poly=1, limiter="SQUEEZE",
+                                                     dt_type="LOCAL", CFL=0.5)
     raise ValueError(kind)
 
 
 # bump: the 2000-iteration first-order channel case, and the same channel at
 # second order without a limiter; ramp and ogrid: the second-order SQUEEZE
-# benchmark meshes at full size
+# benchmark meshes at full size; airfoil: the polygonal dual at 1M points
 @pytest.mark.parametrize("kind,steps", [("bump", 2000), ("bump_o2", 2000), ("ramp", 500),
-                                        ("ogrid", 200)])
+                                        ("ogrid", 200), ("airfoil", 200)])
 def test_fp32_error_vs_fp64(kind, steps):
     if kind == "bump_o2":
         grid, case = _bench_case("bump")
