@@ -1678,6 +1678,25 @@ __global__ __launch_bounds__(kMaxBlock) void k_res_sq(const T* __restrict__ res,
   }
 }
 
+// One thread per residual component k < 4 (blockDim >= 4, one block): the
+// Neumaier-compensated sum of partial[k*nblocks + b] over b in block order,
+// the same operations in the same order as the host sum of fvk_residual_l2,
+// and its square root into out[k].
+__global__ void k_res_norm(const double* __restrict__ partial, int nblocks,
+                           double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= 4) return;
+  double s = 0.0, comp = 0.0;
+  for (int b = 0; b < nblocks; ++b) {
+    const double x = partial[k * (size_t)nblocks + b];
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) comp += (s - t) + x;
+    else comp += (x - t) + s;
+    s = t;
+  }
+  out[k] = sqrt(s + comp);
+}
+
 // Error-free transformations (Knuth's TwoSum; the exact product error by one
 // fused multiply-add): s + e == a + b and p + e == a * b exactly.
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
@@ -1864,6 +1883,11 @@ hipError_t launch_init_sigma(const StageArgsT<T>& a, hipStream_t s) {
 template <typename T>
 hipError_t launch_res_sq(const T* res, int32_t ncp, int32_t blk, double* partial, hipStream_t s) {
   hipLaunchKernelGGL(k_res_sq<T>, dim3(ncp / blk), dim3(blk), 0, s, res, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_res_norm(const double* partial, int32_t nblocks, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_res_norm, dim3(1), dim3(64), 0, s, partial, (int)nblocks, out);
   return hipGetLastError();
 }
 

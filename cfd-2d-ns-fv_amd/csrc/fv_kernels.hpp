@@ -143,6 +143,9 @@ hipError_t launch_init_sigma(const StageArgsT<T>& a, hipStream_t s);
 // per-mesh-block partial sums of R_k^2: partial[k*nblocks + b]
 template <typename T>
 hipError_t launch_res_sq(const T* res, int32_t ncp, int32_t blk, double* partial, hipStream_t s);
+// out[k] = sqrt of the Neumaier sum over b of partial[k*nblocks + b], k < 4,
+// in block order on the device (the host sum's operations, bit for bit)
+hipError_t launch_res_norm(const double* partial, int32_t nblocks, double* out, hipStream_t s);
 // per-mesh-block compensated sums of V*q_k (k = 0..3) and of V:
 // partial[(2j)*nblocks + b] = hi, partial[(2j+1)*nblocks + b] = lo, j = 0..4
 template <typename T>

@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -27,6 +28,13 @@
 #include "fv_kernels.hpp"
 #include "fvk.h"
 #include "mesh_prep.hpp"
+
+// Scratch of fvk_residual_l2_noalloc, one copy per GPU for the life of the
+// process: [4][kScratchBlocks] per-block partial sums, then the 4 norms
+// (2 MiB).  Host code uses only its device address (hipGetSymbolAddress).
+constexpr int kScratchBlocks = 65536;   // most mesh blocks a call may have
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) double scratch[4 * kScratchBlocks + 4];
 
 namespace {
 
@@ -1041,6 +1049,34 @@ int checkpoint_load(Engine* c, const void* buf, int64_t bytes) {
   return FVK_OK;
 }
 
+// ---------------------------------------------------------------------------
+// fvk_residual_l2_noalloc: the residual norm with a static device scratch
+// array in place of the context's partial-sum buffer and the host vector.
+// Every call on a GPU shares that GPU's copy of the array (calls that
+// overlap on one GPU race); each call overwrites the slots it reads.
+// ---------------------------------------------------------------------------
+std::mutex g_scratch_mu;
+double* g_scratch[kScratchDevices] = {};   // device address of `scratch`, per device
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current.
+int scratch_on_device(Engine* c, double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_err(c, e, "hipGetDevice");
+  if (dev != c->device)
+    return set_err(c, FVK_ERR_ARG, "fvk_residual_l2_noalloc: the context's device is not current");
+  if (dev < 0 || dev >= kScratchDevices) return set_err(c, FVK_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_err(c, e, "hipGetSymbolAddress(scratch)");
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return FVK_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -1137,6 +1173,28 @@ int fvk_get_residual(fvk_ctx* c, double* r) {
 int fvk_residual_l2(fvk_ctx* c, double out[4]) {
   if (!c || !out) return FVK_ERR_ARG;
   return residual_l2(&c->e, out);
+}
+
+int fvk_residual_l2_noalloc(fvk_ctx* cx, double out[4]) {
+  if (!cx || !out) return FVK_ERR_ARG;
+  Engine* c = &cx->e;
+  const int32_t nb = c->dm.nblocks;
+  if (nb > kScratchBlocks)
+    return set_err(c, FVK_ERR_ARG, "fvk_residual_l2_noalloc: the mesh has more than 65536 blocks");
+  double* ptr = nullptr;
+  int rc = scratch_on_device(c, &ptr);
+  if (rc) return rc;
+  double* norms = ptr + 4 * (size_t)kScratchBlocks;
+  if (c->fp32)
+    FVK_HIP(c, fvk::launch_res_sq(static_cast<const float*>(c->res), c->dm.ncp, c->dm.blk, ptr, c->stream));
+  else
+    FVK_HIP(c, fvk::launch_res_sq(static_cast<const double*>(c->res), c->dm.ncp, c->dm.blk, ptr, c->stream));
+  FVK_HIP(c, fvk::launch_res_norm(ptr, nb, norms, c->stream));
+  FVK_HIP(c, hipMemcpyAsync(out, norms, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FVK_HIP(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 4; ++k)
+    if (!std::isfinite(out[k])) return set_err(c, FVK_ERR_NONFINITE, "residual norm is not finite");
+  return FVK_OK;
 }
 
 int fvk_conservation_sums(fvk_ctx* c, double out[4]) {

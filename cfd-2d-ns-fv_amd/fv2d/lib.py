@@ -28,7 +28,8 @@ RETIRED_NONE, RETIRED_DROP, RETIRED_NONFINITE, RETIRED_STALL, RETIRED_CALLER = r
 # every symbol include/fvk.h declares
 EXPORTS = ["fvk_default_options", "fvk_create", "fvk_destroy", "fvk_last_error",
            "fvk_init_freestream", "fvk_set_state", "fvk_step", "fvk_get_state",
-           "fvk_use_graph", "fvk_get_residual", "fvk_residual_l2", "fvk_conservation_sums",
+           "fvk_use_graph", "fvk_get_residual", "fvk_residual_l2", "fvk_residual_l2_noalloc",
+           "fvk_conservation_sums",
            "fvk_get_geometry", "fvk_geometry",
            "fvk_get_info", "fvk_get_time", "fvk_get_options", "fvk_time_steps", "fvk_time_kernels",
            "fvk_batch_create", "fvk_batch_destroy", "fvk_batch_last_error", "fvk_batch_init",
@@ -94,6 +95,7 @@ def lib():
         L.fvk_get_state.argtypes = [vp, dp, dp]
         L.fvk_get_residual.argtypes = [vp, dp]
         L.fvk_residual_l2.argtypes = [vp, dp]
+        L.fvk_residual_l2_noalloc.argtypes = [vp, dp]
         L.fvk_use_graph.argtypes = [vp, ctypes.c_int32]
         L.fvk_conservation_sums.argtypes = [vp, dp]
         L.fvk_get_geometry.argtypes = [vp, ctypes.c_char_p, dp]
@@ -241,6 +243,14 @@ class Solver:
     def residual_l2(self) -> np.ndarray:
         out = np.empty(4)
         rc = lib().fvk_residual_l2(self.h, _dp(out))
+        if rc not in (FVK_OK, -5):
+            self._check(rc)
+        return out
+
+    def residual_l2_noalloc(self) -> np.ndarray:
+        """residual_l2 through the allocation-free entry point."""
+        out = np.empty(4)
+        rc = lib().fvk_residual_l2_noalloc(self.h, _dp(out))
         if rc not in (FVK_OK, -5):
             self._check(rc)
         return out

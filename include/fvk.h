@@ -140,6 +140,16 @@ int fvk_get_residual(fvk_ctx* ctx, double* r);
 /* sqrt(sum_i R_k,i^2), k = 1..4, of that residual (save_sol.cpp:79-89);
  * FVK_ERR_NONFINITE if any is not finite (out[] is still written). */
 int fvk_residual_l2(fvk_ctx* ctx, double out[4]);
+/* fvk_residual_l2 without any allocation, for callers that cannot allocate
+ * (serving loops and harnesses that forbid hipMalloc and host-heap use on
+ * their path).  No counterpart in the reference.  The same norms, bit for
+ * bit: the per-block partial sums and their compensated sum go through a
+ * static device array of 2 MiB per GPU (held until the process ends) instead
+ * of the context's buffer and a host vector.  Meshes of at most 65,536 blocks
+ * (8,388,608 cells at 128-cell blocks); larger ones get FVK_ERR_ARG.  The
+ * context's device must be current.  Every call on a GPU shares that GPU's
+ * array, so calls that overlap on one GPU (from several host threads) race. */
+int fvk_residual_l2_noalloc(fvk_ctx* ctx, double out[4]);
 
 /* Global conservation sums of the current state: out[k] = sum_i V_i Q_k,i,
  * k = 1..4 (mass, x and y momentum, energy), with compensated summation
