@@ -657,6 +657,28 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
     for (int k = 0; k < 4; ++k) L[k] = c1.q[k];
   }
   T FL[4], FR[4], VnL, VnR, cL, cR;
+  // the viscous flux, taken as soon as both face states exist, so that the
+  // cells' gradients are no longer live while the inviscid terms are formed
+  T Fv[4] = {0, 0, 0, 0};
+  auto viscous = [&]() {
+    if (NS && ORDER == 2 && (c2 || kind == 3)) {
+#if FVK_NS64
+      if constexpr (kDelta<T>) {
+        double Fd[4];
+        viscous_flux<double>(a, fl, c1, c2, g, L, R, Fd);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Fv[k] = T(Fd[k]);
+      } else
+#elif FVK_FAST32
+      if constexpr (kDelta<T>) {
+        viscous_flux_fast(a, fl, c1, c2, g, L, R, Fv);
+      } else
+#endif
+      {
+        viscous_flux<T>(a, fl, c1, c2, g, L, R, Fv);
+      }
+    }
+  };
   if constexpr (kDelta<T>) {
     SideFlux<T> sl, sr;
     side_flux_delta(fl, L, nx, ny, sl);
@@ -687,6 +709,7 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
       R[2] = -L[2] - T(2.0) * fl.qinf[2];
       R[3] = L[3];
     }
+    viscous();
     side_flux_delta(fl, R, nx, ny, sr);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -720,6 +743,7 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
     } else {                  // NO_SLIP_WALL
       ghost_q(gm1, pl.rho, -pl.u, -pl.v, pl.P, R);
     }
+    viscous();
     Prim<T> pr;
     to_prim(R, gamma, gm1, pr);
     VnL = pl.u * nx + pl.v * ny;
@@ -737,24 +761,6 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
   }
   const T vn_avg = (VnL + VnR) / 2;
   const T c_avg = (cL + cR) / 2;
-  T Fv[4] = {0, 0, 0, 0};
-  if (NS && ORDER == 2 && (c2 || kind == 3)) {
-#if FVK_NS64
-    if constexpr (kDelta<T>) {
-      double Fd[4];
-      viscous_flux<double>(a, fl, c1, c2, g, L, R, Fd);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) Fv[k] = T(Fd[k]);
-    } else
-#elif FVK_FAST32
-    if constexpr (kDelta<T>) {
-      viscous_flux_fast(a, fl, c1, c2, g, L, R, Fv);
-    } else
-#endif
-    {
-      viscous_flux<T>(a, fl, c1, c2, g, L, R, Fv);
-    }
-  }
   const T lam = (fabs(vn_avg) + c_avg) / 2;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -1204,7 +1210,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
 //               there, while the deep form costs FP64 pass B 8% on
 //               triangle/quad meshes);
 //   kMinBlocksA / kMinBlocksB  minimum resident blocks per CU the register
-//               allocation must allow, per state type (FP64, FP32).
+//               allocation must allow, per state type (FP64, FP32);
+//   FVK_NS_LATE the FP64 NS pass B prefetches one face (not kFPF) and loads
+//               the stage-update data after its face phase: 124 VGPRs
+//               instead of 154, 4 waves per SIMD instead of 3.
 #ifndef FVK_PF
 #define FVK_PF 4
 #endif
@@ -1213,6 +1222,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
 #endif
 #ifndef FVK_FPF
 #define FVK_FPF 2
+#endif
+#ifndef FVK_NS_LATE
+#define FVK_NS_LATE 1
 #endif
 #ifndef FVK_MINBLK_A64
 #define FVK_MINBLK_A64 1
@@ -1445,6 +1457,8 @@ void k_pass_b_lds(StageArgsT<T> a) {
   extern __shared__ unsigned char lds_raw[];
   constexpr int NQ = ORDER == 2 ? 12 : 4;
   constexpr bool XY = !kDelta<T>;
+  constexpr bool kLate = FVK_NS_LATE && NS && !kDelta<T>;   // FP64 NS: the leaner prefetch
+  constexpr int kFPFB = kLate ? 1 : kFPF;
   const DevMesh& m = a.m;
   int b = 0, inst = 0;
   const bool live = block_of(a, logical_block(a.xcd_blocks), b, inst);   // uniform
@@ -1462,10 +1476,12 @@ void k_pass_b_lds(StageArgsT<T> a) {
     const int deg = cell_thread ? m.deg[c] : 0;
     const int base = m.chunk_off[c >> 6] + (c & 63);
     // prefetch: this thread's faces, its cell's slot-to-face map and the
-    // cell data of the stage update
-    FaceRec<T, NS> fr[kFPF];
+    // cell data of the stage update.  The FP64 NS kernel, whose face terms
+    // need the most registers, prefetches one face and loads the cell data
+    // after the face phase, so that fewer values stay live through it.
+    FaceRec<T, NS> fr[kFPFB];
 #pragma unroll
-    for (int t = 0; t < kFPF; ++t) {
+    for (int t = 0; t < kFPFB; ++t) {
       const int e = i + t * blockDim.x;
       if (e < nfb) load_face(m, fb + e, fr[t]);
     }
@@ -1475,11 +1491,14 @@ void k_pass_b_lds(StageArgsT<T> a) {
       if (j < deg) sf[j] = m.sface[base + j * 64];
     T qn[4] = {0, 0, 0, 0}, sig_old = T(1);
     double V = 1.0;
-    if (deg > 0) {
-      if (a.stage > 1) load_q(a.qn, c, qn);
-      V = m.vol[c];
-      if (!a.dt_global) sig_old = a.sigma[c];
-    }
+    auto load_update_data = [&]() {
+      if (deg > 0) {
+        if (a.stage > 1) load_q(a.qn, c, qn);
+        V = m.vol[c];
+        if (!a.dt_global) sig_old = a.sigma[c];
+      }
+    };
+    if (!kLate) load_update_data();
     stage_block(a, b, L);
     __syncthreads();
     // one thread per face of the block
@@ -1505,15 +1524,16 @@ void k_pass_b_lds(StageArgsT<T> a) {
       if (a.stage == 3) lf[4 * fstride + e] = st;
     };
 #pragma unroll
-    for (int t = 0; t < kFPF; ++t) {
+    for (int t = 0; t < kFPFB; ++t) {
       const int e = i + t * blockDim.x;
       if (e < nfb) face(e, fr[t]);
     }
-    for (int e = i + kFPF * blockDim.x; e < nfb; e += blockDim.x) {
+    for (int e = i + kFPFB * blockDim.x; e < nfb; e += blockDim.x) {
       FaceRec<T, NS> f;
       load_face(m, fb + e, f);
       face(e, f);
     }
+    if (kLate) load_update_data();
     __syncthreads();
     // one thread per cell: add the face terms in slot order (res.cpp:74-102)
     if (deg > 0) {
@@ -1695,6 +1715,26 @@ __global__ void k_res_norm(const double* __restrict__ partial, int nblocks,
     s = t;
   }
   out[k] = sqrt(s + comp);
+}
+
+// Per instance i < n and component k < 4 (one thread each): the same
+// compensated sum over the instance's blocks [inst_off[i], inst_off[i+1]),
+// out[4*i + k] = its square root.
+__global__ void k_res_norm_inst(const double* __restrict__ partial, int nblocks,
+                                const int32_t* __restrict__ inst_off, int n,
+                                double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t >> 2, k = t & 3;
+  if (i >= n) return;
+  double s = 0.0, comp = 0.0;
+  for (int b = inst_off[i]; b < inst_off[i + 1]; ++b) {
+    const double x = partial[k * (size_t)nblocks + b];
+    const double u = s + x;
+    if (fabs(s) >= fabs(x)) comp += (s - u) + x;
+    else comp += (x - u) + s;
+    s = u;
+  }
+  out[4 * i + k] = sqrt(s + comp);
 }
 
 // Error-free transformations (Knuth's TwoSum; the exact product error by one
@@ -1888,6 +1928,14 @@ hipError_t launch_res_sq(const T* res, int32_t ncp, int32_t blk, double* partial
 
 hipError_t launch_res_norm(const double* partial, int32_t nblocks, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_res_norm, dim3(1), dim3(64), 0, s, partial, (int)nblocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_res_norm_inst(const double* partial, int32_t nblocks, const int32_t* inst_off,
+                               int32_t n, double* out, hipStream_t s) {
+  const int threads = 256, grid = (4 * n + threads - 1) / threads;
+  hipLaunchKernelGGL(k_res_norm_inst, dim3(grid), dim3(threads), 0, s, partial, (int)nblocks,
+                     inst_off, (int)n, out);
   return hipGetLastError();
 }
 

@@ -146,6 +146,10 @@ hipError_t launch_res_sq(const T* res, int32_t ncp, int32_t blk, double* partial
 // out[k] = sqrt of the Neumaier sum over b of partial[k*nblocks + b], k < 4,
 // in block order on the device (the host sum's operations, bit for bit)
 hipError_t launch_res_norm(const double* partial, int32_t nblocks, double* out, hipStream_t s);
+// out[4*i + k] = the same per instance i < n over its blocks
+// [inst_off[i], inst_off[i+1])
+hipError_t launch_res_norm_inst(const double* partial, int32_t nblocks, const int32_t* inst_off,
+                               int32_t n, double* out, hipStream_t s);
 // per-mesh-block compensated sums of V*q_k (k = 0..3) and of V:
 // partial[(2j)*nblocks + b] = hi, partial[(2j+1)*nblocks + b] = lo, j = 0..4
 template <typename T>

@@ -81,6 +81,8 @@ struct Engine {
   unsigned long long* smax = nullptr;         // [2][n], FP64 bits
   double* time = nullptr;                     // [n] physical time of the uniform dt
   double* partial = nullptr;                  // [10][nblocks]: residual and conservation sums
+  int32_t* inst_blocks = nullptr;             // [n+1]: first mesh block of each instance
+  double* norms = nullptr;                    // [4][n]: residual norms
   int device = 0;                             // the HIP device the context was created on
   int block = 256;
   int variant = fvk::kVariantLds;
@@ -581,29 +583,22 @@ struct Neumaier {
   double value() const { return s + comp; }
 };
 
-// sqrt(sum R_k^2) of every instance's last stored residual: out[4*i + k]
+// sqrt(sum R_k^2) of every instance's last stored residual: out[4*i + k].
+// Per-block partial sums, then per instance a Neumaier-compensated sum of its
+// block partials in block order, both on the device; only the 4 norms per
+// instance cross to the host.
 int residual_l2(Engine* c, double* out) {
-  const int32_t nb = c->dm.nblocks;
+  const int32_t nb = c->dm.nblocks, n = (int32_t)c->in.size();
   if (c->fp32)
     FVK_HIP(c, fvk::launch_res_sq(static_cast<const float*>(c->res), c->dm.ncp, c->dm.blk, c->partial, c->stream));
   else
     FVK_HIP(c, fvk::launch_res_sq(static_cast<const double*>(c->res), c->dm.ncp, c->dm.blk, c->partial, c->stream));
-  std::vector<double> part(4 * (size_t)nb);
-  FVK_HIP(c, hipMemcpyAsync(part.data(), c->partial, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  FVK_HIP(c, fvk::launch_res_norm_inst(c->partial, nb, c->inst_blocks, n, c->norms, c->stream));
+  FVK_HIP(c, hipMemcpyAsync(out, c->norms, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
   FVK_HIP(c, hipStreamSynchronize(c->stream));
-  bool finite = true;
-  for (size_t i = 0; i < c->in.size(); ++i) {
-    const Instance& in = c->in[i];
-    for (int k = 0; k < 4; ++k) {
-      // Neumaier-compensated sum of the instance's block partials
-      Neumaier acc;
-      for (int32_t b = in.block_off; b < in.block_off + in.hm.nblocks; ++b)
-        acc.add(part[k * (size_t)nb + b]);
-      out[4 * i + k] = std::sqrt(acc.value());
-      if (!std::isfinite(out[4 * i + k])) finite = false;
-    }
-  }
-  if (!finite) return set_err(c, FVK_ERR_NONFINITE, "residual norm is not finite");
+  for (int32_t i = 0; i < 4 * n; ++i)
+    if (!std::isfinite(out[i])) return set_err(c, FVK_ERR_NONFINITE, "residual norm is not finite");
   return FVK_OK;
 }
 
@@ -914,6 +909,14 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   if ((rc = dev_alloc(c, &c->time, c->in.size()))) return rc;
   if ((rc = fill(c, c->time, (int64_t)c->in.size(), 0.0))) return rc;
   if ((rc = dev_alloc(c, &c->partial, 10 * (size_t)d.nblocks))) return rc;
+  if ((rc = dev_alloc(c, &c->norms, 4 * c->in.size()))) return rc;
+  {
+    std::vector<int32_t> ib;
+    for (const Instance& in : c->in) ib.push_back(in.block_off);
+    ib.push_back(d.nblocks);
+    if ((rc = dev_alloc(c, &c->inst_blocks, ib.size()))) return rc;
+    FVK_HIP(c, hipMemcpy(c->inst_blocks, ib.data(), ib.size() * 4, hipMemcpyHostToDevice));
+  }
   for (int i = 0; i < 3; ++i) {
     if ((rc = fill(c, c->q[i], 4 * (int64_t)ncp, 0.0))) return rc;
     if ((rc = fill(c, c->g[i], 8 * (int64_t)ncp, 0.0))) return rc;
