@@ -1213,7 +1213,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
 //               allocation must allow, per state type (FP64, FP32);
 //   FVK_NS_LATE the FP64 NS pass B prefetches one face (not kFPF) and loads
 //               the stage-update data after its face phase: 124 VGPRs
-//               instead of 154, 4 waves per SIMD instead of 3.
+//               instead of 154, 4 waves per SIMD instead of 3;
+//   FVK_A_LATE  pass A loads the limiter's data (lagged state and gradient,
+//               face positions) after the gradient loop (1), the LSQ
+//               inverse too (2), or all before the staging barrier (0).  1
+//               takes the SQUEEZE pass A from 85 to 66 VGPRs (FP32, 7 waves
+//               per SIMD) and from 145 to 110 (FP64, 4 waves): 8% (FP32) and
+//               5% (FP64) less time per launch on the 256-instance batch.
 #ifndef FVK_PF
 #define FVK_PF 4
 #endif
@@ -1225,6 +1231,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
 #endif
 #ifndef FVK_NS_LATE
 #define FVK_NS_LATE 1
+#endif
+#ifndef FVK_A_LATE
+#define FVK_A_LATE 1
 #endif
 #ifndef FVK_MINBLK_A64
 #define FVK_MINBLK_A64 1
@@ -1319,21 +1328,26 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
   const int c = b * m.blk + i;
   const int deg = m.deg[c];
   const int base = m.chunk_off[c >> 6] + (c & 63);
-  // prefetch: slot codes and face positions, lagged state, LSQ inverse
+  // prefetch: slot codes and face positions, lagged state, LSQ inverse (the
+  // limiter's data after the gradient loop with FVK_A_LATE >= 1, the LSQ
+  // inverse too with 2, so that fewer values are live through the loop)
   int lc[PF];
   SlotPos<T> pfx[PF], pfy[PF];
+  auto load_pf = [&]() {
 #pragma unroll
-  for (int j = 0; j < PF; ++j) {
-    if (j < deg) {
-      const int s = base + j * 64;
-      lc[j] = m.lcode[s];
-      slot_pos<T>(m, s, pfx[j], pfy[j]);
-    }
-  }
+    for (int j = 0; j < PF; ++j)
+      if (j < deg) slot_pos<T>(m, base + j * 64, pfx[j], pfy[j]);
+  };
+#pragma unroll
+  for (int j = 0; j < PF; ++j)
+    if (j < deg) lc[j] = m.lcode[base + j * 64];
   T qp[4], gpx[4], gpy[4];
-  if (squeeze && deg > 0) load_lagged(a, c, qp, gpx, gpy);
+  if (FVK_A_LATE < 1) {
+    load_pf();
+    if (squeeze && deg > 0) load_lagged(a, c, qp, gpx, gpy);
+  }
   double ixx = 0, iyy = 0, ixy = 0;
-  if (deg > 0) load_lsq_inv(m, c, ixx, iyy, ixy);
+  if (FVK_A_LATE < 2 && deg > 0) load_lsq_inv(m, c, ixx, iyy, ixy);
   stage_block(a, b, L);
   __syncthreads();
   if (deg == 0) return;
@@ -1364,6 +1378,11 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
   for (int j = 0; j < PF; ++j)
     if (j < deg) slot(j, lc[j]);
   for (int j = PF; j < deg; ++j) slot(j, m.lcode[base + j * 64]);
+  if (FVK_A_LATE >= 1) {
+    load_pf();
+    if (squeeze) load_lagged(a, c, qp, gpx, gpy);
+  }
+  if (FVK_A_LATE >= 2) load_lsq_inv(m, c, ixx, iyy, ixy);
   T gx[4], gy[4];
   lsq_solve(ixx, iyy, ixy, sx, sy, gx, gy);
   if (squeeze) {
@@ -1494,7 +1513,9 @@ void k_pass_b_lds(StageArgsT<T> a) {
     auto load_update_data = [&]() {
       if (deg > 0) {
         if (a.stage > 1) load_q(a.qn, c, qn);
-        V = m.vol[c];
+        // the FP32 update rounds V to FP32 (cell_update); the uniform dt's
+        // maximum takes Sigma / V in FP64
+        V = (kDelta<T> && m.vol32 && !a.dt_uniform) ? double(m.vol32[c]) : m.vol[c];
         if (!a.dt_global) sig_old = a.sigma[c];
       }
     };
@@ -1698,43 +1719,38 @@ __global__ __launch_bounds__(kMaxBlock) void k_res_sq(const T* __restrict__ res,
   }
 }
 
-// One thread per residual component k < 4 (blockDim >= 4, one block): the
-// Neumaier-compensated sum of partial[k*nblocks + b] over b in block order,
-// the same operations in the same order as the host sum of fvk_residual_l2,
-// and its square root into out[k].
-__global__ void k_res_norm(const double* __restrict__ partial, int nblocks,
-                           double* __restrict__ out) {
-  const int k = threadIdx.x;
-  if (k >= 4) return;
-  double s = 0.0, comp = 0.0;
-  for (int b = 0; b < nblocks; ++b) {
-    const double x = partial[k * (size_t)nblocks + b];
-    const double t = s + x;
-    if (fabs(s) >= fabs(x)) comp += (s - t) + x;
-    else comp += (x - t) + s;
-    s = t;
-  }
-  out[k] = sqrt(s + comp);
-}
-
-// Per instance i < n and component k < 4 (one thread each): the same
-// compensated sum over the instance's blocks [inst_off[i], inst_off[i+1]),
+// Residual norms from the per-block partial sums partial[k*nblocks + b]:
+// one wavefront (blockDim = 64) per instance i < n and component k < 4
+// (grid = 4n, block 4i + k).  The wavefront stages 64 consecutive partials in
+// LDS with one coalesced load and its first lane adds them in block order
+// (Neumaier-compensated), so the sum takes the same operations in the same
+// order as a sequential loop over the instance's blocks [inst_off[i],
+// inst_off[i+1]) (inst_off null: one instance, all blocks);
 // out[4*i + k] = its square root.
-__global__ void k_res_norm_inst(const double* __restrict__ partial, int nblocks,
-                                const int32_t* __restrict__ inst_off, int n,
-                                double* __restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = t >> 2, k = t & 3;
-  if (i >= n) return;
+__global__ __launch_bounds__(64) void k_res_norm(const double* __restrict__ partial, int nblocks,
+                                                 const int32_t* __restrict__ inst_off,
+                                                 double* __restrict__ out) {
+  __shared__ double buf[64];
+  const int i = blockIdx.x >> 2, k = blockIdx.x & 3, lane = threadIdx.x;
+  const int b0 = inst_off ? inst_off[i] : 0, b1 = inst_off ? inst_off[i + 1] : nblocks;
   double s = 0.0, comp = 0.0;
-  for (int b = inst_off[i]; b < inst_off[i + 1]; ++b) {
-    const double x = partial[k * (size_t)nblocks + b];
-    const double u = s + x;
-    if (fabs(s) >= fabs(x)) comp += (s - u) + x;
-    else comp += (x - u) + s;
-    s = u;
+  for (int base = b0; base < b1; base += 64) {
+    const int b = base + lane;
+    buf[lane] = b < b1 ? partial[k * (size_t)nblocks + b] : 0.0;
+    __syncthreads();
+    if (lane == 0) {
+      const int m = min(64, b1 - base);
+      for (int j = 0; j < m; ++j) {
+        const double x = buf[j];
+        const double t = s + x;
+        if (fabs(s) >= fabs(x)) comp += (s - t) + x;
+        else comp += (x - t) + s;
+        s = t;
+      }
+    }
+    __syncthreads();
   }
-  out[4 * i + k] = sqrt(s + comp);
+  if (lane == 0) out[4 * i + k] = sqrt(s + comp);
 }
 
 // Error-free transformations (Knuth's TwoSum; the exact product error by one
@@ -1927,15 +1943,15 @@ hipError_t launch_res_sq(const T* res, int32_t ncp, int32_t blk, double* partial
 }
 
 hipError_t launch_res_norm(const double* partial, int32_t nblocks, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_res_norm, dim3(1), dim3(64), 0, s, partial, (int)nblocks, out);
+  hipLaunchKernelGGL(k_res_norm, dim3(4), dim3(64), 0, s, partial, (int)nblocks,
+                     (const int32_t*)nullptr, out);
   return hipGetLastError();
 }
 
 hipError_t launch_res_norm_inst(const double* partial, int32_t nblocks, const int32_t* inst_off,
                                int32_t n, double* out, hipStream_t s) {
-  const int threads = 256, grid = (4 * n + threads - 1) / threads;
-  hipLaunchKernelGGL(k_res_norm_inst, dim3(grid), dim3(threads), 0, s, partial, (int)nblocks,
-                     inst_off, (int)n, out);
+  hipLaunchKernelGGL(k_res_norm, dim3(4 * n), dim3(64), 0, s, partial, (int)nblocks, inst_off,
+                     out);
   return hipGetLastError();
 }
 

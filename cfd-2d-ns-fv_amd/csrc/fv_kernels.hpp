@@ -42,6 +42,7 @@ struct DevMesh {
   const float* ixx32;          // the same rounded to FP32 (FP32-mixed on well-conditioned
   const float* iyy32;          // meshes; null otherwise, and then ixx.. are read)
   const float* ixy32;
+  const float* vol32;          // FP32-mixed: the volume rounded to FP32 (null otherwise)
   const int32_t* halo_ptr;     // per block: halo cells
   const int32_t* halo_idx;
   const int32_t* bf_ptr;       // per block: face list

@@ -848,8 +848,10 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
     if ((rc = upload_cat(c, &HM::c_iyy, &d.iyy))) return rc;
     if ((rc = upload_cat(c, &HM::c_ixy, &d.ixy))) return rc;
   }
+  d.vol32 = nullptr;
   if (c->fp32) {
     // the FP32 kernels' single-precision geometry (mesh_prep.hpp)
+    if ((rc = upload_cat(c, &HM::c_vol32, &d.vol32))) return rc;
     if ((rc = upload_cat(c, &HM::s_dfx32, &d.s_dfx32))) return rc;
     if ((rc = upload_cat(c, &HM::s_dfy32, &d.s_dfy32))) return rc;
     if ((rc = upload_cat(c, &HM::bf32_nx, &d.bf32_nx))) return rc;
