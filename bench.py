@@ -701,13 +701,20 @@ def bench_batch(args):
                 rwhy = rb.retired()
                 ok32 = np.isin(r["why"], conv_why)
                 ok64 = np.isin(rwhy, (RETIRED_DROP, RETIRED_STALL))
-                ref_err = []
+                ref_err, ref_why = [], []
                 for i, (g, c) in enumerate(cases):
                     if ok32[i] and ok64[i]:
                         ref_err.append(errors_vs_fp64(r["batch"].state(i), rb.state(i),
                                                       c.gamma)["rho_l2_rel"])
+                        ref_why.append(rwhy[i])
                 rb.close()
-                ref_err = np.array(ref_err)
+                ref_err, ref_why = np.array(ref_err), np.array(ref_why)
+
+                def summary(e):
+                    return {"instances": int(e.size),
+                            "rho_l2_rel_median": float(f"{np.median(e):.3e}") if e.size else None,
+                            "rho_l2_rel_max": float(f"{e.max():.3e}") if e.size else None,
+                            "instances_rho_l2_le_1e-5": int((e <= 1e-5).sum())}
                 out["error_at_convergence_vs_converged_fp64"] = {
                     "instances": int(ref_err.size),
                     "fp32_converged": int(ok32.sum()),
@@ -719,6 +726,8 @@ def bench_batch(args):
                     "rho_l2_rel_median": float(f"{np.median(ref_err):.3e}") if ref_err.size else None,
                     "rho_l2_rel_max": float(f"{ref_err.max():.3e}") if ref_err.size else None,
                     "instances_rho_l2_le_1e-5": int((ref_err <= 1e-5).sum()),
+                    "where_fp64_reference_reached_the_drop": summary(ref_err[ref_why == RETIRED_DROP]),
+                    "where_fp64_reference_stalled": summary(ref_err[ref_why == RETIRED_STALL]),
                     "compared": "each FP32-mixed state at its retirement (drop"
                                 + (" or stall" if args.stall_checks else "")
                                 + f") against FP64 run to a {args.reference_drop:g} drop"

@@ -1513,9 +1513,7 @@ void k_pass_b_lds(StageArgsT<T> a) {
     auto load_update_data = [&]() {
       if (deg > 0) {
         if (a.stage > 1) load_q(a.qn, c, qn);
-        // the FP32 update rounds V to FP32 (cell_update); the uniform dt's
-        // maximum takes Sigma / V in FP64
-        V = (kDelta<T> && m.vol32 && !a.dt_uniform) ? double(m.vol32[c]) : m.vol[c];
+        V = m.vol[c];
         if (!a.dt_global) sig_old = a.sigma[c];
       }
     };
@@ -1721,36 +1719,37 @@ __global__ __launch_bounds__(kMaxBlock) void k_res_sq(const T* __restrict__ res,
 
 // Residual norms from the per-block partial sums partial[k*nblocks + b]:
 // one wavefront (blockDim = 64) per instance i < n and component k < 4
-// (grid = 4n, block 4i + k).  The wavefront stages 64 consecutive partials in
-// LDS with one coalesced load and its first lane adds them in block order
-// (Neumaier-compensated), so the sum takes the same operations in the same
-// order as a sequential loop over the instance's blocks [inst_off[i],
-// inst_off[i+1]) (inst_off null: one instance, all blocks);
+// (grid = 4n, block 4i + k), over the instance's blocks [inst_off[i],
+// inst_off[i+1]) (inst_off null: one instance, all blocks).  Lane l keeps a
+// Neumaier-compensated sum of the blocks b = b0 + l, b0 + l + 64, ...; the
+// first lane then adds the 64 lane sums in lane order, with their
+// compensations, so the result does not depend on anything but the partials;
 // out[4*i + k] = its square root.
 __global__ __launch_bounds__(64) void k_res_norm(const double* __restrict__ partial, int nblocks,
                                                  const int32_t* __restrict__ inst_off,
                                                  double* __restrict__ out) {
-  __shared__ double buf[64];
+  __shared__ double ls[64], lc[64];
   const int i = blockIdx.x >> 2, k = blockIdx.x & 3, lane = threadIdx.x;
   const int b0 = inst_off ? inst_off[i] : 0, b1 = inst_off ? inst_off[i + 1] : nblocks;
+  auto add = [](double& s, double& comp, double x) {
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) comp += (s - t) + x;
+    else comp += (x - t) + s;
+    s = t;
+  };
   double s = 0.0, comp = 0.0;
-  for (int base = b0; base < b1; base += 64) {
-    const int b = base + lane;
-    buf[lane] = b < b1 ? partial[k * (size_t)nblocks + b] : 0.0;
-    __syncthreads();
-    if (lane == 0) {
-      const int m = min(64, b1 - base);
-      for (int j = 0; j < m; ++j) {
-        const double x = buf[j];
-        const double t = s + x;
-        if (fabs(s) >= fabs(x)) comp += (s - t) + x;
-        else comp += (x - t) + s;
-        s = t;
-      }
+  for (int b = b0 + lane; b < b1; b += 64) add(s, comp, partial[k * (size_t)nblocks + b]);
+  ls[lane] = s;
+  lc[lane] = comp;
+  __syncthreads();
+  if (lane == 0) {
+    double S = 0.0, C = 0.0;
+    for (int j = 0; j < 64; ++j) {
+      add(S, C, ls[j]);
+      C += lc[j];
     }
-    __syncthreads();
+    out[4 * i + k] = sqrt(S + C);
   }
-  if (lane == 0) out[4 * i + k] = sqrt(s + comp);
 }
 
 // Error-free transformations (Knuth's TwoSum; the exact product error by one

@@ -42,7 +42,6 @@ struct DevMesh {
   const float* ixx32;          // the same rounded to FP32 (FP32-mixed on well-conditioned
   const float* iyy32;          // meshes; null otherwise, and then ixx.. are read)
   const float* ixy32;
-  const float* vol32;          // FP32-mixed: the volume rounded to FP32 (null otherwise)
   const int32_t* halo_ptr;     // per block: halo cells
   const int32_t* halo_idx;
   const int32_t* bf_ptr;       // per block: face list
@@ -144,8 +143,8 @@ hipError_t launch_init_sigma(const StageArgsT<T>& a, hipStream_t s);
 // per-mesh-block partial sums of R_k^2: partial[k*nblocks + b]
 template <typename T>
 hipError_t launch_res_sq(const T* res, int32_t ncp, int32_t blk, double* partial, hipStream_t s);
-// out[k] = sqrt of the Neumaier sum over b of partial[k*nblocks + b], k < 4,
-// in block order on the device (the host sum's operations, bit for bit)
+// out[k] = sqrt of the compensated sum over b of partial[k*nblocks + b],
+// k < 4: 64 strided Neumaier sums, then those added in a fixed order
 hipError_t launch_res_norm(const double* partial, int32_t nblocks, double* out, hipStream_t s);
 // out[4*i + k] = the same per instance i < n over its blocks
 // [inst_off[i], inst_off[i+1])

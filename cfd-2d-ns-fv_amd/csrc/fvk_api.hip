@@ -584,9 +584,9 @@ struct Neumaier {
 };
 
 // sqrt(sum R_k^2) of every instance's last stored residual: out[4*i + k].
-// Per-block partial sums, then per instance a Neumaier-compensated sum of its
-// block partials in block order, both on the device; only the 4 norms per
-// instance cross to the host.
+// Per-block partial sums, then per instance a compensated sum of its block
+// partials (k_res_norm), both on the device; only the 4 norms per instance
+// cross to the host.
 int residual_l2(Engine* c, double* out) {
   const int32_t nb = c->dm.nblocks, n = (int32_t)c->in.size();
   if (c->fp32)
@@ -848,10 +848,8 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
     if ((rc = upload_cat(c, &HM::c_iyy, &d.iyy))) return rc;
     if ((rc = upload_cat(c, &HM::c_ixy, &d.ixy))) return rc;
   }
-  d.vol32 = nullptr;
   if (c->fp32) {
     // the FP32 kernels' single-precision geometry (mesh_prep.hpp)
-    if ((rc = upload_cat(c, &HM::c_vol32, &d.vol32))) return rc;
     if ((rc = upload_cat(c, &HM::s_dfx32, &d.s_dfx32))) return rc;
     if ((rc = upload_cat(c, &HM::s_dfy32, &d.s_dfy32))) return rc;
     if ((rc = upload_cat(c, &HM::bf32_nx, &d.bf32_nx))) return rc;
@@ -1056,7 +1054,7 @@ int checkpoint_load(Engine* c, const void* buf, int64_t bytes) {
 
 // ---------------------------------------------------------------------------
 // fvk_residual_l2_noalloc: the residual norm with a static device scratch
-// array in place of the context's partial-sum buffer and the host vector.
+// array in place of the context's partial-sum and norm buffers.
 // Every call on a GPU shares that GPU's copy of the array (calls that
 // overlap on one GPU race); each call overwrites the slots it reads.
 // ---------------------------------------------------------------------------

@@ -643,7 +643,6 @@ int prepare_mesh(const fvk_mesh& m, const fvk_options& o, HostMesh* out, std::st
     round32(h.c_ixx, h.c_ixx32);
     round32(h.c_iyy, h.c_iyy32);
     round32(h.c_ixy, h.c_ixy32);
-    round32(h.c_vol, h.c_vol32);
   }
   return FVK_OK;
 }

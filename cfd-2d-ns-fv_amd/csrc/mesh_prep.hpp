@@ -94,9 +94,8 @@ struct HostMesh {
   // per cell (new numbering, padded)
   hvec<double> c_ccx, c_ccy, c_vol, c_ixx, c_iyy, c_ixy;
   // FP32-mixed only: the LSQ inverse entries rounded once to FP32 (used when
-  // every cell's normal matrix is well conditioned), and the volume rounded
-  // once (the FP32 stage update rounds the FP64 volume to FP32 in any case)
-  hvec<float> c_ixx32, c_iyy32, c_ixy32, c_vol32;
+  // every cell's normal matrix is well conditioned)
+  hvec<float> c_ixx32, c_iyy32, c_ixy32;
   // per block
   std::vector<int32_t> halo_ptr, halo_idx;            // halo cells (global new index)
   std::vector<int32_t> bf_ptr;                        // block face lists

@@ -145,7 +145,7 @@ int fvk_residual_l2(fvk_ctx* ctx, double out[4]);
  * their path).  No counterpart in the reference.  The same norms, bit for
  * bit: the per-block partial sums and their compensated sum go through a
  * static device array of 2 MiB per GPU (held until the process ends) instead
- * of the context's buffer and a host vector.  Meshes of at most 65,536 blocks
+ * of the context's buffers.  Meshes of at most 65,536 blocks
  * (8,388,608 cells at 128-cell blocks); larger ones get FVK_ERR_ARG.  The
  * context's device must be current.  Every call on a GPU shares that GPU's
  * array, so calls that overlap on one GPU (from several host threads) race. */

@@ -1,10 +1,9 @@
 """fvk_residual_l2_noalloc (the allocation-free residual norm) against
 fvk_residual_l2, through the C ABI.
 
-The two entry points take the same per-block partial sums and the same
-compensated sum in the same order, one through the context's buffer and a
-host vector, the other through a static device array, so their norms are
-equal bit for bit.  Checked on two calls in a row on different contexts, the
+The two entry points run the same kernels (per-block partial sums, then
+their compensated sum), one through the context's buffers, the other through
+a static device array, so their norms are equal bit for bit.  Checked on two calls in a row on different contexts, the
 second with a smaller mesh (the static array keeps the first call's larger
 contents beyond what the second reads), in both precisions, and on a mesh
 above the array's 65,536-block limit, which is refused.
