@@ -257,13 +257,13 @@ def errors_vs_fp64(q32, q64, gamma):
 PERTURBATION = 1e-7
 
 
-def perturbed_freestream(grid, case, seed: int = 7):
+def perturbed_freestream(grid, case, seed: int = 7, amplitude: float = PERTURBATION):
     """Q_inf * (1 + 1e-7 u), u uniform in [-1, 1]: an FP32-rounding-sized
     perturbation of the freestream start."""
     qinf = np.array([case.rho, case.rho * case.u, case.rho * case.v,
                      case.P / (case.gamma - 1) + 0.5 * case.rho * (case.u ** 2 + case.v ** 2)])
     u = np.random.default_rng(seed).uniform(-1.0, 1.0, (4, grid.num_cells))
-    return qinf[:, None] * (1.0 + PERTURBATION * u)
+    return qinf[:, None] * (1.0 + amplitude * u)
 
 
 def roofline(per_launch, algo, head, args, child):
@@ -484,12 +484,14 @@ def bench_single(args):
         # the FP64 scheme's own response to an FP32-rounding-sized perturbation
         log("FP64 sensitivity floor run")
         fl = Solver(grid, case)
-        fl.set_state(perturbed_freestream(grid, case))
+        fl.set_state(perturbed_freestream(grid, case, seed=args.floor_seed,
+                                          amplitude=args.floor_amplitude))
         fl.step(runs["fp64"]["convergence"]["steps"])
         floor = errors_vs_fp64(fl.state(), q64, case.gamma)
         out["fp64_sensitivity_floor"] = {k: float(f"{v:.3e}") for k, v in floor.items()}
-        out["fp64_sensitivity_floor"]["how"] = ("FP64 from Q_inf(1+1e-7 u), u in [-1,1], "
-                                                "against FP64 from Q_inf, same steps")
+        out["fp64_sensitivity_floor"]["how"] = (f"FP64 from Q_inf(1+{args.floor_amplitude:g} u), "
+                                                f"u in [-1,1] (seed {args.floor_seed}), against "
+                                                "FP64 from Q_inf, same steps")
         if case.BC1 in ("SLIP_WALL", "NO_SLIP_WALL"):
             # wall pressure and force coefficients (fv2d.analysis), FP32-mixed vs FP64,
             # and the same comparison for the perturbed FP64 run
@@ -654,7 +656,9 @@ def bench_batch(args):
             fb = Batch(cases, **(tuned["fp64"]["choice"] if tuned else opts))
             if args.stall_checks:
                 fb.set_stall(args.stall_checks, args.stall_factor)
-            fb.init([perturbed_freestream(g, c, seed=11 + i) for i, (g, c) in enumerate(cases)])
+            fb.init([perturbed_freestream(g, c, seed=args.floor_seed + 4 + i,
+                                          amplitude=args.floor_amplitude)
+                     for i, (g, c) in enumerate(cases)])
             fb.run_to_drop(args.drop, args.steps, args.check_every)
             fl_rho = []
             for i, (g, c) in enumerate(cases):
@@ -667,7 +671,8 @@ def bench_batch(args):
             out["fp64_sensitivity_floor"] = {
                 "rho_l2_rel_median": float(f"{np.median(fl_rho):.3e}"),
                 "rho_l2_rel_max": float(f"{fl_rho.max():.3e}"),
-                "how": "FP64 batch from Q_inf(1+1e-7 u) against the FP64 batch from Q_inf"}
+                "how": f"FP64 batch from Q_inf(1+{args.floor_amplitude:g} u) against the FP64 "
+                       "batch from Q_inf"}
             out["error_vs_fp64"] = {
                 "rho_l2_rel_median": float(f"{np.median(rho):.3e}"),
                 "rho_l2_rel_max": float(f"{rho.max():.3e}"),
@@ -763,6 +768,10 @@ def main():
                          "without its density residual falling below --stall-factor times "
                          "its lowest counted value (0: off)")
     ap.add_argument("--stall-factor", type=float, default=0.5)
+    ap.add_argument("--floor-amplitude", type=float, default=PERTURBATION,
+                    help="relative size of the FP64 sensitivity-floor run's start perturbation")
+    ap.add_argument("--floor-seed", type=int, default=7,
+                    help="seed of the floor run's perturbation (batch: instance i uses seed + 4 + i)")
     ap.add_argument("--reference-drop", type=float, default=0.0,
                     help="batch: also compare the converged FP32-mixed states with an FP64 "
                          "batch run to this residual drop (untimed; 0: off)")

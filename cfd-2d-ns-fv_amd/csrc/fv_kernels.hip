@@ -630,6 +630,32 @@ __device__ __forceinline__ void viscous_flux_fast(const StageArgsT<T>& a, const 
 }
 #endif
 
+// The viscous face flux Fv of face_flux (zero unless NS at second order on an
+// interior or no-slip face), in the path's arithmetic.
+template <int ORDER, bool NS, typename T>
+__device__ __forceinline__ void face_viscous(const StageArgsT<T>& a, const FlowT<T>& fl,
+                                             const CellQG<T>& c1, const CellQG<T>* c2, int kind,
+                                             const FaceGeo<T>& g, const T L[4], const T R[4],
+                                             T Fv[4]) {
+  if (NS && ORDER == 2 && (c2 || kind == 3)) {
+#if FVK_NS64
+    if constexpr (kDelta<T>) {
+      double Fd[4];
+      viscous_flux<double>(a, fl, c1, c2, g, L, R, Fd);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) Fv[k] = T(Fd[k]);
+    } else
+#elif FVK_FAST32
+    if constexpr (kDelta<T>) {
+      viscous_flux_fast(a, fl, c1, c2, g, L, R, Fv);
+    } else
+#endif
+    {
+      viscous_flux<T>(a, fl, c1, c2, g, L, R, Fv);
+    }
+  }
+}
+
 // Pass B: one face with owner c1 and neighbour c2 (interior) or a BC ghost of
 // the given kind.  x = (Fv - F)*A, the owner's residual term (the neighbour's
 // is -x, res.cpp:81-101); sterm = (|Vn_avg| + c_avg) * A (ssp_rk3.cpp:32-34).
@@ -657,28 +683,9 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
     for (int k = 0; k < 4; ++k) L[k] = c1.q[k];
   }
   T FL[4], FR[4], VnL, VnR, cL, cR;
-  // the viscous flux, taken as soon as both face states exist, so that the
+  // the viscous flux is taken as soon as both face states exist, so that the
   // cells' gradients are no longer live while the inviscid terms are formed
   T Fv[4] = {0, 0, 0, 0};
-  auto viscous = [&]() {
-    if (NS && ORDER == 2 && (c2 || kind == 3)) {
-#if FVK_NS64
-      if constexpr (kDelta<T>) {
-        double Fd[4];
-        viscous_flux<double>(a, fl, c1, c2, g, L, R, Fd);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) Fv[k] = T(Fd[k]);
-      } else
-#elif FVK_FAST32
-      if constexpr (kDelta<T>) {
-        viscous_flux_fast(a, fl, c1, c2, g, L, R, Fv);
-      } else
-#endif
-      {
-        viscous_flux<T>(a, fl, c1, c2, g, L, R, Fv);
-      }
-    }
-  };
   if constexpr (kDelta<T>) {
     SideFlux<T> sl, sr;
     side_flux_delta(fl, L, nx, ny, sl);
@@ -709,7 +716,7 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
       R[2] = -L[2] - T(2.0) * fl.qinf[2];
       R[3] = L[3];
     }
-    viscous();
+    face_viscous<ORDER, NS>(a, fl, c1, c2, kind, g, L, R, Fv);
     side_flux_delta(fl, R, nx, ny, sr);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -743,7 +750,7 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
     } else {                  // NO_SLIP_WALL
       ghost_q(gm1, pl.rho, -pl.u, -pl.v, pl.P, R);
     }
-    viscous();
+    face_viscous<ORDER, NS>(a, fl, c1, c2, kind, g, L, R, Fv);
     Prim<T> pr;
     to_prim(R, gamma, gm1, pr);
     VnL = pl.u * nx + pl.v * ny;
@@ -1510,14 +1517,11 @@ void k_pass_b_lds(StageArgsT<T> a) {
       if (j < deg) sf[j] = m.sface[base + j * 64];
     T qn[4] = {0, 0, 0, 0}, sig_old = T(1);
     double V = 1.0;
-    auto load_update_data = [&]() {
-      if (deg > 0) {
-        if (a.stage > 1) load_q(a.qn, c, qn);
-        V = m.vol[c];
-        if (!a.dt_global) sig_old = a.sigma[c];
-      }
-    };
-    if (!kLate) load_update_data();
+    if (!kLate && deg > 0) {
+      if (a.stage > 1) load_q(a.qn, c, qn);
+      V = m.vol[c];
+      if (!a.dt_global) sig_old = a.sigma[c];
+    }
     stage_block(a, b, L);
     __syncthreads();
     // one thread per face of the block
@@ -1552,7 +1556,11 @@ void k_pass_b_lds(StageArgsT<T> a) {
       load_face(m, fb + e, f);
       face(e, f);
     }
-    if (kLate) load_update_data();
+    if (kLate && deg > 0) {
+      if (a.stage > 1) load_q(a.qn, c, qn);
+      V = m.vol[c];
+      if (!a.dt_global) sig_old = a.sigma[c];
+    }
     __syncthreads();
     // one thread per cell: add the face terms in slot order (res.cpp:74-102)
     if (deg > 0) {
