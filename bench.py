@@ -54,7 +54,7 @@ import numpy as np  # noqa: E402
 from fv2d import inputs  # noqa: E402
 from fv2d import workloads as W  # noqa: E402
 from fv2d.lib import (FP32_MIXED, FP64, RETIRED_DROP, RETIRED_NONFINITE,  # noqa: E402
-                      RETIRED_STALL, Batch, Solver)
+                      RETIRED_STALL, Batch, Solver, finish_stalled)
 
 METRIC = "cell-RK-stage updates/s, FP32-mixed vs FP64 at stated error; % HBM peak"
 HBM_PEAK_GBS = 8000.0
@@ -555,6 +555,180 @@ def sequential_fp64(cases, steps, reps: int = 20):
     return total_work / total_t, total_t
 
 
+def _conv_summary(e):
+    e = np.asarray(e, dtype=np.float64)
+    return {"instances": int(e.size),
+            "rho_l2_rel_median": float(f"{np.median(e):.3e}") if e.size else None,
+            "rho_l2_rel_max": float(f"{e.max():.3e}") if e.size else None,
+            "instances_rho_l2_le_1e-5": int((e <= 1e-5).sum())}
+
+
+def batch_errors(args, cases, r, r64, fp64_opts):
+    """The FP32-mixed batch's errors against FP64: after the timed run, at
+    convergence, against an FP64 run converged further (--reference-drop),
+    and with the stalled instances finished in FP64 (--finish-fp64)."""
+    out = {}
+    why32 = r["why"]
+    errs, bad32, bad64, conv = [], 0, 0, []
+    # instances converged in both precisions: FP64 retired by the residual
+    # drop, FP32-mixed by the drop or, with stall retirement, by a stalled
+    # residual (its state's rounding stops the residual above the drop)
+    conv_why = (RETIRED_DROP, RETIRED_STALL) if args.stall_checks else (RETIRED_DROP,)
+    reached = np.isin(why32, conv_why) & (r64["why"] == RETIRED_DROP)
+    q64s = [r64["batch"].state(i) for i in range(len(cases))]
+    for i, (g, c) in enumerate(cases):
+        q32 = r["batch"].state(i)
+        f32, f64 = bool(np.isfinite(q32).all()), bool(np.isfinite(q64s[i]).all())
+        bad32 += not f32
+        bad64 += not f64
+        if f32 and f64:
+            errs.append(errors_vs_fp64(q32, q64s[i], c.gamma))
+            if reached[i]:
+                conv.append(errs[-1]["rho_l2_rel"])
+    rho = np.array([e["rho_l2_rel"] for e in errs])
+    pl2 = np.array([e["p_l2_rel"] for e in errs])
+    # the FP64 scheme's own response to an FP32-rounding-sized perturbation
+    log("FP64 sensitivity floor batch")
+    fb = Batch(cases, **fp64_opts)
+    if args.stall_checks:
+        fb.set_stall(args.stall_checks, args.stall_factor)
+    fb.init([perturbed_freestream(g, c, seed=args.floor_seed + 4 + i,
+                                  amplitude=args.floor_amplitude)
+             for i, (g, c) in enumerate(cases)])
+    fb.run_to_drop(args.drop, args.steps, args.check_every)
+    fl_rho = []
+    for i, (g, c) in enumerate(cases):
+        qf = fb.state(i)
+        if np.isfinite(q64s[i]).all() and np.isfinite(qf).all():
+            fl_rho.append(errors_vs_fp64(qf, q64s[i], c.gamma)["rho_l2_rel"])
+    fb.close()
+    fl_rho = np.array(fl_rho)
+    out["fp64_sensitivity_floor"] = {
+        "rho_l2_rel_median": float(f"{np.median(fl_rho):.3e}"),
+        "rho_l2_rel_max": float(f"{fl_rho.max():.3e}"),
+        "how": f"FP64 batch from Q_inf(1+{args.floor_amplitude:g} u) against the FP64 "
+               "batch from Q_inf"}
+    out["error_vs_fp64"] = {
+        "rho_l2_rel_median": float(f"{np.median(rho):.3e}"),
+        "rho_l2_rel_max": float(f"{rho.max():.3e}"),
+        "p_l2_rel_median": float(f"{np.median(pl2):.3e}"),
+        "p_l2_rel_max": float(f"{pl2.max():.3e}"),
+        "instances_rho_l2_le_1e-5": int((rho <= 1e-5).sum()),
+        "instances_compared": len(errs), "nonfinite_fp32": bad32, "nonfinite_fp64": bad64,
+        "compared": "each instance's final FP32-mixed state vs its final FP64 state"}
+    out["error_at_convergence"] = {
+        **_conv_summary(conv),
+        "compared": f"instances whose FP64 density residual fell to {args.drop:g} of "
+                    "its first-step value and whose FP32-mixed residual did too"
+                    + (" or stalled" if args.stall_checks else "")
+                    + "; each at its own final step"}
+
+    # the stalled instances continued in FP64 from their FP32-mixed states
+    fin = None
+    if args.finish_fp64:
+        log("FP64 finish of the stalled FP32-mixed instances")
+        pb = Batch(cases, **fp64_opts)
+        if args.stall_checks:
+            pb.set_stall(args.stall_checks, args.stall_factor)
+        t0 = time.perf_counter()
+        # the timed run's retirement codes (time_kernels re-activated the batch)
+        finished, fsteps, _, fwhy = finish_stalled(r["batch"], pb, r["r0"], args.drop,
+                                                   args.steps, args.check_every,
+                                                   which=why32 == RETIRED_STALL)
+        fwall = time.perf_counter() - t0
+        nc = np.array([g.num_cells for g, _ in cases], dtype=np.float64)
+        fwork = float((nc * 3 * fsteps).sum())
+        value = (r["work"] + fwork) / (r["wall"] + fwall)
+        why_fin = np.where(finished, fwhy, why32)
+        fin = {"batch": pb, "finished": finished, "why": why_fin}
+        conv_f = []
+        reached_f = (why_fin == RETIRED_DROP) & (r64["why"] == RETIRED_DROP)
+        for i, (g, c) in enumerate(cases):
+            if reached_f[i]:
+                q = pb.state(i) if finished[i] else r["batch"].state(i)
+                conv_f.append(errors_vs_fp64(q, q64s[i], c.gamma)["rho_l2_rel"])
+        out["with_fp64_finish"] = {
+            "value": value, "unit": "cell-stage/s",
+            "speedup_vs_fp64_batched": round(value / r64["value"], 4),
+            "wall_s": round(r["wall"] + fwall, 3), "finish_wall_s": round(fwall, 3),
+            "instances_finished": int(finished.sum()),
+            "finish_steps_median_max": ([int(np.median(fsteps[finished])),
+                                         int(fsteps[finished].max())]
+                                        if finished.any() else [0, 0]),
+            "finish_cell_stage_share": round(fwork / (r["work"] + fwork), 4),
+            "retired": {"drop": int((why_fin == RETIRED_DROP).sum()),
+                        "stall": int((why_fin == RETIRED_STALL).sum()),
+                        "nonfinite": int((why_fin == RETIRED_NONFINITE).sum())},
+            "error_at_convergence": {
+                **_conv_summary(conv_f),
+                "compared": f"instances whose FP64 density residual fell to {args.drop:g} of "
+                            "its first-step value and whose FP32-mixed run, with the stalled "
+                            "instances continued in FP64 from their FP32-mixed states, did "
+                            "too; each at its own final step"},
+            "how": "FP32-mixed batch with stall retirement, then an FP64 batch continuing "
+                   "the stalled instances to the same drop of the same reference norms "
+                   "(fv2d.lib.finish_stalled); value = all cell-stages of both runs over "
+                   "both runs' wall time (the finish's state hand-over included)"}
+
+    if args.reference_drop:
+        # two states that both meet a residual drop still differ along the
+        # slowly converging modes by about drop / (their decay rate), so
+        # compare the converged FP32-mixed states with an FP64 reference
+        # converged much further (untimed)
+        log(f"FP64 reference batch to a {args.reference_drop:g} drop")
+        rb = Batch(cases, **fp64_opts)
+        if args.stall_checks:
+            rb.set_stall(args.stall_checks, args.stall_factor)
+        rb.init()
+        rsteps, _, _ = rb.run_to_drop(args.reference_drop, args.reference_steps,
+                                      args.check_every)
+        rwhy = rb.retired()
+        ok64 = np.isin(rwhy, (RETIRED_DROP, RETIRED_STALL))
+        qref = [rb.state(i) if ok64[i] else None for i in range(len(cases))]
+        rb.close()
+
+        def versus_reference(why, state):
+            ok = np.isin(why, conv_why) & ok64
+            err, wref = [], []
+            for i, (g, c) in enumerate(cases):
+                if ok[i]:
+                    err.append(errors_vs_fp64(state(i), qref[i], c.gamma)["rho_l2_rel"])
+                    wref.append(rwhy[i])
+            err, wref = np.array(err), np.array(wref)
+            return {**_conv_summary(err),
+                    "converged": int(np.isin(why, conv_why).sum()),
+                    "where_fp64_reference_reached_the_drop": _conv_summary(
+                        err[wref == RETIRED_DROP] if err.size else err),
+                    "where_fp64_reference_stalled": _conv_summary(
+                        err[wref == RETIRED_STALL] if err.size else err)}
+
+        out["error_at_convergence_vs_converged_fp64"] = {
+            **versus_reference(why32, r["batch"].state),
+            "fp64_reference_converged": int(ok64.sum()),
+            "fp64_reference_retired": {
+                "drop": int((rwhy == RETIRED_DROP).sum()),
+                "stall": int((rwhy == RETIRED_STALL).sum())},
+            "fp64_reference_steps_median_max": [int(np.median(rsteps)), int(rsteps.max())],
+            "compared": "each FP32-mixed state at its retirement (drop"
+                        + (" or stall" if args.stall_checks else "")
+                        + f") against FP64 run to a {args.reference_drop:g} drop"
+                        + (" or its stall" if args.stall_checks else "")}
+        # the stopping rule's own share: the FP64 batch at the drop against the
+        # same reference
+        out["error_at_convergence_vs_converged_fp64"]["fp64_batch_at_the_drop"] = \
+            versus_reference(r64["why"], r64["batch"].state)
+        if fin is not None:
+            pb = fin["batch"]
+            out["with_fp64_finish"]["error_at_convergence_vs_converged_fp64"] = {
+                **versus_reference(fin["why"], lambda i: (pb.state(i) if fin["finished"][i]
+                                                          else r["batch"].state(i))),
+                "compared": "each final state (FP32-mixed, or FP64 where the FP32-mixed run "
+                            "stalled) against the same FP64 reference"}
+    if fin is not None:
+        fin["batch"].close()
+    return out
+
+
 def bench_batch(args):
     name, plan = W.batch(args.instances)
     cases = [(i.grid, i.case) for i in plan]
@@ -634,109 +808,8 @@ def bench_batch(args):
                 out["speedup_vs_fp64_sequential"] = round(r["value"] / seq, 4)
         if head == "fp32":
             out["speedup_vs_fp64_batched"] = round(r["value"] / r64["value"], 4)
-            errs, bad32, bad64, conv = [], 0, 0, []
-            # instances converged in both precisions: FP64 retired by the residual
-            # drop, FP32-mixed by the drop or, with stall retirement, by a stalled
-            # residual (its state's rounding stops the residual above the drop)
-            conv_why = (RETIRED_DROP, RETIRED_STALL) if args.stall_checks else (RETIRED_DROP,)
-            reached = np.isin(r["why"], conv_why) & (r64["why"] == RETIRED_DROP)
-            for i, (g, c) in enumerate(cases):
-                q32, q64 = r["batch"].state(i), r64["batch"].state(i)
-                f32, f64 = bool(np.isfinite(q32).all()), bool(np.isfinite(q64).all())
-                bad32 += not f32
-                bad64 += not f64
-                if f32 and f64:
-                    errs.append(errors_vs_fp64(q32, q64, c.gamma))
-                    if reached[i]:
-                        conv.append(errs[-1]["rho_l2_rel"])
-            rho = np.array([e["rho_l2_rel"] for e in errs])
-            pl2 = np.array([e["p_l2_rel"] for e in errs])
-            # the FP64 scheme's own response to an FP32-rounding-sized perturbation
-            log("FP64 sensitivity floor batch")
-            fb = Batch(cases, **(tuned["fp64"]["choice"] if tuned else opts))
-            if args.stall_checks:
-                fb.set_stall(args.stall_checks, args.stall_factor)
-            fb.init([perturbed_freestream(g, c, seed=args.floor_seed + 4 + i,
-                                          amplitude=args.floor_amplitude)
-                     for i, (g, c) in enumerate(cases)])
-            fb.run_to_drop(args.drop, args.steps, args.check_every)
-            fl_rho = []
-            for i, (g, c) in enumerate(cases):
-                q64 = r64["batch"].state(i)
-                qf = fb.state(i)
-                if np.isfinite(q64).all() and np.isfinite(qf).all():
-                    fl_rho.append(errors_vs_fp64(qf, q64, c.gamma)["rho_l2_rel"])
-            fb.close()
-            fl_rho = np.array(fl_rho)
-            out["fp64_sensitivity_floor"] = {
-                "rho_l2_rel_median": float(f"{np.median(fl_rho):.3e}"),
-                "rho_l2_rel_max": float(f"{fl_rho.max():.3e}"),
-                "how": f"FP64 batch from Q_inf(1+{args.floor_amplitude:g} u) against the FP64 "
-                       "batch from Q_inf"}
-            out["error_vs_fp64"] = {
-                "rho_l2_rel_median": float(f"{np.median(rho):.3e}"),
-                "rho_l2_rel_max": float(f"{rho.max():.3e}"),
-                "p_l2_rel_median": float(f"{np.median(pl2):.3e}"),
-                "p_l2_rel_max": float(f"{pl2.max():.3e}"),
-                "instances_rho_l2_le_1e-5": int((rho <= 1e-5).sum()),
-                "instances_compared": len(errs), "nonfinite_fp32": bad32, "nonfinite_fp64": bad64,
-                "compared": "each instance's final FP32-mixed state vs its final FP64 state"}
-            conv = np.array(conv)
-            out["error_at_convergence"] = {
-                "instances": int(conv.size),
-                "rho_l2_rel_median": float(f"{np.median(conv):.3e}") if conv.size else None,
-                "rho_l2_rel_max": float(f"{conv.max():.3e}") if conv.size else None,
-                "instances_rho_l2_le_1e-5": int((conv <= 1e-5).sum()),
-                "compared": f"instances whose FP64 density residual fell to {args.drop:g} of "
-                            "its first-step value and whose FP32-mixed residual did too"
-                            + (" or stalled" if args.stall_checks else "")
-                            + "; each at its own final step"}
-            if args.reference_drop:
-                # two states that both meet a residual drop still differ along the
-                # slowly converging modes by about drop / (their decay rate), so
-                # compare the converged FP32-mixed states with an FP64 reference
-                # converged much further (untimed)
-                log(f"FP64 reference batch to a {args.reference_drop:g} drop")
-                rb = Batch(cases, **(tuned["fp64"]["choice"] if tuned else opts))
-                if args.stall_checks:
-                    rb.set_stall(args.stall_checks, args.stall_factor)
-                rb.init()
-                rsteps, _, _ = rb.run_to_drop(args.reference_drop, args.reference_steps,
-                                              args.check_every)
-                rwhy = rb.retired()
-                ok32 = np.isin(r["why"], conv_why)
-                ok64 = np.isin(rwhy, (RETIRED_DROP, RETIRED_STALL))
-                ref_err, ref_why = [], []
-                for i, (g, c) in enumerate(cases):
-                    if ok32[i] and ok64[i]:
-                        ref_err.append(errors_vs_fp64(r["batch"].state(i), rb.state(i),
-                                                      c.gamma)["rho_l2_rel"])
-                        ref_why.append(rwhy[i])
-                rb.close()
-                ref_err, ref_why = np.array(ref_err), np.array(ref_why)
-
-                def summary(e):
-                    return {"instances": int(e.size),
-                            "rho_l2_rel_median": float(f"{np.median(e):.3e}") if e.size else None,
-                            "rho_l2_rel_max": float(f"{e.max():.3e}") if e.size else None,
-                            "instances_rho_l2_le_1e-5": int((e <= 1e-5).sum())}
-                out["error_at_convergence_vs_converged_fp64"] = {
-                    "instances": int(ref_err.size),
-                    "fp32_converged": int(ok32.sum()),
-                    "fp64_reference_converged": int(ok64.sum()),
-                    "fp64_reference_retired": {
-                        "drop": int((rwhy == RETIRED_DROP).sum()),
-                        "stall": int((rwhy == RETIRED_STALL).sum())},
-                    "fp64_reference_steps_median_max": [int(np.median(rsteps)), int(rsteps.max())],
-                    "rho_l2_rel_median": float(f"{np.median(ref_err):.3e}") if ref_err.size else None,
-                    "rho_l2_rel_max": float(f"{ref_err.max():.3e}") if ref_err.size else None,
-                    "instances_rho_l2_le_1e-5": int((ref_err <= 1e-5).sum()),
-                    "where_fp64_reference_reached_the_drop": summary(ref_err[ref_why == RETIRED_DROP]),
-                    "where_fp64_reference_stalled": summary(ref_err[ref_why == RETIRED_STALL]),
-                    "compared": "each FP32-mixed state at its retirement (drop"
-                                + (" or stall" if args.stall_checks else "")
-                                + f") against FP64 run to a {args.reference_drop:g} drop"
-                                + (" or its stall" if args.stall_checks else "")}
+            fp64_opts = tuned["fp64"]["choice"] if tuned else opts
+            out.update(batch_errors(args, cases, r, r64, fp64_opts))
     if not args.no_cpu:
         # a bounded sample: the smallest, the median and the largest instance
         order = np.argsort(nc)
@@ -772,6 +845,10 @@ def main():
                     help="relative size of the FP64 sensitivity-floor run's start perturbation")
     ap.add_argument("--floor-seed", type=int, default=7,
                     help="seed of the floor run's perturbation (batch: instance i uses seed + 4 + i)")
+    ap.add_argument("--finish-fp64", action="store_true",
+                    help="batch: continue the instances the FP32-mixed run retired by a stall "
+                         "in FP64 from their states, to the same drop (timed; reported "
+                         "beside the FP32-mixed run)")
     ap.add_argument("--reference-drop", type=float, default=0.0,
                     help="batch: also compare the converged FP32-mixed states with an FP64 "
                          "batch run to this residual drop (untimed; 0: off)")

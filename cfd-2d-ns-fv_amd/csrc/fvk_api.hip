@@ -54,6 +54,7 @@ struct Instance {
   int32_t why = 0;              // FVK_RETIRED_*: why the instance is not active
   double best = 0.0;            // run_to_drop: lowest density residual norm that counted
   int32_t since_best = 0;       // run_to_drop: checks since `best` was set
+  double ref0 = 0.0;            // run_to_drop: caller's reference norm (> 0), else the first step's
 };
 
 struct Engine {
@@ -1374,9 +1375,8 @@ int fvk_batch_init(fvk_batch* b, const double* const* q) {
   Engine* c = &b->e;
   size_t given = 0;
   for (size_t i = 0; q && i < c->in.size(); ++i) given += q[i] != nullptr;
-  if (given != 0 && given != c->in.size())
-    return set_err(c, FVK_ERR_ARG, "fvk_batch_init: give every instance a state, or none");
   for (size_t i = 0; i < c->in.size(); ++i) {
+    // an instance without a state starts from its freestream
     int rc = put_state(c, i, given ? q[i] : nullptr);
     if (rc) return rc;
     c->in[i].active = true;
@@ -1409,6 +1409,15 @@ int fvk_batch_set_stall(fvk_batch* b, int32_t checks, double factor) {
     return set_err(&b->e, FVK_ERR_ARG, "fvk_batch_set_stall: checks >= 0 and 0 < factor < 1");
   b->e.stall_checks = checks;
   b->e.stall_factor = factor;
+  return FVK_OK;
+}
+
+int fvk_batch_set_reference(fvk_batch* b, const double* res0) {
+  if (!b) return FVK_ERR_ARG;
+  for (size_t i = 0; i < b->e.in.size(); ++i) {
+    const double r = res0 ? res0[i] : 0.0;
+    b->e.in[i].ref0 = (std::isfinite(r) && r > 0.0) ? r : 0.0;
+  }
   return FVK_OK;
 }
 
@@ -1453,7 +1462,7 @@ int fvk_batch_run_to_drop(fvk_batch* b, double drop, int32_t max_steps, int32_t 
       if (!in.active) continue;
       const double rho_res = r[4 * i];
       if (r0[i] < 0) {
-        r0[i] = rho_res;
+        r0[i] = in.ref0 > 0.0 ? in.ref0 : rho_res;
         in.best = rho_res;
       }
       if (res_end) res_end[i] = rho_res;

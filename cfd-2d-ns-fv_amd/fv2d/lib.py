@@ -35,6 +35,7 @@ EXPORTS = ["fvk_default_options", "fvk_create", "fvk_destroy", "fvk_last_error",
            "fvk_batch_create", "fvk_batch_destroy", "fvk_batch_last_error", "fvk_batch_init",
            "fvk_batch_step", "fvk_batch_set_active", "fvk_batch_residual_l2",
            "fvk_batch_conservation_sums", "fvk_batch_set_stall", "fvk_batch_retired",
+           "fvk_batch_set_reference",
            "fvk_batch_run_to_drop", "fvk_batch_get_state", "fvk_batch_get_info",
            "fvk_batch_get_time", "fvk_batch_get_options", "fvk_batch_use_graph",
            "fvk_batch_time_kernels",
@@ -117,6 +118,7 @@ def lib():
         L.fvk_batch_conservation_sums.argtypes = [vp, dp]
         L.fvk_batch_set_stall.argtypes = [vp, ctypes.c_int32, ctypes.c_double]
         L.fvk_batch_retired.argtypes = [vp, ip]
+        L.fvk_batch_set_reference.argtypes = [vp, dp]
         L.fvk_batch_run_to_drop.argtypes = [vp, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                                             ip, dp, dp]
         L.fvk_batch_get_state.argtypes = [vp, ctypes.c_int32, dp, dp]
@@ -352,12 +354,20 @@ class Batch:
         self.close()
 
     def init(self, states=None):
-        """Freestream start (states None) or one conserved state per instance."""
+        """Freestream start (states None) or one conserved state per instance
+        (an entry None: that instance's freestream start)."""
         if states is None:
             self._check(lib().fvk_batch_init(self.h, None))
             return
-        keep = [np.ascontiguousarray(q, dtype=np.float64).reshape(-1) for q in states]
-        arr = (ctypes.POINTER(ctypes.c_double) * self.n)(*[_dp(q) for q in keep])
+        if len(states) != self.n:
+            raise ValueError(f"init: {len(states)} states for {self.n} instances")
+        keep = [None if q is None else np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+                for q in states]
+        for i, q in enumerate(keep):
+            if q is not None and q.size != 4 * self.nc[i]:
+                raise ValueError(f"init: state {i} has {q.size} values, not 4 x {self.nc[i]}")
+        arr = (ctypes.POINTER(ctypes.c_double) * self.n)(
+            *[ctypes.POINTER(ctypes.c_double)() if q is None else _dp(q) for q in keep])
         self._check(lib().fvk_batch_init(self.h, arr))
 
     def step(self, n: int = 1):
@@ -380,6 +390,17 @@ class Batch:
     def set_stall(self, checks: int, factor: float = 0.5):
         """Stall retirement in run_to_drop (checks = 0: off)."""
         self._check(lib().fvk_batch_set_stall(self.h, checks, factor))
+
+    def set_reference(self, res0=None):
+        """Reference density residual norms for the following run_to_drop
+        calls (None, or an entry <= 0: the first step's norm)."""
+        if res0 is None:
+            self._check(lib().fvk_batch_set_reference(self.h, None))
+            return
+        r = np.ascontiguousarray(np.asarray(res0, dtype=np.float64)).reshape(-1)
+        if r.size != self.n:
+            raise ValueError(f"set_reference: {r.size} norms for {self.n} instances")
+        self._check(lib().fvk_batch_set_reference(self.h, _dp(r)))
 
     def retired(self) -> np.ndarray:
         """Per instance: RETIRED_* code (0 active, 1 drop, 2 non-finite,
@@ -451,3 +472,30 @@ class Batch:
         out = np.zeros(2)
         self._check(lib().fvk_batch_time_kernels(self.h, reps, _dp(out)))
         return {"pass_a": float(out[0]), "pass_b": float(out[1])}
+
+
+def finish_stalled(src: Batch, dst: Batch, res0, drop: float, max_steps: int,
+                   check_every: int, which=None):
+    """Continue, in `dst` (a batch of the same instances, typically FP64),
+    the instances that `src` (typically FP32-mixed) retired by a stall
+    (or those `which` selects), from their states in `src`, to the same
+    density-residual drop measured from `src`'s reference norms `res0`.
+    The other instances of `dst` start from their freestream and stay
+    inactive.  `which` is needed once `src.set_active` has reset the
+    retirement codes.
+
+    Returns (finished, steps, res_end, why): the boolean mask of the
+    instances continued, and dst.run_to_drop's steps and last norms and
+    dst's retirement codes (RETIRED_CALLER for the instances not continued).
+    """
+    if dst.n != src.n or list(dst.nc) != list(src.nc):
+        raise ValueError("finish_stalled: the two batches hold different instances")
+    finished = (src.retired() == RETIRED_STALL) if which is None else \
+        np.asarray(which, dtype=bool).reshape(-1)
+    if finished.size != src.n:
+        raise ValueError(f"finish_stalled: {finished.size} flags for {src.n} instances")
+    dst.init([src.state(i) if finished[i] else None for i in range(src.n)])
+    dst.set_active(finished)
+    dst.set_reference(np.where(finished, np.asarray(res0, dtype=np.float64), 0.0))
+    steps, _, r1 = dst.run_to_drop(drop, max_steps, check_every)
+    return finished, steps, r1, dst.retired()

@@ -227,9 +227,10 @@ int fvk_batch_destroy(fvk_batch* b);
 /* message of the last failing call on b (or of the last fvk_batch_create
  * when b is NULL) */
 const char* fvk_batch_last_error(const fvk_batch* b);
-/* Start every instance: q == NULL, or every q[i] NULL, for the freestream
- * start of fvk_init_freestream; else q[i] = instance i's conserved state as
- * for fvk_set_state.  All instances become active. */
+/* Start every instance: q == NULL, or q[i] == NULL, for instance i's
+ * freestream start of fvk_init_freestream; else q[i] = instance i's
+ * conserved state as for fvk_set_state (a state taken from a batch of the
+ * other precision included).  All instances become active. */
 int fvk_batch_init(fvk_batch* b, const double* const* q);
 /* nsteps steps of every active instance */
 int fvk_batch_step(fvk_batch* b, int32_t nsteps);
@@ -258,6 +259,14 @@ int fvk_batch_run_to_drop(fvk_batch* b, double drop, int32_t max_steps, int32_t 
  * perturbation state carries a residual of about its rounding).  No
  * counterpart in the reference, which runs a fixed nmax. */
 int fvk_batch_set_stall(fvk_batch* b, int32_t checks, double factor);
+/* Reference norms for the following fvk_batch_run_to_drop calls: res0[i] > 0
+ * replaces instance i's first-step norm as the norm its drop is measured
+ * from (and is returned as its res0); res0 == NULL, or res0[i] <= 0, goes
+ * back to the first-step norm.  This lets a run that continues another
+ * one, such as an FP64 batch finishing the instances an FP32-mixed batch
+ * retired by a stall (their states passed to fvk_batch_init), keep the
+ * first run's target.  No counterpart in the reference. */
+int fvk_batch_set_reference(fvk_batch* b, const double* res0);
 /* why[i] = why instance i is not active: one of FVK_RETIRED_* */
 #define FVK_RETIRED_NONE 0       /* active */
 #define FVK_RETIRED_DROP 1       /* residual norm fell to the drop */

@@ -9,7 +9,8 @@ mesh and options -- which for FP64 is the reference's own result
 switches and BC sets in one batch; GLOBAL and LOCAL dt; Euler and NS; FP64
 and FP32-mixed; graph replay against step-by-step launches; retirement
 (retired instances keep their state, the others go on unchanged);
-run-to-residual-drop; option checks.
+run-to-residual-drop; stall retirement; the FP64 finish of stalled
+FP32-mixed instances; option checks.
 """
 import numpy as np
 import pytest
@@ -249,3 +250,66 @@ def test_batch_stall_retirement_and_error_at_convergence():
         q32, q64 = final[FP32_MIXED][i], final[0][i]
         err = float(np.linalg.norm(q32[0] - q64[0]) / np.linalg.norm(q64[0]))
         assert err <= 1e-5, (i, err)
+
+
+def test_finish_stalled_in_fp64():
+    """The FP32-mixed batch retires first-order instances by a stall above an
+    FP64-reachable drop; finish_stalled continues exactly those instances in
+    an FP64 batch from their FP32-mixed states, measured from the FP32-mixed
+    run's reference norms, to the drop.  Each continued instance equals a
+    single FP64 context started from the same state and run the same steps
+    (bit for bit), its norm is within the drop of the FP32-mixed reference,
+    and its density is within 1e-5 (relative L2) of the FP64 batch's own
+    converged density."""
+    from fv2d.lib import RETIRED_CALLER, RETIRED_DROP, RETIRED_STALL, finish_stalled
+    u, v = inputs.mach_velocity(0.5, angle_deg=1.0)
+    u2, v2 = inputs.mach_velocity(2.2, angle_deg=2.0)
+    cases = [(gg.bump(64, 32), inputs.Case(u=u, v=v, poly=0, dt_type="LOCAL", CFL=0.5)),
+             (gg.ramp(48, 24), inputs.Case(u=u2, v=v2, poly=0, dt_type="LOCAL", CFL=0.5)),
+             (gg.bump(32, 16), inputs.Case(u=u, v=v, poly=0, dt_type="LOCAL", CFL=0.5))]
+    drop = 1e-9
+    b32 = Batch(cases, precision=FP32_MIXED)
+    b32.set_stall(5, 0.5)
+    b32.init()
+    b32.set_active([1, 1, 0])           # the third instance stays out (retired by the caller)
+    s32, r0, r1 = b32.run_to_drop(drop, 40000, 200)
+    why32 = b32.retired()
+    print(f"[gpu] finish: FP32-mixed steps {s32.tolist()} drop {(r1 / r0).tolist()} "
+          f"why {why32.tolist()}", flush=True)
+    assert (why32[:2] == RETIRED_STALL).all() and why32[2] == RETIRED_CALLER, why32
+    q32 = [b32.state(i) for i in range(2)]
+
+    b64 = Batch(cases)
+    b64.set_stall(5, 0.5)
+    finished, steps, res_end, why = finish_stalled(b32, b64, r0, drop, 40000, 200)
+    print(f"[gpu] finish: FP64 steps {steps.tolist()} drop {(res_end / r0).tolist()} "
+          f"why {why.tolist()}", flush=True)
+    assert finished.tolist() == [True, True, False]
+    assert (why[:2] == RETIRED_DROP).all() and why[2] == RETIRED_CALLER, why
+    assert (res_end[:2] <= drop * r0[:2]).all()
+    assert steps[2] == 0
+    ref = Batch(cases[:2])
+    ref.init()
+    ref.run_to_drop(drop, 40000, 200)
+    for i in range(2):
+        s = Solver(*cases[i])
+        s.set_state(q32[i])
+        s.step(int(steps[i]))
+        assert same(b64.state(i), s.state()), f"instance {i}"
+        assert np.isclose(s.residual_l2()[0], res_end[i], rtol=1e-12, atol=0)
+        q64 = ref.state(i)
+        err = float(np.linalg.norm(b64.state(i)[0] - q64[0]) / np.linalg.norm(q64[0]))
+        assert err <= 1e-5, (i, err)
+    # the third instance was not continued: freestream state, untouched
+    s = Solver(*cases[2])
+    s.init_freestream()
+    assert same(b64.state(2), s.state())
+    # a reference norm > 0 replaces the first step's; <= 0 or None restores it
+    b64.set_reference([5.0, 0.0, -1.0])
+    b64.init()
+    _, r0b, r1b = b64.run_to_drop(1e-2, 1, 1)
+    assert r0b[0] == 5.0 and r0b[1] == r1b[1] and r0b[2] == r1b[2]
+    b64.set_reference(None)
+    b64.init()
+    _, r0c, r1c = b64.run_to_drop(1e-2, 1, 1)
+    assert (r0c == r1c).all()
