@@ -630,6 +630,41 @@ __device__ __forceinline__ void viscous_flux_fast(const StageArgsT<T>& a, const 
 }
 #endif
 
+// F(Q).n of one face side from its primitives (rusanov.cpp:20-31): the
+// normal velocity Vn and the four flux components
+template <typename T>
+__device__ __forceinline__ void side_flux(const Prim<T>& p, T nx, T ny, T F[4], T& Vn) {
+  Vn = p.u * nx + p.v * ny;
+  F[0] = p.rho * Vn;
+  F[1] = p.rho * p.u * Vn + p.P * nx;
+  F[2] = p.rho * p.v * Vn + p.P * ny;
+  F[3] = Vn * (p.E + p.P);
+}
+
+// The Rusanov flux of the two sides and the face's residual terms
+// (rusanov.cpp:32-37, res.cpp:81-101, ssp_rk3.cpp:32-34): x = (Fv - F)*A,
+// the owner's term, and sterm = (|Vn_avg| + c_avg)*A.
+template <typename T>
+__device__ __forceinline__ void rusanov_combine(const T L[4], const T R[4], const T FL[4],
+                                                const T FR[4], T VnL, T VnR, T cL, T cR,
+                                                const T Fv[4], T A, T x[4], T& sterm) {
+  const T vn_avg = (VnL + VnR) / 2;
+  const T c_avg = (cL + cR) / 2;
+  const T lam = (fabs(vn_avg) + c_avg) / 2;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const T F = (FL[k] + FR[k]) / 2 - lam * (R[k] - L[k]);
+#if FVK_FAST32
+    if constexpr (kDelta<T>) {
+      x[k] = (Fv[k] - F) * A;
+      continue;
+    }
+#endif
+    x[k] = (Fv[k] * A) - (F * A);
+  }
+  sterm = (fabs(vn_avg) + c_avg) * A;
+}
+
 // The viscous face flux Fv of face_flux (zero unless NS at second order on an
 // interior or no-slip face), in the path's arithmetic.
 template <int ORDER, bool NS, typename T>
@@ -753,34 +788,12 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
     face_viscous<ORDER, NS>(a, fl, c1, c2, kind, g, L, R, Fv);
     Prim<T> pr;
     to_prim(R, gamma, gm1, pr);
-    VnL = pl.u * nx + pl.v * ny;
-    VnR = pr.u * nx + pr.v * ny;
-    FL[0] = pl.rho * VnL;
-    FL[1] = pl.rho * pl.u * VnL + pl.P * nx;
-    FL[2] = pl.rho * pl.v * VnL + pl.P * ny;
-    FL[3] = VnL * (pl.E + pl.P);
-    FR[0] = pr.rho * VnR;
-    FR[1] = pr.rho * pr.u * VnR + pr.P * nx;
-    FR[2] = pr.rho * pr.v * VnR + pr.P * ny;
-    FR[3] = VnR * (pr.E + pr.P);
+    side_flux(pl, nx, ny, FL, VnL);
+    side_flux(pr, nx, ny, FR, VnR);
     cL = pl.c;
     cR = pr.c;
   }
-  const T vn_avg = (VnL + VnR) / 2;
-  const T c_avg = (cL + cR) / 2;
-  const T lam = (fabs(vn_avg) + c_avg) / 2;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const T F = (FL[k] + FR[k]) / 2 - lam * (R[k] - L[k]);
-#if FVK_FAST32
-    if constexpr (kDelta<T>) {
-      x[k] = (Fv[k] - F) * A;
-      continue;
-    }
-#endif
-    x[k] = (Fv[k] * A) - (F * A);
-  }
-  sterm = (fabs(vn_avg) + c_avg) * A;
+  rusanov_combine(L, R, FL, FR, VnL, VnR, cL, cR, Fv, A, x, sterm);
 }
 
 // Residual / volume, dt, SSP-RK3 stage combination and stores of one cell.
@@ -1526,9 +1539,9 @@ void k_pass_b_lds(StageArgsT<T> a) {
     __syncthreads();
     // one thread per face of the block
     auto face = [&](int e, const FaceRec<T, NS>& f) {
+      T x[4], st;
       CellQG<T> c1, c2;
       lds_cell<T, ORDER>(L, f.ia, c1);
-      T x[4], st;
       if (f.ib >= 0) {
         lds_cell<T, ORDER>(L, f.ib, c2);
         FaceGeo<T> g;
