@@ -665,6 +665,69 @@ __device__ __forceinline__ void rusanov_combine(const T L[4], const T R[4], cons
   sterm = (fabs(vn_avg) + c_avg) * A;
 }
 
+// Roe's flux difference |A~|(Q_R - Q_L) across a face of unit normal n (P. L.
+// Roe, J. Comput. Phys. 43, 1981: density-weighted averages of u, v and H;
+// split into the two acoustic waves, the entropy wave and the shear wave,
+// as in Blazek's textbook treatment of the scheme), from the two face
+// states' primitives and total energies and their differences (dr, du, dv,
+// dp).  The acoustic eigenvalues take Harten's entropy fix with delta =
+// c~/10.  The reference reads ROE (read_inputs.cpp:89-91) but its
+// roe_flux.cpp is an unfinished draft outside its solver library
+// (res.cpp:66-68 leaves the flux at -101), so this is an extension.
+template <typename T>
+__device__ __forceinline__ void roe_dissipation(T rL, T uL, T vL, T PL, T EL, T rR, T uR, T vR,
+                                                T PR, T ER, T dr, T du, T dv, T dp, T nx, T ny,
+                                                T gm1, T D[4]) {
+  const T sL = sqrt(rL), sR = sqrt(rR);
+  const T w = sL + sR;
+  const T HL = (EL + PL) / rL;
+  const T HR = (ER + PR) / rR;
+  const T rt = sL * sR;                          // Roe averages
+  const T ut = (sL * uL + sR * uR) / w;
+  const T vt = (sL * vL + sR * vR) / w;
+  const T Ht = (sL * HL + sR * HR) / w;
+  const T kt = (ut * ut + vt * vt) / 2;
+  const T ct = sqrt(gm1 * (Ht - kt));
+  const T Vt = ut * nx + vt * ny;
+  const T dV = du * nx + dv * ny;
+  const T del = ct / 10;
+  T l1 = fabs(Vt - ct), l5 = fabs(Vt + ct);
+  const T l2 = fabs(Vt);
+  if (l1 < del) l1 = (l1 * l1 + del * del) / (2 * del);
+  if (l5 < del) l5 = (l5 * l5 + del * del) / (2 * del);
+  const T c2 = ct * ct;
+  const T a1 = l1 * (dp - rt * ct * dV) / (2 * c2);   // acoustic wave Vn - c
+  const T a2 = l2 * (dr - dp / c2);                   // entropy wave
+  const T a3 = l2 * rt;                               // shear wave
+  const T a5 = l5 * (dp + rt * ct * dV) / (2 * c2);   // acoustic wave Vn + c
+  D[0] = a1 + a2 + a5;
+  D[1] = a1 * (ut - ct * nx) + a2 * ut + a3 * (du - dV * nx) + a5 * (ut + ct * nx);
+  D[2] = a1 * (vt - ct * ny) + a2 * vt + a3 * (dv - dV * ny) + a5 * (vt + ct * ny);
+  D[3] = a1 * (Ht - ct * Vt) + a2 * kt + a3 * (ut * du + vt * dv - Vt * dV) + a5 * (Ht + ct * Vt);
+}
+
+// The Roe flux F = (F_L + F_R)/2 - D/2 and the face's residual terms, as
+// rusanov_combine (sterm keeps the side-averaged |Vn| + c of the time step)
+template <typename T>
+__device__ __forceinline__ void roe_combine(const T FL[4], const T FR[4], const T D[4], T VnL,
+                                            T VnR, T cL, T cR, const T Fv[4], T A, T x[4],
+                                            T& sterm) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const T F = (FL[k] + FR[k]) / 2 - D[k] / 2;
+#if FVK_FAST32
+    if constexpr (kDelta<T>) {
+      x[k] = (Fv[k] - F) * A;
+      continue;
+    }
+#endif
+    x[k] = (Fv[k] * A) - (F * A);
+  }
+  const T vn_avg = (VnL + VnR) / 2;
+  const T c_avg = (cL + cR) / 2;
+  sterm = (fabs(vn_avg) + c_avg) * A;
+}
+
 // The viscous face flux Fv of face_flux (zero unless NS at second order on an
 // interior or no-slip face), in the path's arithmetic.
 template <int ORDER, bool NS, typename T>
@@ -701,7 +764,7 @@ __device__ __forceinline__ void face_viscous(const StageArgsT<T>& a, const FlowT
 // central flux is (dF_L + dF_R)/2: the freestream flux F(Q_inf).n it leaves
 // out adds F(Q_inf) . sum(n A) over a closed cell, zero up to the rounding of
 // the face normals.
-template <int ORDER, bool NS, typename T>
+template <int ORDER, bool NS, bool ROE = false, typename T>
 __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>& fl,
                                           const CellQG<T>& c1,
                                           const CellQG<T>* c2, int kind, const FaceGeo<T>& g,
@@ -759,6 +822,14 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
       FR[k] = sr.dF[k];
     }
     VnL = sl.Vn; VnR = sr.Vn; cL = sl.c; cR = sr.c;
+    if constexpr (ROE) {
+      T D[4];
+      roe_dissipation(sl.rho, sl.u, sl.v, sl.P, fl.qinf[3] + L[3], sr.rho, sr.u, sr.v, sr.P,
+                      fl.qinf[3] + R[3], R[0] - L[0], sr.u - sl.u, sr.v - sl.v, sr.P - sl.P, nx,
+                      ny, gm1, D);
+      roe_combine(FL, FR, D, VnL, VnR, cL, cR, Fv, A, x, sterm);
+      return;
+    }
   } else {
     Prim<T> pl;
     to_prim(L, gamma, gm1, pl);
@@ -792,6 +863,13 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
     side_flux(pr, nx, ny, FR, VnR);
     cL = pl.c;
     cR = pr.c;
+    if constexpr (ROE) {
+      T D[4];
+      roe_dissipation(pl.rho, pl.u, pl.v, pl.P, pl.E, pr.rho, pr.u, pr.v, pr.P, pr.E,
+                      pr.rho - pl.rho, pr.u - pl.u, pr.v - pl.v, pr.P - pl.P, nx, ny, gm1, D);
+      roe_combine(FL, FR, D, VnL, VnR, cL, cR, Fv, A, x, sterm);
+      return;
+    }
   }
   rusanov_combine(L, R, FL, FR, VnL, VnR, cL, cR, Fv, A, x, sterm);
 }
@@ -1488,7 +1566,7 @@ __device__ __forceinline__ void lds_cell(const LdsCells<T, NQ, XY>& L, int l, Ce
   d.y = XY ? L.xy[st + l] : 0.0;
 }
 
-template <typename T, int ORDER, bool NS, int PF>
+template <typename T, int ORDER, bool NS, int PF, bool ROE = false>
 __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
 void k_pass_b_lds(StageArgsT<T> a) {
   // cell rows, then [4 or 5][max_bfaces] face terms (the spectral-sum term
@@ -1547,13 +1625,13 @@ void k_pass_b_lds(StageArgsT<T> a) {
         FaceGeo<T> g;
         if constexpr (kDelta<T>) g = f.g;
         else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, &c2, -1);
-        face_flux<ORDER, NS>(a, fl, c1, &c2, -1, g, x, st);
+        face_flux<ORDER, NS, ROE>(a, fl, c1, &c2, -1, g, x, st);
       } else {
         const int kind = (~f.ib) & 3;
         FaceGeo<T> g;
         if constexpr (kDelta<T>) g = f.g;
         else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, nullptr, kind);
-        face_flux<ORDER, NS>(a, fl, c1, (const CellQG<T>*)nullptr, kind, g, x, st);
+        face_flux<ORDER, NS, ROE>(a, fl, c1, (const CellQG<T>*)nullptr, kind, g, x, st);
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) lf[k * fstride + e] = x[k];
@@ -1926,7 +2004,17 @@ hipError_t launch_pass_b(const StageArgsT<T>& a0, int order, int ns, int block, 
       return hipGetLastError();
     }
     const size_t lds = lds_bytes_pass_b(a.m, order, a.stage, sizeof(T));
-    if (a.m.max_deg > kPF) {
+    if (a.roe) {   // the Roe-flux pass B (an extension)
+      if (a.m.max_deg > kPF) {
+        if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false, kPFDeep, true>), dim3(grid), blk, lds, s, a);
+        else if (ns) hipLaunchKernelGGL((k_pass_b_lds<T, 2, true, kPFDeep, true>), dim3(grid), blk, lds, s, a);
+        else hipLaunchKernelGGL((k_pass_b_lds<T, 2, false, kPFDeep, true>), dim3(grid), blk, lds, s, a);
+      } else {
+        if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false, kPF, true>), dim3(grid), blk, lds, s, a);
+        else if (ns) hipLaunchKernelGGL((k_pass_b_lds<T, 2, true, kPF, true>), dim3(grid), blk, lds, s, a);
+        else hipLaunchKernelGGL((k_pass_b_lds<T, 2, false, kPF, true>), dim3(grid), blk, lds, s, a);
+      }
+    } else if (a.m.max_deg > kPF) {
       if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false, kPFDeep>), dim3(grid), blk, lds, s, a);
       else if (ns) hipLaunchKernelGGL((k_pass_b_lds<T, 2, true, kPFDeep>), dim3(grid), blk, lds, s, a);
       else hipLaunchKernelGGL((k_pass_b_lds<T, 2, false, kPFDeep>), dim3(grid), blk, lds, s, a);

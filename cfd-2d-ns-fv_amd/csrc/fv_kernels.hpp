@@ -106,6 +106,7 @@ struct StageArgsT {
   int32_t xcd_remap;           // XCD block order: 0 off, 1 contiguous ranges, G>1 runs of G
   int32_t xcd_blocks;          // blocks per XCD range, or -G (set by the launcher)
   int32_t variant;             // kVariantCell, kVariantSlot or kVariantLds
+  int32_t roe;                 // flux_solver ROE: the Roe-flux pass B (LDS-staged gather only)
   const T* qc;                 // stage input state (4 per cell)
   const T* qp;                 // previous stage's input state (lagged limiter)
   const T* gp;                 // previous stage's limited gradient (8 per cell)

@@ -223,8 +223,11 @@ int check_options(const fvk_options& o, std::string* msg) {
   if (o.order != 1 && o.order != 2) { *msg = "order must be 1 or 2 (solution_poly_order 0 or 1)"; return FVK_ERR_OPTION; }
   if (o.limiter < 0 || o.limiter > 2) { *msg = "limiter must be NONE, MINMOD or SQUEEZE"; return FVK_ERR_OPTION; }
   if (o.eqn != 1 && o.eqn != 2) { *msg = "eqn must be EULER or NS"; return FVK_ERR_OPTION; }
-  if (o.flux_solver == 2) { *msg = "flux_solver ROE is not implemented by the reference solver library"; return FVK_ERR_UNSUPPORTED; }
-  if (o.flux_solver != 1) { *msg = "flux_solver must be RUSANOV"; return FVK_ERR_OPTION; }
+  if (o.flux_solver != 1 && o.flux_solver != 2) { *msg = "flux_solver must be RUSANOV or ROE"; return FVK_ERR_OPTION; }
+  if (o.flux_solver == 2 && ((o.kernel_variant != 0 && o.kernel_variant != fvk::kVariantLds) || o.accumulate)) {
+    *msg = "flux_solver ROE runs in the LDS-staged gather pass B only (kernel_variant 0 or 3, accumulate 0)";
+    return FVK_ERR_OPTION;
+  }
   if (o.time_scheme != 1 && o.time_scheme != 2) { *msg = "time_scheme must be SSP_RK2 or SSP_RK3"; return FVK_ERR_OPTION; }
   if (o.time_step_type < 1 || o.time_step_type > 3) { *msg = "time_step_type must be GLOBAL, LOCAL or 3 (uniform)"; return FVK_ERR_OPTION; }
   if (o.bc1 < -4 || o.bc1 > -1 || o.bc2 < -4 || o.bc2 > -1) { *msg = "BC1/BC2 must be FREESTREAM, SLIP_WALL, EXTRAPOLATION or NO_SLIP_WALL"; return FVK_ERR_OPTION; }
@@ -252,10 +255,10 @@ int check_common(const fvk_options& a, const fvk_options& b, std::string* msg) {
   const bool same = a.order == b.order && a.eqn == b.eqn && a.time_step_type == b.time_step_type &&
                     a.precision == b.precision && a.block_size == b.block_size &&
                     a.kernel_variant == b.kernel_variant && a.xcd_remap == b.xcd_remap &&
-                    a.accumulate == b.accumulate;
+                    a.accumulate == b.accumulate && a.flux_solver == b.flux_solver;
   if (!same) {
     *msg = "batch instances must agree on order, eqn, time_step_type, precision, block_size, "
-           "kernel_variant, xcd_remap and accumulate";
+           "kernel_variant, xcd_remap, accumulate and flux_solver";
     return FVK_ERR_OPTION;
   }
   if (a.kernel_variant == fvk::kVariantCell || a.kernel_variant == fvk::kVariantSlot) {
@@ -324,6 +327,7 @@ fvk::StageArgsT<T> base_args(Engine* c) {
   a.time = c->time;
   a.xcd_remap = c->opt.xcd_remap;
   a.variant = c->variant;
+  a.roe = c->opt.flux_solver == 2;
   a.sigma = static_cast<T*>(c->sigma);
   a.res = static_cast<T*>(c->res);
   a.accumulate = c->opt.accumulate;
@@ -878,7 +882,7 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
     // above what fits in LDS)
     const bool codes_fit = d.blk + d.max_halo < (1 << 14) && d.max_bfaces < (1 << 14);
     if (need > prop.sharedMemPerBlock || !codes_fit) {
-      if (o0.kernel_variant == fvk::kVariantLds || batched || o0.accumulate)
+      if (o0.kernel_variant == fvk::kVariantLds || batched || o0.accumulate || o0.flux_solver == 2)
         return set_err(c, FVK_ERR_MESH, "mesh blocks need " + std::to_string(need) +
                                             " bytes of LDS, more than the device allows");
       c->variant = fvk::kVariantCell;

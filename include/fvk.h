@@ -27,7 +27,8 @@ extern "C" {
 #define FVK_ERR_OPTION (-3)       /* option value the reference does not define */
 #define FVK_ERR_HIP (-4)          /* HIP runtime error (no device, launch failure) */
 #define FVK_ERR_NONFINITE (-5)    /* non-finite residual norm */
-#define FVK_ERR_UNSUPPORTED (-6)  /* option the reference declares but never ran (ROE) */
+#define FVK_ERR_UNSUPPORTED (-6)  /* reserved: no option returns it (ROE, which the
+                                     reference declares but never ran, is an extension) */
 
 /* BC kinds, read_inputs.cpp:171-205 (struct_BC values) */
 #define FVK_BC_FREESTREAM (-1)
@@ -68,7 +69,11 @@ typedef struct fvk_options {
   int32_t order;           /* solution_poly_order + 1: 1 or 2 */
   int32_t limiter;         /* 0 NONE, 1 MINMOD (a no-op, res.cpp:36-38), 2 SQUEEZE */
   int32_t eqn;             /* 1 EULER, 2 NS */
-  int32_t flux_solver;     /* 1 RUSANOV (2 ROE -> FVK_ERR_UNSUPPORTED) */
+  int32_t flux_solver;     /* 1 RUSANOV; 2 ROE, an extension: the reference reads it
+                              (read_inputs.cpp:89-91) but its Roe flux is an unfinished
+                              draft (res.cpp:66-68 leaves the flux at -101); here Roe's
+                              flux with Harten's entropy fix, in the LDS-staged gather
+                              pass B only (kernel_variant 0 or 3, accumulate 0) */
   int32_t time_scheme;     /* 1 SSP_RK2 (runs SSP-RK3, main.cpp:123-126), 2 SSP_RK3 */
   int32_t time_step_type;  /* 1 GLOBAL, 2 LOCAL; 3 UNIFORM, an extension: one dt for
                               every cell, 2*CFL / max_i(Sigma_i / V_i), so that a run

@@ -518,6 +518,71 @@ static void reconstruction(or_case* C, qset* Q) {
   }
 }
 
+/* Roe's flux difference |A~|(Q_R - Q_L) for the face normal n: an
+ * EXTENSION, not a restatement.  The reference reads ROE
+ * (read_inputs.cpp:89-91) but its roe_flux.cpp is an unfinished draft that
+ * its solver library does not build, and res.cpp:66-68 leaves the flux at
+ * -101.  Roe's averages (P. L. Roe, J. Comput. Phys. 43, 1981), the
+ * decomposition into two acoustic waves, the entropy wave and the shear wave,
+ * and Harten's entropy fix (delta = c~/10) on the acoustic eigenvalues; the
+ * device path (fv_kernels.hip roe_dissipation) evaluates the same expressions.
+ * Pinned by tests/test_oracle_roe.py: consistency, upwinding of supersonic
+ * states (Roe's property A~ dQ = dF), symmetry, rotation, closed-box
+ * conservation and freestream preservation. */
+static void roe_dissipation(double rL, double uL, double vL, double PL, double EL, double rR,
+                            double uR, double vR, double PR, double ER, double dr, double du,
+                            double dv, double dp, double nx, double ny, double gm1, double D[4]) {
+  double sL = sqrt(rL), sR = sqrt(rR);
+  double w = sL + sR;
+  double HL = (EL + PL) / rL;
+  double HR = (ER + PR) / rR;
+  double rt = sL * sR;
+  double ut = (sL * uL + sR * uR) / w;
+  double vt = (sL * vL + sR * vR) / w;
+  double Ht = (sL * HL + sR * HR) / w;
+  double kt = (ut * ut + vt * vt) / 2;
+  double ct = sqrt(gm1 * (Ht - kt));
+  double Vt = ut * nx + vt * ny;
+  double dV = du * nx + dv * ny;
+  double del = ct / 10;
+  double l1 = fabs(Vt - ct), l5 = fabs(Vt + ct);
+  double l2 = fabs(Vt);
+  if (l1 < del) l1 = (l1 * l1 + del * del) / (2 * del);
+  if (l5 < del) l5 = (l5 * l5 + del * del) / (2 * del);
+  double c2 = ct * ct;
+  double a1 = l1 * (dp - rt * ct * dV) / (2 * c2);
+  double a2 = l2 * (dr - dp / c2);
+  double a3 = l2 * rt;
+  double a5 = l5 * (dp + rt * ct * dV) / (2 * c2);
+  D[0] = a1 + a2 + a5;
+  D[1] = a1 * (ut - ct * nx) + a2 * ut + a3 * (du - dV * nx) + a5 * (ut + ct * nx);
+  D[2] = a1 * (vt - ct * ny) + a2 * vt + a3 * (dv - dV * ny) + a5 * (vt + ct * ny);
+  D[3] = a1 * (Ht - ct * Vt) + a2 * kt + a3 * (ut * du + vt * dv - Vt * dV) + a5 * (Ht + ct * Vt);
+}
+
+/* the primitives of a conserved state, as updateflow (class_q.hpp:35-48) */
+static void prims_of(const double q[4], double gamma, double* rho, double* u, double* v,
+                     double* E, double* P) {
+  *rho = q[0];
+  *E = q[3];
+  *u = q[1] / *rho;
+  *v = q[2] / *rho;
+  *P = (*E - (0.5 * *rho * (*u * *u + *v * *v))) * (gamma - 1);
+}
+
+void or_roe_flux(const double qL[4], const double qR[4], double nx, double ny, double gamma,
+                 double F[4]) {
+  double rL, uL, vL, EL, PL, rR, uR, vR, ER, PR, D[4];
+  prims_of(qL, gamma, &rL, &uL, &vL, &EL, &PL);
+  prims_of(qR, gamma, &rR, &uR, &vR, &ER, &PR);
+  double VnL = uL * nx + vL * ny, VnR = uR * nx + vR * ny;
+  double FL[4] = {rL * VnL, rL * uL * VnL + PL * nx, rL * vL * VnL + PL * ny, VnL * (EL + PL)};
+  double FR[4] = {rR * VnR, rR * uR * VnR + PR * nx, rR * vR * VnR + PR * ny, VnR * (ER + PR)};
+  roe_dissipation(rL, uL, vL, PL, EL, rR, uR, vR, PR, ER, rR - rL, uR - uL, vR - vL, PR - PL,
+                  nx, ny, gamma - 1, D);
+  for (int k = 0; k < 4; ++k) F[k] = (FL[k] + FR[k]) / 2 - D[k] / 2;
+}
+
 /* res  res.cpp:21-109 (with rusanov.cpp:12-36 and class_f.hpp:26-59) */
 static void res(or_case* C, qset* Q) {
   int nf = C->nf, nc = C->nc;
@@ -564,6 +629,32 @@ static void res(or_case* C, qset* Q) {
       R->vn_avg[f] = vn_avg; R->c_avg[f] = c_avg;
       for (int k = 0; k < 4; ++k)
         Fp[k][f] = (FL[k] + FR[k]) / 2 - (fabs(vn_avg) + c_avg) / 2 * (R->p[k][f] - L->p[k][f]);
+    }
+  } else if (C->o.flux == 2) {     /* Roe (an extension, roe_dissipation above) */
+    qset* L = &C->Qc1;
+    qset* R = &C->Qc2;
+    double gm1 = C->o.gamma - 1;
+    for (int f = 0; f < nf; ++f) {
+      double nx = C->nx[f], ny = C->ny[f];
+      double VnL = L->u[f] * nx + L->v[f] * ny;
+      double VnR = R->u[f] * nx + R->v[f] * ny;
+      double FL[4], FR[4], D[4];
+      FL[0] = L->rho[f] * VnL;
+      FL[1] = L->rho[f] * L->u[f] * VnL + L->P[f] * nx;
+      FL[2] = L->rho[f] * L->v[f] * VnL + L->P[f] * ny;
+      FL[3] = VnL * (L->E[f] + L->P[f]);
+      FR[0] = R->rho[f] * VnR;
+      FR[1] = R->rho[f] * R->u[f] * VnR + R->P[f] * nx;
+      FR[2] = R->rho[f] * R->v[f] * VnR + R->P[f] * ny;
+      FR[3] = VnR * (R->E[f] + R->P[f]);
+      double vn_avg = (VnL + VnR) / 2;
+      double c_avg = (L->c[f] + R->c[f]) / 2;
+      L->vn_avg[f] = vn_avg; L->c_avg[f] = c_avg;
+      R->vn_avg[f] = vn_avg; R->c_avg[f] = c_avg;
+      roe_dissipation(L->rho[f], L->u[f], L->v[f], L->P[f], L->E[f], R->rho[f], R->u[f],
+                      R->v[f], R->P[f], R->E[f], R->rho[f] - L->rho[f], R->u[f] - L->u[f],
+                      R->v[f] - L->v[f], R->P[f] - L->P[f], nx, ny, gm1, D);
+      for (int k = 0; k < 4; ++k) Fp[k][f] = (FL[k] + FR[k]) / 2 - D[k] / 2;
     }
   }
   for (int k = 0; k < 4; ++k)      /* residual->reset()  res.cpp:73 */

@@ -15,6 +15,7 @@
  *   squeeze limiter src/solver/squeeze.cpp:9-154
  *   reconstruction  src/solver/reconstruction.cpp:15-167
  *   Rusanov flux    src/solver/rusanov.cpp:12-36, src/support/class_f.hpp:41-59
+ *   Roe flux        an extension (the reference's draft is not built; res.cpp:66-68)
  *   residual norms  src/base/save_sol.cpp:79-89
  *   wall CL/CD      src/base/save_sol.cpp:20-75
  * Unlike the reference it keeps each cell's faces in a CSR list, so cells with
@@ -39,7 +40,8 @@ typedef struct {
   int32_t order;          /* solution_poly_order + 1 (1 or 2) */
   int32_t limiter;        /* 0 NONE, 1 MINMOD (no-op), 2 SQUEEZE */
   int32_t eqn;            /* 1 EULER, 2 NS */
-  int32_t flux;           /* 1 RUSANOV, 2 ROE (flux left at -101, as res.cpp:66-68) */
+  int32_t flux;           /* 1 RUSANOV, 2 ROE: an extension (the reference leaves the
+                             flux at -101, res.cpp:66-68), see or_roe_flux */
   int32_t dt_type;        /* 1 GLOBAL, 2 LOCAL, 3 UNIFORM (extension) */
   int32_t bc1, bc2;       /* BC kinds of file markers 0 and -1: -1..-4 */
   int32_t pad_;
@@ -67,6 +69,10 @@ int or_res(or_case* c);
 void or_residual_l2(const or_case* c, double out[4]);
 /* CL, CD of the monitor block, save_sol.cpp:20-75 (numWALL faces first) */
 void or_wall_forces(const or_case* c, double length, double out[2]);
+/* the Roe flux F.n of the extension (flux = 2) between conserved states qL
+ * and qR across the unit normal (nx, ny) (tests) */
+void or_roe_flux(const double qL[4], const double qR[4], double nx, double ny, double gamma,
+                 double F[4]);
 
 /* named array access: returns pointer and sets *n (elements), *kind 'd' or 'i' */
 const void* or_array(const or_case* c, const char* name, int64_t* n, char* kind);

@@ -57,6 +57,8 @@ def lib():
         L.or_res.argtypes = [ctypes.c_void_p]
         L.or_residual_l2.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
         L.or_wall_forces.argtypes = [ctypes.c_void_p, ctypes.c_double, P(ctypes.c_double)]
+        L.or_roe_flux.argtypes = [P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, P(ctypes.c_double)]
         L.or_array.restype = ctypes.c_void_p
         L.or_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, P(ctypes.c_int64), P(ctypes.c_char)]
         _lib = L
@@ -146,6 +148,16 @@ class Oracle:
         out = (ctypes.c_double * 2)()
         lib().or_wall_forces(self.h, length, out)
         return np.array(out[:])
+
+
+def roe_flux(qL, qR, nx: float, ny: float, gamma: float = 1.4) -> np.ndarray:
+    """The oracle's Roe flux F.n (the flux = 2 extension) between two conserved
+    states across the unit normal (nx, ny)."""
+    a = np.ascontiguousarray(qL, dtype=np.float64)
+    b = np.ascontiguousarray(qR, dtype=np.float64)
+    out = np.zeros(4)
+    lib().or_roe_flux(_dp(a), _dp(b), nx, ny, gamma, _dp(out))
+    return out
 
 
 def read_dump(path: str) -> Dict[str, np.ndarray]:

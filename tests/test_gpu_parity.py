@@ -181,14 +181,6 @@ def test_freestream_preserved():
     assert np.allclose(q, qinf[:, None], rtol=1e-9, atol=1e-6 * np.abs(qinf).max())
 
 
-def test_unsupported_options_fail_loudly():
-    g = Golden("bump_o1_global")
-    case = inputs.Case(**{**g.meta["case"], "flux": "ROE"})
-    with pytest.raises(FvkError) as e:
-        Solver(g.grid, case)
-    assert e.value.code == -6
-
-
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("name", ["ramp_o2_squeeze", "ogrid_ns_squeeze_global"])
 def test_checkpoint_resumes_exactly(name, precision):
