@@ -874,9 +874,10 @@ __device__ __forceinline__ void face_flux(const StageArgsT<T>& a, const FlowT<T>
   rusanov_combine(L, R, FL, FR, VnL, VnR, cL, cR, Fv, A, x, sterm);
 }
 
-// Residual / volume, dt, SSP-RK3 stage combination and stores of one cell.
-// The combinations' coefficients sum to one, so they apply unchanged to a
-// perturbation state.
+// Residual / volume, dt, SSP-RK3 (or two-stage SSP-RK2) stage combination and
+// stores of one cell.  The combinations' coefficients sum to one, so they
+// apply unchanged to a perturbation state.  The step's last stage stores the
+// spectral sums of the next step's dt.
 // Uniform time step (time_step_type 3, an extension: one dt for every cell,
 // dt = 2*CFL / max_i(Sigma_i / V_i), so the run is time-accurate): the dt of
 // the step whose maximum has FP64 bits `bits`.
@@ -918,16 +919,15 @@ __device__ __forceinline__ void cell_update(const StageArgsT<T>& a, const FlowT<
   if (a.stage == 1) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) out[k] = q[k] + dt * r[k];
-  } else if (a.stage == 2) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) out[k] = T(0.75) * qn[k] + T(0.25) * q[k] + T(0.25) * dt * r[k];
   } else {
+    // ca Q_n + cb (Q + dt R): SSP-RK3 stage 2 (3/4, 1/4) and 3 (1/3, 2/3)
+    // (ssp_rk3.cpp:58-59,68-69), the two-stage SSP-RK2's stage 2 (1/2, 1/2)
+    const T ca = a.ca, cb = a.cb;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      out[k] = T(1.0 / 3.0) * qn[k] + T(2.0 / 3.0) * q[k] + T(2.0 / 3.0) * dt * r[k];
+    for (int k = 0; k < 4; ++k) out[k] = ca * qn[k] + cb * q[k] + cb * dt * r[k];
   }
   st4(a.qout + 4 * (size_t)c, out);
-  if (a.stage == 3) {
+  if (a.last) {
     a.sigma[c] = sg;
     if (a.store_res) st4(a.res + 4 * (size_t)c, r);
   }
@@ -1106,7 +1106,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_cell(StageArgsT<T> a) {
     cell_update_ld(a, fl, c, me.q, r, sg);
     sig_new = dt_key(a, sg, m.vol[c]);
   }
-  if (a.stage == 3 && a.dt_global) block_max_to(sig_new, a.smax_next);
+  if (a.last && a.dt_global) block_max_to(sig_new, a.smax_next);
   if (a.stage == 1 && blockIdx.x == 0 && threadIdx.x == 0) step_start(a, fl, 0);
 }
 
@@ -1286,7 +1286,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
       sig_new = dt_key(a, sg, m.vol[c]);
     }
   }
-  if (a.stage == 3 && a.dt_global) block_max_to(sig_new, a.smax_next);
+  if (a.last && a.dt_global) block_max_to(sig_new, a.smax_next);
   if (a.stage == 1 && blockIdx.x == 0 && threadIdx.x == 0) step_start(a, fl, 0);
 }
 
@@ -1635,7 +1635,7 @@ void k_pass_b_lds(StageArgsT<T> a) {
       }
 #pragma unroll
       for (int k = 0; k < 4; ++k) lf[k * fstride + e] = x[k];
-      if (a.stage == 3) lf[4 * fstride + e] = st;
+      if (a.last) lf[4 * fstride + e] = st;
     };
 #pragma unroll
     for (int t = 0; t < kFPFB; ++t) {
@@ -1664,7 +1664,7 @@ void k_pass_b_lds(StageArgsT<T> a) {
           const T x = lf[k * fstride + e];
           r[k] = r[k] + (own ? x : -x);
         }
-        if (a.stage == 3) sg += lf[4 * fstride + e];
+        if (a.last) sg += lf[4 * fstride + e];
       };
 #pragma unroll
       for (int j = 0; j < PF; ++j)
@@ -1677,7 +1677,7 @@ void k_pass_b_lds(StageArgsT<T> a) {
       sig_new = dt_key(a, sg, V);
     }
   }
-  if (live && a.stage == 3 && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
+  if (live && a.last && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
   if (live && a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) step_start(a, fl, inst);
 }
 
@@ -1724,7 +1724,7 @@ void k_pass_b_atomic(StageArgsT<T> a) {
         unsafeAtomicAdd(a.acc + 4 * (size_t)g1 + k, x[k]);
         unsafeAtomicAdd(a.acc + 4 * (size_t)g2 + k, -x[k]);
       }
-      if (a.stage == 3) {
+      if (a.last) {
         unsafeAtomicAdd(a.sacc + g1, st);
         unsafeAtomicAdd(a.sacc + g2, st);
       }
@@ -1736,7 +1736,7 @@ void k_pass_b_atomic(StageArgsT<T> a) {
       face_flux<ORDER, NS>(a, fl, c1, (const CellQG<T>*)nullptr, kind, g, x, st);
 #pragma unroll
       for (int k = 0; k < 4; ++k) unsafeAtomicAdd(a.acc + 4 * (size_t)g1 + k, x[k]);
-      if (a.stage == 3) unsafeAtomicAdd(a.sacc + g1, st);
+      if (a.last) unsafeAtomicAdd(a.sacc + g1, st);
     }
   }
 }
@@ -1757,7 +1757,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_update_atomic(StageArgsT<T> a) {
     const T zero[4] = {0, 0, 0, 0};
     st4(a.acc + 4 * (size_t)c, zero);
     T sg = 0;
-    if (a.stage == 3) {
+    if (a.last) {
       sg = a.sacc[c];
       a.sacc[c] = T(0);
     }
@@ -1767,7 +1767,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_update_atomic(StageArgsT<T> a) {
     cell_update(a, fl, inst, c, q, qn, r, sg, m.vol[c], sig_old);
     sig_new = dt_key(a, sg, m.vol[c]);
   }
-  if (a.stage == 3 && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
+  if (a.last && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
   if (a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) step_start(a, fl, inst);
 }
 
@@ -1947,9 +1947,9 @@ size_t lds_bytes_pass_a(const DevMesh& m, size_t tsize) {
   return lds_cells_bytes(m.blk + m.max_halo, 4, tsize);
 }
 
-size_t lds_bytes_pass_b(const DevMesh& m, int order, int stage, size_t tsize) {
+size_t lds_bytes_pass_b(const DevMesh& m, int order, bool last, size_t tsize) {
   return lds_cells_bytes(m.blk + m.max_halo, order == 2 ? 12 : 4, tsize, tsize == 8) +
-         (stage == 3 ? 5 : 4) * (size_t)m.max_bfaces * tsize;
+         (last ? 5 : 4) * (size_t)m.max_bfaces * tsize;
 }
 
 // cell rows only (the atomic-scatter pass B keeps no face terms in LDS)
@@ -2003,7 +2003,7 @@ hipError_t launch_pass_b(const StageArgsT<T>& a0, int order, int ns, int block, 
       hipLaunchKernelGGL((k_update_atomic<T>), dim3(grid), dim3(a.m.blk), 0, s, a);
       return hipGetLastError();
     }
-    const size_t lds = lds_bytes_pass_b(a.m, order, a.stage, sizeof(T));
+    const size_t lds = lds_bytes_pass_b(a.m, order, a.last != 0, sizeof(T));
     if (a.roe) {   // the Roe-flux pass B (an extension)
       if (a.m.max_deg > kPF) {
         if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false, kPFDeep, true>), dim3(grid), blk, lds, s, a);

@@ -98,6 +98,9 @@ struct StageArgsT {
   int32_t nactive;             // active blocks (batch only)
   int32_t accumulate;          // 1: pass B scatters with atomics, a cell pass updates
   int32_t stage;               // 1, 2, 3
+  int32_t last;                // the step's last stage (3, or 2 for the two-stage SSP-RK2):
+                               // spectral sums for the next dt, GLOBAL maximum, residual
+  T ca, cb;                    // stage > 1: Q_out = ca Q_n + cb (Q + dt R)
   int32_t dt_global;           // one maximum per step: time_step_type GLOBAL or uniform
   int32_t dt_uniform;          // time_step_type 3: one dt = 2 CFL / max(Sigma_i / V_i)
   int32_t sentinel;            // lagged face states are the -101 initial values
@@ -132,7 +135,7 @@ enum { kVariantCell = 1, kVariantSlot = 2, kVariantLds = 3 };
 // (pass B stages centroids only for FP64: the FP32 kernels read per-face
 // offsets instead)
 size_t lds_bytes_pass_a(const DevMesh& m, size_t tsize);
-size_t lds_bytes_pass_b(const DevMesh& m, int order, int stage, size_t tsize);
+size_t lds_bytes_pass_b(const DevMesh& m, int order, bool last, size_t tsize);
 
 template <typename T>
 hipError_t launch_pass_a(const StageArgsT<T>& a, int squeeze, int block, hipStream_t s);

@@ -219,6 +219,10 @@ int upload_cat(Engine* c, Vec fvk::HostMesh::*field, const T** out) {
 // boundary code ~((src << 2) | kind) with the ghost source cell moved by `off`
 inline int32_t shift_boundary(int32_t code, int32_t off) { return ~((~code) + (off << 2)); }
 
+// stages per step: 3 (SSP_RK3, and SSP_RK2, which the reference runs as
+// SSP-RK3, main.cpp:123-126), or 2 for the two-stage SSP-RK2 extension
+inline int stages_of(const fvk_options& o) { return o.time_scheme == 3 ? 2 : 3; }
+
 int check_options(const fvk_options& o, std::string* msg) {
   if (o.order != 1 && o.order != 2) { *msg = "order must be 1 or 2 (solution_poly_order 0 or 1)"; return FVK_ERR_OPTION; }
   if (o.limiter < 0 || o.limiter > 2) { *msg = "limiter must be NONE, MINMOD or SQUEEZE"; return FVK_ERR_OPTION; }
@@ -228,7 +232,7 @@ int check_options(const fvk_options& o, std::string* msg) {
     *msg = "flux_solver ROE runs in the LDS-staged gather pass B only (kernel_variant 0 or 3, accumulate 0)";
     return FVK_ERR_OPTION;
   }
-  if (o.time_scheme != 1 && o.time_scheme != 2) { *msg = "time_scheme must be SSP_RK2 or SSP_RK3"; return FVK_ERR_OPTION; }
+  if (o.time_scheme < 1 || o.time_scheme > 3) { *msg = "time_scheme must be SSP_RK2, SSP_RK3 or 3 (two-stage SSP-RK2)"; return FVK_ERR_OPTION; }
   if (o.time_step_type < 1 || o.time_step_type > 3) { *msg = "time_step_type must be GLOBAL, LOCAL or 3 (uniform)"; return FVK_ERR_OPTION; }
   if (o.bc1 < -4 || o.bc1 > -1 || o.bc2 < -4 || o.bc2 > -1) { *msg = "BC1/BC2 must be FREESTREAM, SLIP_WALL, EXTRAPOLATION or NO_SLIP_WALL"; return FVK_ERR_OPTION; }
   if (o.precision != FVK_FP64 && o.precision != FVK_FP32_MIXED) { *msg = "precision must be FVK_FP64 or FVK_FP32_MIXED"; return FVK_ERR_OPTION; }
@@ -255,10 +259,11 @@ int check_common(const fvk_options& a, const fvk_options& b, std::string* msg) {
   const bool same = a.order == b.order && a.eqn == b.eqn && a.time_step_type == b.time_step_type &&
                     a.precision == b.precision && a.block_size == b.block_size &&
                     a.kernel_variant == b.kernel_variant && a.xcd_remap == b.xcd_remap &&
-                    a.accumulate == b.accumulate && a.flux_solver == b.flux_solver;
+                    a.accumulate == b.accumulate && a.flux_solver == b.flux_solver &&
+                    stages_of(a) == stages_of(b);
   if (!same) {
     *msg = "batch instances must agree on order, eqn, time_step_type, precision, block_size, "
-           "kernel_variant, xcd_remap, accumulate and flux_solver";
+           "kernel_variant, xcd_remap, accumulate, flux_solver and the stage count of time_scheme";
     return FVK_ERR_OPTION;
   }
   if (a.kernel_variant == fvk::kVariantCell || a.kernel_variant == fvk::kVariantSlot) {
@@ -341,15 +346,32 @@ fvk::StageArgsT<T> base_args(Engine* c) {
 
 // stage roles of the three state and gradient buffers (ssp_rk3.cpp:38-76):
 // stage s reads Q_{s-1} and the previous stage's Q and limited gradient (the
-// lagged limiter), writes G_s and Q_s; stage 3 writes Q_n in place
+// lagged limiter), writes G_s and Q_s; stage 3 writes Q_n in place.  The
+// two-stage SSP-RK2 (time_scheme 3) writes Q* to buffer 1 and Q_n+1 in place;
+// its stage 1 takes the lagged data of the previous step's stage 2 (Q*, G_2).
 template <typename T>
-void set_roles(Engine* c, int stage, fvk::StageArgsT<T>& a) {
+void set_roles(Engine* c, int stage, bool rk2, fvk::StageArgsT<T>& a) {
   T* const* q = reinterpret_cast<T* const*>(c->q);
   T* const* g = reinterpret_cast<T* const*>(c->g);
   a.qn = q[0];
+  if (rk2) {
+    if (stage == 1) { a.qc = q[0]; a.qp = q[1]; a.gp = g[1]; a.g = g[0]; a.qout = q[1]; }
+    if (stage == 2) { a.qc = q[1]; a.qp = q[0]; a.gp = g[0]; a.g = g[1]; a.qout = q[0]; }
+    return;
+  }
   if (stage == 1) { a.qc = q[0]; a.qp = q[2]; a.gp = g[2]; a.g = g[0]; a.qout = q[1]; }
   if (stage == 2) { a.qc = q[1]; a.qp = q[0]; a.gp = g[0]; a.g = g[1]; a.qout = q[2]; }
   if (stage == 3) { a.qc = q[2]; a.qp = q[1]; a.gp = g[1]; a.g = g[2]; a.qout = q[0]; }
+}
+
+
+
+// the stage combination Q_out = ca Q_n + cb (Q + dt R) of stages 2 and 3
+template <typename T>
+void stage_coefficients(int nst, int stage, fvk::StageArgsT<T>& a) {
+  if (nst == 2) { a.ca = T(0.5); a.cb = T(0.5); }
+  else if (stage == 2) { a.ca = T(0.75); a.cb = T(0.25); }
+  else { a.ca = T(1.0 / 3.0); a.cb = T(2.0 / 3.0); }
 }
 
 template <typename T>
@@ -397,13 +419,16 @@ int run_step_t(Engine* c, bool store_res, Launch&& launch) {
   const fvk_options& o = c->opt;
   // the SQUEEZE pass A when any instance limits; per-instance switch inside
   const bool squeeze = any_squeeze(c);
-  for (int stage = 1; stage <= 3; ++stage) {
+  const int nst = stages_of(o);
+  for (int stage = 1; stage <= nst; ++stage) {
     fvk::StageArgsT<T> a = base_args<T>(c);
     a.stage = stage;
+    a.last = stage == nst ? 1 : 0;
+    stage_coefficients(nst, stage, a);
     a.sentinel = (stage == 1 && c->sentinel) ? 1 : 0;
     a.fs_prims = (stage == 1 && c->fs_prims) ? 1 : 0;
-    a.store_res = (stage == 3 && store_res) ? 1 : 0;
-    set_roles(c, stage, a);
+    a.store_res = (a.last && store_res) ? 1 : 0;
+    set_roles(c, stage, nst == 2, a);
     if (o.order == 2) {
       int rc = launch(kKindA, [&]() { return fvk::launch_pass_a(a, squeeze, c->block, c->stream); });
       if (rc) return rc;
@@ -659,7 +684,15 @@ int time_kernels_t(Engine* c, int32_t reps, double avg_us[2]) {
   const fvk_options& o = c->opt;
   fvk::StageArgsT<T> a = base_args<T>(c);
   a.stage = 2;
-  set_roles(c, 2, a);
+  a.last = 0;
+  stage_coefficients(3, 2, a);
+  set_roles(c, 2, false, a);
+  if (stages_of(o) == 2) {
+    // the two-stage scheme's lagged data lives in buffers 0 and 1: the timed
+    // launches write buffers 2 only, which it does not use
+    T* const* g = reinterpret_cast<T* const*>(c->g);
+    a.g = g[2];
+  }
   const bool squeeze = any_squeeze(c);
   hipEvent_t e0, e1;
   FVK_HIP(c, hipEventCreate(&e0));
@@ -877,7 +910,7 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
     if ((e = hipGetDevice(&dev)) != hipSuccess || (e = hipGetDeviceProperties(&prop, dev)) != hipSuccess)
       return hip_err(c, e, "hipGetDeviceProperties");
     const size_t need = std::max(fvk::lds_bytes_pass_a(d, c->tsize),
-                                 fvk::lds_bytes_pass_b(d, o0.order, 3, c->tsize));
+                                 fvk::lds_bytes_pass_b(d, o0.order, true, c->tsize));
     // (16-bit local codes: blk + max_halo and max_bfaces below 2^14, far
     // above what fits in LDS)
     const bool codes_fit = d.blk + d.max_halo < (1 << 14) && d.max_bfaces < (1 << 14);
@@ -952,7 +985,8 @@ struct CheckpointHeader {
 
 // FNV-1a hash of every instance's options that the stored state depends on:
 // the freestream (the FP32-mixed state is stored relative to it), the gas
-// constants, CFL and the scheme switches.  A checkpoint is refused by a
+// constants, CFL and the scheme switches (flux, and the stage count, which
+// fixes the roles of the stored buffers).  A checkpoint is refused by a
 // context whose hash differs.
 uint64_t options_hash(const Engine* c) {
   uint64_t h = 1469598103934665603ull;
@@ -967,7 +1001,8 @@ uint64_t options_hash(const Engine* c) {
     const fvk_options& o = in.opt;
     const double d[10] = {in.qinf[0], in.qinf[1], in.qinf[2], in.qinf[3], o.cfl, o.gamma,
                           o.R, o.mu, o.Pr, o.P};
-    const int32_t k[7] = {o.order, o.limiter, o.eqn, o.time_step_type, o.bc1, o.bc2, o.precision};
+    const int32_t k[9] = {o.order, o.limiter, o.eqn, o.time_step_type, o.bc1, o.bc2, o.precision,
+                          o.flux_solver, stages_of(o)};
     mix(d, sizeof(d));
     mix(k, sizeof(k));
   }

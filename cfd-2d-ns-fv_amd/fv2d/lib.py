@@ -151,7 +151,7 @@ def make_options(case, **overrides) -> FvkOptions:
     o.limiter = {"NONE": 0, "MINMOD": 1, "SQUEEZE": 2}[case.limiter]
     o.eqn = {"EULER": 1, "NS": 2}[case.eqn]
     o.flux_solver = {"RUSANOV": 1, "ROE": 2}[case.flux]
-    o.time_scheme = {"SSP_RK2": 1, "SSP_RK3": 2}[case.scheme]
+    o.time_scheme = {"SSP_RK2": 1, "SSP_RK3": 2, "SSP_RK2_TWO_STAGE": 3}[case.scheme]
     o.time_step_type = {"GLOBAL": 1, "LOCAL": 2, "UNIFORM": 3}[case.dt_type]
     o.bc1 = BC_CODES[case.BC1]
     o.bc2 = BC_CODES[case.BC2]

@@ -74,7 +74,10 @@ typedef struct fvk_options {
                               draft (res.cpp:66-68 leaves the flux at -101); here Roe's
                               flux with Harten's entropy fix, in the LDS-staged gather
                               pass B only (kernel_variant 0 or 3, accumulate 0) */
-  int32_t time_scheme;     /* 1 SSP_RK2 (runs SSP-RK3, main.cpp:123-126), 2 SSP_RK3 */
+  int32_t time_scheme;     /* 1 SSP_RK2 (runs SSP-RK3, main.cpp:123-126), 2 SSP_RK3;
+                              3 two-stage SSP-RK2, an extension (the reference's
+                              ssp_rk2.cpp is a draft it never builds): Q* = Q + dt R(Q),
+                              Q_n+1 = Q/2 + (Q* + dt R(Q*))/2 */
   int32_t time_step_type;  /* 1 GLOBAL, 2 LOCAL; 3 UNIFORM, an extension: one dt for
                               every cell, 2*CFL / max_i(Sigma_i / V_i), so that a run
                               is time-accurate (GLOBAL's dt_i = 2*CFL*V_i / max Sigma,

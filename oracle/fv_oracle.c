@@ -47,6 +47,7 @@ struct or_case {
   /* scratch */
   double *dt;
   double time;                     /* physical time of the uniform dt (extension) */
+  double fixed_dt;                 /* > 0: every step takes this dt (test hook) */
 };
 
 static char g_err[512];
@@ -674,13 +675,16 @@ static void res(or_case* C, qset* Q) {
   for (int k = 0; k < 4; ++k) { free(Fp[k]); free(Fv[k]); }
 }
 
-/* ssp_rk3  ssp_rk3.cpp:18-77 */
-static void ssp_rk3(or_case* C) {
+/* the step's dt from the last res() call's face wave speeds (ssp_rk3.cpp:18-47);
+ * a fixed dt for every cell instead when C->fixed_dt > 0 (a test hook) */
+static void step_dt(or_case* C) {
   int nf = C->nf, nc = C->nc;
-  qset S;
-  qset_alloc(&S, nc);
-  qset_init_plain(&S, C->o.gamma, C->o.R);
   double* dt = C->dt;
+  if (C->fixed_dt > 0) {
+    for (int c = 0; c < nc; ++c) dt[c] = C->fixed_dt;
+    C->time += C->fixed_dt;
+    return;
+  }
   for (int c = 0; c < nc; ++c) dt[c] = 0;
   for (int f = 0; f < nf; ++f) {   /* :28-36 */
     int c1 = C->fc1[f], c2 = C->fc2[f];
@@ -707,6 +711,16 @@ static void ssp_rk3(or_case* C) {
     for (int c = 0; c < nc; ++c) dt[c] = d;
     C->time += d;
   }
+}
+
+/* ssp_rk3  ssp_rk3.cpp:18-77 */
+static void ssp_rk3(or_case* C) {
+  int nc = C->nc;
+  qset S;
+  qset_alloc(&S, nc);
+  qset_init_plain(&S, C->o.gamma, C->o.R);
+  double* dt = C->dt;
+  step_dt(C);
   qset* Q = &C->Qbar;
   res(C, Q);                       /* :50-57 */
   for (int c = 0; c < nc; ++c)
@@ -721,6 +735,33 @@ static void ssp_rk3(or_case* C) {
   for (int c = 0; c < nc; ++c)
     for (int k = 0; k < 4; ++k)
       Q->p[k][c] = (1.0 / 3.0) * Q->p[k][c] + (2.0 / 3.0) * S.p[k][c] + (2.0 / 3.0) * dt[c] * C->res[k][c];
+  updateflow(Q);
+  qset_free(&S);
+}
+
+/* Two-stage SSP-RK2 (scheme = 3): an EXTENSION.  The reference's
+ * ssp_rk2.cpp is a draft its solver library never builds (main.cpp:123-126
+ * runs ssp_rk3 for SSP_RK2); this follows that draft's stage structure
+ * (ssp_rk2.cpp:44-60: Q* = Q + dt R(Q), Q_n+1 = (Q + Q* + dt R(Q*))/2, the
+ * lagged limiter fed by the previous res() call), the last stage written in
+ * ssp_rk3's Shu-Osher form (the three terms each halved), with the dt of the
+ * scheme's time_step_type, as ssp_rk3.  Pinned by tests/test_oracle_rk2.py (second
+ * order in time, conservation, freestream preservation). */
+static void ssp_rk2(or_case* C) {
+  int nc = C->nc;
+  qset S;
+  qset_alloc(&S, nc);
+  qset_init_plain(&S, C->o.gamma, C->o.R);
+  double* dt = C->dt;
+  step_dt(C);
+  qset* Q = &C->Qbar;
+  res(C, Q);
+  for (int c = 0; c < nc; ++c)
+    for (int k = 0; k < 4; ++k) S.p[k][c] = Q->p[k][c] + dt[c] * C->res[k][c];
+  updateflow(&S);
+  res(C, &S);
+  for (int c = 0; c < nc; ++c)
+    for (int k = 0; k < 4; ++k) Q->p[k][c] = 0.5 * Q->p[k][c] + 0.5 * S.p[k][c] + 0.5 * dt[c] * C->res[k][c];
   updateflow(Q);
   qset_free(&S);
 }
@@ -861,9 +902,14 @@ int or_set_state(or_case* C, const double* p1, const double* p2, const double* p
 }
 
 int or_step(or_case* C, int32_t n) {
-  for (int i = 0; i < n; ++i) ssp_rk3(C);
+  for (int i = 0; i < n; ++i) {
+    if (C->o.scheme == 3) ssp_rk2(C);
+    else ssp_rk3(C);
+  }
   return 0;
 }
+
+void or_set_fixed_dt(or_case* C, double dt) { C->fixed_dt = dt; }
 
 int or_res(or_case* C) {
   res(C, &C->Qbar);

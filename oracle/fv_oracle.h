@@ -9,7 +9,7 @@
  * Avery-Hantla/CFD-2D-NS-FV:
  *   geometry        src/base/readmesh.cpp:113-411
  *   initialisation  src/base/main.cpp:77-118, src/support/class_q.hpp:19-61
- *   time step       src/solver/ssp_rk3.cpp:18-77
+ *   time step       src/solver/ssp_rk3.cpp:18-77 (and a two-stage SSP-RK2, an extension)
  *   residual        src/solver/res.cpp:21-109
  *   LSQ gradient    src/solver/cell_gradient.cpp:9-130
  *   squeeze limiter src/solver/squeeze.cpp:9-154
@@ -44,7 +44,8 @@ typedef struct {
                              flux at -101, res.cpp:66-68), see or_roe_flux */
   int32_t dt_type;        /* 1 GLOBAL, 2 LOCAL, 3 UNIFORM (extension) */
   int32_t bc1, bc2;       /* BC kinds of file markers 0 and -1: -1..-4 */
-  int32_t pad_;
+  int32_t scheme;         /* time_scheme: 1 SSP_RK2 and 2 SSP_RK3 both run SSP-RK3
+                             (main.cpp:123-126); 3 the two-stage SSP-RK2, an extension */
   double cfl;
   double gamma, R, mu, Pr;
   double P, rho, u, v;    /* [init] freestream */
@@ -64,6 +65,9 @@ int or_set_state(or_case* c, const double* p1, const double* p2, const double* p
                  const double* p4);
 /* n calls of ssp_rk3 */
 int or_step(or_case* c, int32_t n);
+/* dt > 0: every following step takes this dt in every cell (tests of the
+ * time integrators' order); 0: the scheme's own dt */
+void or_set_fixed_dt(or_case* c, double dt);
 /* one res() call on the current state (tests) */
 int or_res(or_case* c);
 void or_residual_l2(const or_case* c, double out[4]);

@@ -22,7 +22,7 @@ BC_CODES = {"FREESTREAM": -1, "SLIP_WALL": -2, "EXTRAPOLATION": -3, "NO_SLIP_WAL
 class OrOptions(ctypes.Structure):
     _fields_ = [("order", ctypes.c_int32), ("limiter", ctypes.c_int32), ("eqn", ctypes.c_int32),
                 ("flux", ctypes.c_int32), ("dt_type", ctypes.c_int32), ("bc1", ctypes.c_int32),
-                ("bc2", ctypes.c_int32), ("pad_", ctypes.c_int32), ("cfl", ctypes.c_double),
+                ("bc2", ctypes.c_int32), ("scheme", ctypes.c_int32), ("cfl", ctypes.c_double),
                 ("gamma", ctypes.c_double), ("R", ctypes.c_double), ("mu", ctypes.c_double),
                 ("Pr", ctypes.c_double), ("P", ctypes.c_double), ("rho", ctypes.c_double),
                 ("u", ctypes.c_double), ("v", ctypes.c_double)]
@@ -57,6 +57,7 @@ def lib():
         L.or_res.argtypes = [ctypes.c_void_p]
         L.or_residual_l2.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
         L.or_wall_forces.argtypes = [ctypes.c_void_p, ctypes.c_double, P(ctypes.c_double)]
+        L.or_set_fixed_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.or_roe_flux.argtypes = [P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double,
                                   ctypes.c_double, ctypes.c_double, P(ctypes.c_double)]
         L.or_array.restype = ctypes.c_void_p
@@ -79,7 +80,9 @@ def options_from_case(case) -> OrOptions:
     return OrOptions(order=case.poly + 1, limiter=lim, eqn={"EULER": 1, "NS": 2}[case.eqn],
                      flux={"RUSANOV": 1, "ROE": 2}[case.flux],
                      dt_type={"GLOBAL": 1, "LOCAL": 2, "UNIFORM": 3}[case.dt_type],
-                     bc1=BC_CODES[case.BC1], bc2=BC_CODES[case.BC2], pad_=0, cfl=case.CFL,
+                     bc1=BC_CODES[case.BC1], bc2=BC_CODES[case.BC2],
+                     scheme={"SSP_RK2": 1, "SSP_RK3": 2, "SSP_RK2_TWO_STAGE": 3}[case.scheme],
+                     cfl=case.CFL,
                      gamma=case.gamma, R=case.R, mu=case.mu, Pr=case.Pr, P=case.P, rho=case.rho,
                      u=case.u, v=case.v)
 
@@ -138,6 +141,10 @@ class Oracle:
 
     def residual(self) -> np.ndarray:
         return np.stack([self.get(f"R{k}") for k in range(1, 5)])
+
+    def set_fixed_dt(self, dt: float) -> None:
+        """dt > 0: every following step takes this dt in every cell (tests)."""
+        lib().or_set_fixed_dt(self.h, dt)
 
     def residual_l2(self) -> np.ndarray:
         out = (ctypes.c_double * 4)()
