@@ -336,13 +336,15 @@ __device__ __forceinline__ void lsq_boundary(const StageArgsT<T>& a, const FlowT
 // Pass A, second loop: the squeeze factor of one face for this cell from the
 // face state the previous res() call left on this cell's side
 // (squeeze.cpp:68-121): Q_prev + grad_prev . (x_f - x_i), or -101 at first.
+// The Barth-Jespersen limiter (limiter 3, an extension) passes the current
+// state and unlimited gradient instead, and no sentinel.
 template <typename T>
-__device__ __forceinline__ void face_phi(const StageArgsT<T>& a, const FlowT<T>& fl, T dx, T dy,
+__device__ __forceinline__ void face_phi(bool sentinel, const FlowT<T>& fl, T dx, T dy,
                                          const T q[4], const T qmin[4], const T qmax[4],
                                          const T qp[4], const T gpx[4], const T gpy[4], T ph[4]) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const T qf = a.sentinel ? fl.sent[k] : qp[k] + gpx[k] * dx + gpy[k] * dy;
+    const T qf = sentinel ? fl.sent[k] : qp[k] + gpx[k] * dx + gpy[k] * dy;
     T p = T(1.0);
     if (qf < qmin[k]) p = (qmin[k] - q[k]) / (qf - q[k]);
     else if (qf > qmax[k]) p = (qmax[k] - q[k]) / (qf - q[k]);
@@ -993,6 +995,23 @@ __device__ __forceinline__ void load_lagged(const StageArgsT<T>& a, int c, T qp[
   load_g(a.gp, c, gpx, gpy);
 }
 
+// The Barth-Jespersen limiter (limiter 3, an extension: T. J. Barth and D. C.
+// Jespersen, AIAA 89-0366, the multidimensional form of the minmod-type
+// limiter min(1, r)) bounds each face value of the CURRENT reconstruction,
+// Q + grad Q . (x_f - x_i), by the min/max of the cell and its face
+// neighbours: the squeeze's test with the current state and unlimited
+// gradient in place of the lagged ones.
+template <typename T>
+__device__ __forceinline__ void current_as_lagged(const T q[4], const T gx[4], const T gy[4],
+                                                  T qp[4], T gpx[4], T gpy[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    qp[k] = q[k];
+    gpx[k] = gx[k];
+    gpy[k] = gy[k];
+  }
+}
+
 // the reference's min/max start values 1e11 / -1e11 (squeeze.cpp:10-11)
 template <typename T>
 __device__ __forceinline__ void init_minmax(T qmin[4], T qmax[4]) {
@@ -1054,11 +1073,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_cell(StageArgsT<T> a) {
   lsq_solve(ixx, iyy, ixy, sx, sy, gx, gy);
   if (SQUEEZE) {
     T phi[4] = {1, 1, 1, 1}, qp[4], gpx[4], gpy[4];
-    load_lagged(a, c, qp, gpx, gpy);
+    if (fl.bj) current_as_lagged(q, gx, gy, qp, gpx, gpy);
+    else load_lagged(a, c, qp, gpx, gpy);
+    const bool sent = a.sentinel && !fl.bj;
     for (int j = 0; j < deg; ++j) {
       const int s = base + j * 64;
       T ph[4];
-      face_phi(a, fl, T(m.s_fcx[s] - xc), T(m.s_fcy[s] - yc), q, qmin, qmax, qp, gpx, gpy, ph);
+      face_phi(sent, fl, T(m.s_fcx[s] - xc), T(m.s_fcy[s] - yc), q, qmin, qmax, qp, gpx, gpy, ph);
 #pragma unroll
       for (int k = 0; k < 4; ++k) phi[k] = min2(ph[k], phi[k]);
     }
@@ -1117,7 +1138,7 @@ template <typename T, bool SQUEEZE>
 __global__ __launch_bounds__(kMaxBlock) void k_pass_a_slot(StageArgsT<T> a) {
   const FlowT<T>& fl = a.flows[0];   // single-instance shape
   __shared__ T t1[kRows][16][64];        // per row: px[4] py[4] tmin[4] tmax[4]
-  __shared__ T cv[8][64];                // qmin[4] qmax[4] of each cell
+  __shared__ T cv[16][64];               // qmin[4] qmax[4] (and gx[4] gy[4]) of each cell
   const DevMesh& m = a.m;
   const int chunk = logical_block(a.xcd_blocks);
   if (chunk * 64 >= m.ncp) return;         // whole block idle (uniform)
@@ -1190,6 +1211,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_slot(StageArgsT<T> a) {
       for (int k = 0; k < 4; ++k) {
         cv[k][lane] = qmin[k];
         cv[4 + k][lane] = qmax[k];
+        if (fl.bj && active) {
+          cv[8 + k][lane] = gx[k];
+          cv[12 + k][lane] = gy[k];
+        }
       }
     }
     __syncthreads();
@@ -1199,14 +1224,27 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_a_slot(StageArgsT<T> a) {
       qmax[k] = cv[4 + k][lane];
     }
     T qp[4] = {0, 0, 0, 0}, gpx[4] = {0, 0, 0, 0}, gpy[4] = {0, 0, 0, 0};
-    if (active) load_lagged(a, c, qp, gpx, gpy);
+    if (active) {
+      if (fl.bj) {
+        T ux[4], uy[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          ux[k] = cv[8 + k][lane];
+          uy[k] = cv[12 + k][lane];
+        }
+        current_as_lagged(q, ux, uy, qp, gpx, gpy);
+      } else {
+        load_lagged(a, c, qp, gpx, gpy);
+      }
+    }
+    const bool sent = a.sentinel && !fl.bj;
     T phi[4] = {1, 1, 1, 1};
     for (int j0 = 0; j0 < nrows; j0 += kRows) {
       const int j = j0 + row;
       if (j < deg) {
         const int s = base + j * 64;
         T ph[4];
-        face_phi(a, fl, T(m.s_fcx[s] - xc), T(m.s_fcy[s] - yc), q, qmin, qmax, qp, gpx, gpy, ph);
+        face_phi(sent, fl, T(m.s_fcx[s] - xc), T(m.s_fcy[s] - yc), q, qmin, qmax, qp, gpx, gpy, ph);
 #pragma unroll
         for (int k = 0; k < 4; ++k) t1[row][k][lane] = ph[k];
       }
@@ -1442,7 +1480,7 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
   T qp[4], gpx[4], gpy[4];
   if (FVK_A_LATE < 1) {
     load_pf();
-    if (squeeze && deg > 0) load_lagged(a, c, qp, gpx, gpy);
+    if (squeeze && !fl.bj && deg > 0) load_lagged(a, c, qp, gpx, gpy);
   }
   double ixx = 0, iyy = 0, ixy = 0;
   if (FVK_A_LATE < 2 && deg > 0) load_lsq_inv(m, c, ixx, iyy, ixy);
@@ -1478,17 +1516,19 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
   for (int j = PF; j < deg; ++j) slot(j, m.lcode[base + j * 64]);
   if (FVK_A_LATE >= 1) {
     load_pf();
-    if (squeeze) load_lagged(a, c, qp, gpx, gpy);
+    if (squeeze && !fl.bj) load_lagged(a, c, qp, gpx, gpy);
   }
   if (FVK_A_LATE >= 2) load_lsq_inv(m, c, ixx, iyy, ixy);
   T gx[4], gy[4];
   lsq_solve(ixx, iyy, ixy, sx, sy, gx, gy);
   if (squeeze) {
     T phi[4] = {1, 1, 1, 1};
+    if (fl.bj) current_as_lagged(q, gx, gy, qp, gpx, gpy);
+    const bool sent = a.sentinel && !fl.bj;
     auto lim = [&](SlotPos<T> px, SlotPos<T> py) {
       T ph[4];
-      if constexpr (kDelta<T>) face_phi(a, fl, px, py, q, qmin, qmax, qp, gpx, gpy, ph);
-      else face_phi(a, fl, T(px - xc), T(py - yc), q, qmin, qmax, qp, gpx, gpy, ph);
+      if constexpr (kDelta<T>) face_phi(sent, fl, px, py, q, qmin, qmax, qp, gpx, gpy, ph);
+      else face_phi(sent, fl, T(px - xc), T(py - yc), q, qmin, qmax, qp, gpx, gpy, ph);
 #pragma unroll
       for (int k = 0; k < 4; ++k) phi[k] = min2(ph[k], phi[k]);
     };

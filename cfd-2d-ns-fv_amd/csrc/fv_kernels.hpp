@@ -81,7 +81,8 @@ struct FlowT {
   T sent[4];                   // the -101 initial face state, in stored-state units
   double qinf64[4];            // FP64 copies for steps an FP32 path takes in FP64
   double gm1_64, Rgas64, mu64, k64;
-  int32_t squeeze;             // limiter SQUEEZE (1) or none (0)
+  int32_t squeeze;             // limiter SQUEEZE or Barth-Jespersen (1) or none (0)
+  int32_t bj;                  // limiter 3, Barth-Jespersen: the current reconstruction, unlagged
   int32_t first_block;         // first mesh block of the instance
 };
 

@@ -225,7 +225,7 @@ inline int stages_of(const fvk_options& o) { return o.time_scheme == 3 ? 2 : 3; 
 
 int check_options(const fvk_options& o, std::string* msg) {
   if (o.order != 1 && o.order != 2) { *msg = "order must be 1 or 2 (solution_poly_order 0 or 1)"; return FVK_ERR_OPTION; }
-  if (o.limiter < 0 || o.limiter > 2) { *msg = "limiter must be NONE, MINMOD or SQUEEZE"; return FVK_ERR_OPTION; }
+  if (o.limiter < 0 || o.limiter > 3) { *msg = "limiter must be NONE, MINMOD, SQUEEZE or 3 (Barth-Jespersen)"; return FVK_ERR_OPTION; }
   if (o.eqn != 1 && o.eqn != 2) { *msg = "eqn must be EULER or NS"; return FVK_ERR_OPTION; }
   if (o.flux_solver != 1 && o.flux_solver != 2) { *msg = "flux_solver must be RUSANOV or ROE"; return FVK_ERR_OPTION; }
   if (o.flux_solver == 2 && ((o.kernel_variant != 0 && o.kernel_variant != fvk::kVariantLds) || o.accumulate)) {
@@ -302,7 +302,8 @@ fvk::FlowT<T> make_flow(const fvk_options& o, bool delta, int32_t first_block) {
   fl.Rgas64 = o.R;
   fl.mu64 = o.mu;
   fl.k64 = (o.mu * Cp) / o.Pr;
-  fl.squeeze = o.limiter == 2;
+  fl.squeeze = o.limiter == 2 || o.limiter == 3;
+  fl.bj = o.limiter == 3;
   fl.first_block = first_block;
   return fl;
 }
@@ -390,7 +391,7 @@ void* elem(Engine* c, void* base, size_t index) {
 
 bool any_squeeze(const Engine* c) {
   for (const Instance& i : c->in)
-    if (i.opt.limiter == 2) return true;
+    if (i.opt.limiter == 2 || i.opt.limiter == 3) return true;
   return false;
 }
 

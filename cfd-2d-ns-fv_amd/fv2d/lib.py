@@ -148,7 +148,7 @@ def make_options(case, **overrides) -> FvkOptions:
     o = FvkOptions()
     lib().fvk_default_options(ctypes.byref(o))
     o.order = case.poly + 1
-    o.limiter = {"NONE": 0, "MINMOD": 1, "SQUEEZE": 2}[case.limiter]
+    o.limiter = {"NONE": 0, "MINMOD": 1, "SQUEEZE": 2, "BARTH_JESPERSEN": 3}[case.limiter]
     o.eqn = {"EULER": 1, "NS": 2}[case.eqn]
     o.flux_solver = {"RUSANOV": 1, "ROE": 2}[case.flux]
     o.time_scheme = {"SSP_RK2": 1, "SSP_RK3": 2, "SSP_RK2_TWO_STAGE": 3}[case.scheme]

@@ -67,7 +67,10 @@ typedef struct fvk_mesh {
  * class_flow.hpp) with read_inputs.cpp's enum codes. */
 typedef struct fvk_options {
   int32_t order;           /* solution_poly_order + 1: 1 or 2 */
-  int32_t limiter;         /* 0 NONE, 1 MINMOD (a no-op, res.cpp:36-38), 2 SQUEEZE */
+  int32_t limiter;         /* 0 NONE, 1 MINMOD (a no-op, res.cpp:36-38), 2 SQUEEZE;
+                              3 Barth-Jespersen, an extension (the reference's minmod.cpp
+                              is a draft it never builds): the multidimensional
+                              minmod-type limiter on the current reconstruction */
   int32_t eqn;             /* 1 EULER, 2 NS */
   int32_t flux_solver;     /* 1 RUSANOV; 2 ROE, an extension: the reference reads it
                               (read_inputs.cpp:89-91) but its Roe flux is an unfinished

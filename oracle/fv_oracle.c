@@ -430,6 +430,66 @@ static void squeeze(or_case* C, qset* Q) {
   for (int k = 0; k < 4; ++k) { free(qmin[k]); free(qmax[k]); free(phc1[k]); free(phc2[k]); free(ph[k]); }
 }
 
+/* Barth-Jespersen limiter (limiter 3): an EXTENSION.  The reference maps
+ * MINMOD to a no-op (res.cpp:36-38) and never builds its minmod.cpp draft;
+ * limiter 1 keeps that.  This is the multidimensional minmod-type limiter of
+ * T. J. Barth and D. C. Jespersen (AIAA 89-0366): squeeze's bounds and test
+ * (the min/max of each cell and its face neighbours, phi = min over the
+ * cell's faces of the factor that brings the face value inside them) applied
+ * to the CURRENT reconstruction Q + grad Q . (x_f - x_i) instead of the
+ * previous res() call's face states.  Pinned by tests/test_oracle_bj.py. */
+static void barth_jespersen(or_case* C, qset* Q) {
+  int nf = C->nf, nc = C->nc;
+  double *qmin[4], *qmax[4], *ph[4];
+  for (int k = 0; k < 4; ++k) {
+    qmin[k] = dfill(nc, 100000000000.0);
+    qmax[k] = dfill(nc, -100000000000.0);
+    ph[k] = dfill(nc, 1);
+  }
+  for (int f = 0; f < nf; ++f) {   /* the bounds, as squeeze */
+    int c1 = C->fc1[f], c2 = C->fc2[f];
+    double tmin[4], tmax[4];
+    if (c2 >= 0) {
+      for (int k = 0; k < 4; ++k) {
+        tmin[k] = min2(Q->p[k][c1], Q->p[k][c2]);
+        tmax[k] = max2(Q->p[k][c1], Q->p[k][c2]);
+      }
+      for (int k = 0; k < 4; ++k) {
+        qmin[k][c2] = min2(qmin[k][c2], tmin[k]);
+        qmax[k][c2] = max2(qmax[k][c2], tmax[k]);
+      }
+    } else {
+      for (int k = 0; k < 4; ++k) { tmin[k] = Q->p[k][c1]; tmax[k] = Q->p[k][c1]; }
+    }
+    for (int k = 0; k < 4; ++k) {
+      qmin[k][c1] = min2(qmin[k][c1], tmin[k]);
+      qmax[k][c1] = max2(qmax[k][c1], tmax[k]);
+    }
+  }
+  for (int f = 0; f < nf; ++f) {
+    int cs[2] = {C->fc1[f], C->fc2[f]};
+    const double* dxs[2] = {C->c1dx, C->c2dx};
+    const double* dys[2] = {C->c1dy, C->c2dy};
+    for (int side = 0; side < 2; ++side) {
+      int c = cs[side];
+      if (c < 0) continue;
+      for (int k = 0; k < 4; ++k) {
+        double q = Q->p[k][c] + Q->qx[k][c] * dxs[side][f] + Q->qy[k][c] * dys[side][f];
+        double p = 1.0;
+        if (q < qmin[k][c]) p = (qmin[k][c] - Q->p[k][c]) / (q - Q->p[k][c]);
+        else if (q > qmax[k][c]) p = (qmax[k][c] - Q->p[k][c]) / (q - Q->p[k][c]);
+        ph[k][c] = min2(p, ph[k][c]);
+      }
+    }
+  }
+  for (int c = 0; c < nc; ++c)
+    for (int k = 0; k < 4; ++k) {
+      Q->qx[k][c] = Q->qx[k][c] * ph[k][c];
+      Q->qy[k][c] = Q->qy[k][c] * ph[k][c];
+    }
+  for (int k = 0; k < 4; ++k) { free(qmin[k]); free(qmax[k]); free(ph[k]); }
+}
+
 /* reconstruction  reconstruction.cpp:15-167 */
 static void reconstruction(or_case* C, qset* Q) {
   int nf = C->nf;
@@ -595,6 +655,7 @@ static void res(or_case* C, qset* Q) {
   } else {
     compute_gradient(C, Q);
     if (C->o.limiter == 2) squeeze(C, Q);
+    else if (C->o.limiter == 3) barth_jespersen(C, Q);
     reconstruction(C, Q);
     if (C->o.eqn == 2) {           /* class_F::update_vis  class_f.hpp:52-59 */
       qset* F = &C->Qface;

@@ -38,7 +38,8 @@ typedef struct or_case or_case;
 
 typedef struct {
   int32_t order;          /* solution_poly_order + 1 (1 or 2) */
-  int32_t limiter;        /* 0 NONE, 1 MINMOD (no-op), 2 SQUEEZE */
+  int32_t limiter;        /* 0 NONE, 1 MINMOD (no-op), 2 SQUEEZE, 3 Barth-Jespersen (an
+                             extension) */
   int32_t eqn;            /* 1 EULER, 2 NS */
   int32_t flux;           /* 1 RUSANOV, 2 ROE: an extension (the reference leaves the
                              flux at -101, res.cpp:66-68), see or_roe_flux */

@@ -76,7 +76,7 @@ def _ip(a):
 
 def options_from_case(case) -> OrOptions:
     """OrOptions from a fv2d.inputs.Case (same enum codes as read_inputs.cpp)."""
-    lim = {"NONE": 0, "MINMOD": 1, "SQUEEZE": 2}[case.limiter]
+    lim = {"NONE": 0, "MINMOD": 1, "SQUEEZE": 2, "BARTH_JESPERSEN": 3}[case.limiter]
     return OrOptions(order=case.poly + 1, limiter=lim, eqn={"EULER": 1, "NS": 2}[case.eqn],
                      flux={"RUSANOV": 1, "ROE": 2}[case.flux],
                      dt_type={"GLOBAL": 1, "LOCAL": 2, "UNIFORM": 3}[case.dt_type],
