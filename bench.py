@@ -88,7 +88,7 @@ def max_faces(grid) -> int:
     return int(deg.max())
 
 
-def cpu_baseline(samples, target_s: float = 15.0):
+def cpu_baseline(samples, target_s: float = 15.0, all_cores: bool = False):
     """Time the reference solver (oracle/_ref/ref_harness) or the C oracle on
     one core over a bounded number of steps of each (grid, case) sample.  The
     reference stores at most 6 faces per cell (readmesh.cpp:38), so meshes
@@ -100,6 +100,7 @@ def cpu_baseline(samples, target_s: float = 15.0):
     work = 0.0
     secs = 0.0
     parts = []
+    runs = []
     for grid, case in samples:
         nc = grid.num_cells
         if harness and os.path.exists(harness):
@@ -130,10 +131,53 @@ def cpu_baseline(samples, target_s: float = 15.0):
         work += nc * 3 * n
         secs += t
         parts.append(f"{nc} cells x {n} steps")
+        if kind == "reference":
+            runs.append((d, nc, n, t))
     what = ("reference solver built from its own sources" if kind == "reference"
             else "C oracle (serial restatement)")
-    return {"value": work / secs, "unit": "cell-stage/s", "cores": 1, "kind": kind,
-            "sample": f"{what}, {'; '.join(parts)} of the same workload ({secs:.2f} s of step loop)"}
+    out = {"value": work / secs, "unit": "cell-stage/s", "cores": 1, "kind": kind,
+           "sample": f"{what}, {'; '.join(parts)} of the same workload ({secs:.2f} s of step loop)"}
+    if all_cores and runs:
+        out["all_cores"] = cpu_all_cores(harness, runs)
+    import shutil
+    for d, _, _, _ in runs:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
+def cpu_all_cores(harness, runs, max_procs: int = 16):
+    """The same samples as concurrent single-core reference processes, one
+    per core of the host's share (independent instances need no more): P
+    processes, process j pinned to core j and running sample j mod len(runs)
+    for that sample's step count; value = all their cell-stages over the
+    group's wall time."""
+    import shutil
+    cores = sorted(os.sched_getaffinity(0))[:max_procs]
+    procs, work = [], 0.0
+    dirs = []
+    for j, core in enumerate(cores):
+        d, nc, n, _ = runs[j % len(runs)]
+        dj = tempfile.mkdtemp()
+        for f in ("grid.in", "input.in"):
+            shutil.copy(os.path.join(d, f), dj)
+        dirs.append(dj)
+        work += nc * 3 * n
+        procs.append((core, dj, n))
+    t0 = time.perf_counter()
+    running = [subprocess.Popen(["taskset", "-c", str(core), harness, "input.in", "out.bin",
+                                 str(n), "--dump", "0", "--time"], cwd=dj,
+                                stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+               for core, dj, n in procs]
+    rcs = [p.wait() for p in running]
+    wall = time.perf_counter() - t0
+    for dj in dirs:
+        shutil.rmtree(dj, ignore_errors=True)
+    if any(rcs):
+        return {"error": f"exit codes {rcs}"}
+    return {"value": work / wall, "unit": "cell-stage/s", "cores": len(procs),
+            "kind": "reference",
+            "how": f"{len(procs)} concurrent single-core reference processes (the samples in "
+                   f"turn), {wall:.2f} s wall"}
 
 
 def copy_calibration(mbytes: int) -> None:
@@ -814,7 +858,7 @@ def bench_batch(args):
         # a bounded sample: the smallest, the median and the largest instance
         order = np.argsort(nc)
         pick = [order[0], order[len(order) // 2], order[-1]]
-        out["cpu_baseline"] = cpu_baseline([cases[i] for i in pick])
+        out["cpu_baseline"] = cpu_baseline([cases[i] for i in pick], all_cores=True)
     return out
 
 
