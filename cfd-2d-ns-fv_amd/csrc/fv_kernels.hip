@@ -1391,45 +1391,76 @@ constexpr int kMinBlocksA = kDelta<T> ? FVK_MINBLK_A32 : FVK_MINBLK_A64;
 template <typename T>
 constexpr int kMinBlocksB = kDelta<T> ? FVK_MINBLK_B32 : FVK_MINBLK_B64;
 
+// A block's staged cells in LDS: one record of NQ values per cell (state,
+// then the limited gradient's x and y parts), and the centroids as (x, y)
+// pairs.  A record is a whole number of 16-byte words, so a cell's state or
+// gradient part moves with one 128-bit LDS access (FP32) or two (FP64)
+// instead of four 32- or 64-bit ones at four addresses.
 template <typename T, int NQ, bool XY>
 struct LdsCells {
-  T* q;        // [NQ][stride]
-  double* xy;  // [2][stride] (XY only)
+  static_assert((NQ * sizeof(T)) % 16 == 0, "records of whole 16-byte words");
+  T* q;        // [stride][NQ]
+  double* xy;  // [stride][2] (XY only)
   int stride;
   __device__ LdsCells(unsigned char* base, int s)
       : q(reinterpret_cast<T*>(base)),
-        xy(reinterpret_cast<double*>(base + ((NQ * s * (int)sizeof(T) + 7) & ~7))),
+        xy(reinterpret_cast<double*>(base + NQ * s * (int)sizeof(T))),
         stride(s) {}
   __device__ size_t bytes() const {
-    return ((NQ * stride * sizeof(T) + 7) & ~size_t(7)) + (XY ? 2 * stride * sizeof(double) : 0);
+    return NQ * stride * sizeof(T) + (XY ? 2 * stride * sizeof(double) : 0);
   }
+  __device__ T* rec(int l) const { return q + NQ * l; }
 };
+
+// four consecutive values of a 16-byte aligned LDS record
+template <typename T>
+__device__ __forceinline__ void lds_ld4(const T* p, T v[4]) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+  } else {
+    const double2 t0 = *reinterpret_cast<const double2*>(p);
+    const double2 t1 = *reinterpret_cast<const double2*>(p + 2);
+    v[0] = t0.x; v[1] = t0.y; v[2] = t1.x; v[3] = t1.y;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_st4(T* p, const T v[4]) {
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+  } else {
+    *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
+  }
+}
+
+// centroid of staged cell l
+template <typename T, int NQ, bool XY>
+__device__ __forceinline__ void lds_xy(const LdsCells<T, NQ, XY>& L, int l, double& x, double& y) {
+  const double2 c = *reinterpret_cast<const double2*>(L.xy + 2 * l);
+  x = c.x;
+  y = c.y;
+}
 
 template <typename T, int NQ, bool XY>
 __device__ __forceinline__ void stage_block(const StageArgsT<T>& a, int b, LdsCells<T, NQ, XY>& L) {
   const DevMesh& m = a.m;
   const int hb = m.halo_ptr[b], nh = m.halo_ptr[b + 1] - hb;
   const int n = m.blk + nh;
-  const int st = L.stride;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int gc = i < m.blk ? b * m.blk + i : m.halo_idx[hb + i - m.blk];
     T v[4];
     load_q(a.qc, gc, v);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) L.q[k * st + i] = v[k];
+    T* r = L.rec(i);
+    lds_st4(r, v);
     if (NQ == 12) {
       T gx[4], gy[4];
       load_g(a.g, gc, gx, gy);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        L.q[(4 + k) * st + i] = gx[k];
-        L.q[(8 + k) * st + i] = gy[k];
-      }
+      lds_st4(r + 4, gx);
+      lds_st4(r + 8, gy);
     }
-    if (XY) {
-      L.xy[i] = m.ccx[gc];
-      L.xy[st + i] = m.ccy[gc];
-    }
+    if (XY) *reinterpret_cast<double2*>(L.xy + 2 * i) = make_double2(m.ccx[gc], m.ccy[gc]);
   }
 }
 
@@ -1452,14 +1483,13 @@ __device__ __forceinline__ void slot_pos(const DevMesh& m, int s, SlotPos<T>& px
 
 template <typename T, bool SQUEEZE, int PF>
 __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageArgsT<T> a) {
-  extern __shared__ unsigned char lds_raw[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   const DevMesh& m = a.m;
   int b, inst;
   if (!block_of(a, logical_block(a.xcd_blocks), b, inst)) return;   // uniform over the block
   const FlowT<T>& fl = a.flows[inst];
   const bool squeeze = SQUEEZE && fl.squeeze;
   LdsCells<T, 4, true> L(lds_raw, m.blk + m.max_halo);
-  const int st = L.stride;
   const int i = threadIdx.x;
   const int c = b * m.blk + i;
   const int deg = m.deg[c];
@@ -1488,9 +1518,9 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
   __syncthreads();
   if (deg == 0) return;
   T q[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) q[k] = L.q[k * st + i];
-  const double xc = L.xy[i], yc = L.xy[st + i];
+  lds_ld4(L.rec(i), q);
+  double xc, yc;
+  lds_xy(L, i, xc, yc);
   T sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, qmin[4], qmax[4];
   init_minmax(qmin, qmax);
   auto slot = [&](int j, int code) {
@@ -1498,9 +1528,10 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
     if (code >= 0) {
       const int nb = code >> 1;
       T qn[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) qn[k] = L.q[k * st + nb];
-      lsq_interior(code & 1, q, xc, yc, qn, L.xy[nb], L.xy[st + nb], px, py, tmin, tmax);
+      lds_ld4(L.rec(nb), qn);
+      double xn, yn;
+      lds_xy(L, nb, xn, yn);
+      lsq_interior(code & 1, q, xc, yc, qn, xn, yn, px, py, tmin, tmax);
     } else {
       // the 16-bit local code holds the kind only; the global code also
       // names the no-slip ghost's source cell
@@ -1594,16 +1625,20 @@ __device__ __forceinline__ void load_face(const DevMesh& m, int e, FaceRec<T, NS
 
 template <typename T, int ORDER, int NQ, bool XY>
 __device__ __forceinline__ void lds_cell(const LdsCells<T, NQ, XY>& L, int l, CellQG<T>& d) {
-  const int st = L.stride;
+  const T* r = L.rec(l);
+  lds_ld4(r, d.q);
+  if constexpr (ORDER == 2) {
+    lds_ld4(r + 4, d.gx);
+    lds_ld4(r + 8, d.gy);
+  } else {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) d.q[k] = L.q[k * st + l];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    d.gx[k] = ORDER == 2 ? L.q[(4 + k) * st + l] : T(0);
-    d.gy[k] = ORDER == 2 ? L.q[(8 + k) * st + l] : T(0);
+    for (int k = 0; k < 4; ++k) d.gx[k] = d.gy[k] = T(0);
   }
-  d.x = XY ? L.xy[l] : 0.0;
-  d.y = XY ? L.xy[st + l] : 0.0;
+  if constexpr (XY) {
+    lds_xy(L, l, d.x, d.y);
+  } else {
+    d.x = d.y = 0.0;
+  }
 }
 
 template <typename T, int ORDER, bool NS, int PF, bool ROE = false>
@@ -1611,7 +1646,7 @@ __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
 void k_pass_b_lds(StageArgsT<T> a) {
   // cell rows, then [4 or 5][max_bfaces] face terms (the spectral-sum term
   // only at stage 3, where the next step's dt needs it)
-  extern __shared__ unsigned char lds_raw[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int NQ = ORDER == 2 ? 12 : 4;
   constexpr bool XY = !kDelta<T>;
   constexpr bool kLate = FVK_NS_LATE && NS && !kDelta<T>;   // FP64 NS: the leaner prefetch
@@ -1711,8 +1746,7 @@ void k_pass_b_lds(StageArgsT<T> a) {
         if (j < deg) add(sf[j]);
       for (int j = PF; j < deg; ++j) add(m.sface[base + j * 64]);
       T q[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) q[k] = L.q[k * L.stride + i];
+      lds_ld4(L.rec(i), q);
       cell_update(a, fl, inst, c, q, qn, r, sg, V, sig_old);
       sig_new = dt_key(a, sg, V);
     }
@@ -1732,7 +1766,7 @@ void k_pass_b_lds(StageArgsT<T> a) {
 template <typename T, int ORDER, bool NS>
 __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
 void k_pass_b_atomic(StageArgsT<T> a) {
-  extern __shared__ unsigned char lds_raw[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int NQ = ORDER == 2 ? 12 : 4;
   constexpr bool XY = !kDelta<T>;
   const DevMesh& m = a.m;
@@ -1978,7 +2012,7 @@ void shape(const StageArgsT<T>& a, int nlogical, int* grid, int* per_xcd) {
 }
 
 size_t lds_cells_bytes(int stride, int nq, size_t tsize, bool xy = true) {
-  return ((nq * (size_t)stride * tsize + 7) & ~size_t(7)) + (xy ? 2 * (size_t)stride * 8 : 0);
+  return nq * (size_t)stride * tsize + (xy ? 2 * (size_t)stride * 8 : 0);
 }
 
 }  // namespace
