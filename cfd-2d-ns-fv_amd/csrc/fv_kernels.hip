@@ -47,6 +47,11 @@
 #define FVK_FAST32 1
 #endif
 // FP32-mixed NS: 1 = the viscous face flux in FP64, 0 = in FP32.
+// FP32-mixed squeeze factors with one hardware reciprocal instead of a
+// correctly rounded quotient
+#ifndef FVK_A_RCP
+#define FVK_A_RCP 1
+#endif
 #ifndef FVK_NS64
 #define FVK_NS64 0
 #endif
@@ -346,6 +351,20 @@ __device__ __forceinline__ void face_phi(bool sentinel, const FlowT<T>& fl, T dx
   for (int k = 0; k < 4; ++k) {
     const T qf = sentinel ? fl.sent[k] : qp[k] + gpx[k] * dx + gpy[k] * dy;
     T p = T(1.0);
+#if FVK_FAST32 && FVK_A_RCP
+    if constexpr (kDelta<T>) {
+      // FP32-mixed: one hardware reciprocal (1 ulp); the factor stays within
+      // [0, 1 + ulp] and the minimum with 1 below caps it.  A cell at its
+      // own bound gets 0 outright: the quotient's 0 / (qf - q) would be
+      // 0 * inf when qf - q is too small for the reciprocal.
+      const T bound = qf < qmin[k] ? qmin[k] : qmax[k];
+      const T num = bound - q[k];
+      if (qf < qmin[k] || qf > qmax[k])
+        p = num == T(0) ? T(0) : num * __builtin_amdgcn_rcpf(qf - q[k]);
+      ph[k] = p;
+      continue;
+    }
+#endif
     if (qf < qmin[k]) p = (qmin[k] - q[k]) / (qf - q[k]);
     else if (qf > qmax[k]) p = (qmax[k] - q[k]) / (qf - q[k]);
     ph[k] = p;
