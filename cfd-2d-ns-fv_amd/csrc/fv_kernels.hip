@@ -1776,11 +1776,18 @@ void k_pass_b_lds(StageArgsT<T> a) {
 
 // ---------------------------------------------------------------------------
 // Atomic-scatter accumulation (accumulate = 1), the alternative to the owner
-// gather above.  Pass B computes each face once, in the block that holds the
-// face's owner, and adds its terms to both cells' accumulators with hardware
-// atomics; a cell pass then applies the stage update and clears them.  The
-// sums take their terms in run-time order, so results differ from the
-// gather (and the reference) by summation order only.
+// gather above.  Each block computes the faces of its cells, as the gather
+// does, and scatters each face's terms with LDS atomics (ds_add) into the
+// accumulators of whichever of the face's two cells are its own; the halo
+// side of a face on a block boundary is skipped, since the neighbouring block
+// computes that face too.  The stage update then reads each cell's sums from
+// LDS.  No device-scope atomics: on gfx950 those are performed beyond the
+// per-XCD L2 caches, at a small fraction of the memory bandwidth.  The
+// accumulators are FP64 in both precisions: ds_add_f64 keeps this pass within
+// 1.1x of the gather on the 256-instance batch, where ds_add_f32 made it 2.8x
+// (and the FP32-mixed face terms are summed in FP64).  The sums take their
+// terms in run-time order, so results differ from the gather (and the
+// reference) by summation order only.
 // ---------------------------------------------------------------------------
 template <typename T, int ORDER, bool NS>
 __global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
@@ -1790,78 +1797,69 @@ void k_pass_b_atomic(StageArgsT<T> a) {
   constexpr bool XY = !kDelta<T>;
   const DevMesh& m = a.m;
   int b = 0, inst = 0;
-  if (!block_of(a, logical_block(a.xcd_blocks), b, inst)) return;   // uniform
+  const bool live = block_of(a, logical_block(a.xcd_blocks), b, inst);   // uniform
   const FlowT<T>& fl = a.flows[inst];
-  LdsCells<T, NQ, XY> L(lds_raw, m.blk + m.max_halo);
-  const int fb = m.bf_ptr[b], nfb = m.bf_ptr[b + 1] - fb;
-  const int hb = m.halo_ptr[b];
-  stage_block(a, b, L);
-  __syncthreads();
-  for (int e = threadIdx.x; e < nfb; e += blockDim.x) {
-    FaceRec<T, NS> f;
-    load_face(m, fb + e, f);
-    if (f.ia >= m.blk) continue;                    // the owner's block takes the face
-    const int g1 = b * m.blk + f.ia;
-    CellQG<T> c1, c2;
-    lds_cell<T, ORDER>(L, f.ia, c1);
-    T x[4], st;
-    if (f.ib >= 0) {
-      lds_cell<T, ORDER>(L, f.ib, c2);
-      FaceGeo<T> g;
-      if constexpr (kDelta<T>) g = f.g;
-      else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, &c2, -1);
-      face_flux<ORDER, NS>(a, fl, c1, &c2, -1, g, x, st);
-      const int g2 = f.ib < m.blk ? b * m.blk + f.ib : m.halo_idx[hb + f.ib - m.blk];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        unsafeAtomicAdd(a.acc + 4 * (size_t)g1 + k, x[k]);
-        unsafeAtomicAdd(a.acc + 4 * (size_t)g2 + k, -x[k]);
-      }
-      if (a.last) {
-        unsafeAtomicAdd(a.sacc + g1, st);
-        unsafeAtomicAdd(a.sacc + g2, st);
-      }
-    } else {
-      const int kind = (~f.ib) & 3;
-      FaceGeo<T> g;
-      if constexpr (kDelta<T>) g = f.g;
-      else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, nullptr, kind);
-      face_flux<ORDER, NS>(a, fl, c1, (const CellQG<T>*)nullptr, kind, g, x, st);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) unsafeAtomicAdd(a.acc + 4 * (size_t)g1 + k, x[k]);
-      if (a.last) unsafeAtomicAdd(a.sacc + g1, st);
-    }
-  }
-}
-
-// The stage update from the accumulated sums; clears the accumulators for the
-// next stage.  One block per mesh block (blockDim = blk).
-template <typename T>
-__global__ __launch_bounds__(kMaxBlock) void k_update_atomic(StageArgsT<T> a) {
-  const DevMesh& m = a.m;
-  int b = 0, inst = 0;
-  if (!block_of(a, logical_block(a.xcd_blocks), b, inst)) return;   // uniform
-  const FlowT<T>& fl = a.flows[inst];
-  const int c = b * m.blk + threadIdx.x;
   double sig_new = 0.0;
-  if (m.deg[c] > 0) {
-    T r[4], q[4], qn[4] = {0, 0, 0, 0};
-    ld4(a.acc + 4 * (size_t)c, r);
-    const T zero[4] = {0, 0, 0, 0};
-    st4(a.acc + 4 * (size_t)c, zero);
-    T sg = 0;
-    if (a.last) {
-      sg = a.sacc[c];
-      a.sacc[c] = T(0);
+  if (live) {
+    LdsCells<T, NQ, XY> L(lds_raw, m.blk + m.max_halo);
+    // [5][blk]: the face sums and the spectral sum of each cell of the block
+    double* acl = reinterpret_cast<double*>(lds_raw + ((L.bytes() + 7) & ~size_t(7)));
+    const int fb = m.bf_ptr[b], nfb = m.bf_ptr[b + 1] - fb;
+    const int i = threadIdx.x;
+    if (i < m.blk) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) acl[k * m.blk + i] = 0.0;
     }
-    load_q(a.qc, c, q);
-    if (a.stage > 1) load_q(a.qn, c, qn);
-    const T sig_old = a.dt_global ? T(1) : a.sigma[c];
-    cell_update(a, fl, inst, c, q, qn, r, sg, m.vol[c], sig_old);
-    sig_new = dt_key(a, sg, m.vol[c]);
+    stage_block(a, b, L);
+    __syncthreads();
+    for (int e = i; e < nfb; e += blockDim.x) {
+      FaceRec<T, NS> f;
+      load_face(m, fb + e, f);
+      CellQG<T> c1, c2;
+      lds_cell<T, ORDER>(L, f.ia, c1);
+      T x[4], st;
+      if (f.ib >= 0) {
+        lds_cell<T, ORDER>(L, f.ib, c2);
+        FaceGeo<T> g;
+        if constexpr (kDelta<T>) g = f.g;
+        else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, &c2, -1);
+        face_flux<ORDER, NS>(a, fl, c1, &c2, -1, g, x, st);
+        if (f.ib < m.blk) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) atomicAdd(acl + k * m.blk + f.ib, -double(x[k]));
+          if (a.last) atomicAdd(acl + 4 * m.blk + f.ib, double(st));
+        }
+      } else {
+        const int kind = (~f.ib) & 3;
+        FaceGeo<T> g;
+        if constexpr (kDelta<T>) g = f.g;
+        else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, nullptr, kind);
+        face_flux<ORDER, NS>(a, fl, c1, (const CellQG<T>*)nullptr, kind, g, x, st);
+      }
+      if (f.ia < m.blk) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) atomicAdd(acl + k * m.blk + f.ia, double(x[k]));
+        if (a.last) atomicAdd(acl + 4 * m.blk + f.ia, double(st));
+      }
+    }
+    __syncthreads();
+    if (i < m.blk) {
+      const int c = b * m.blk + i;
+      if (m.deg[c] > 0) {
+        T r[4], q[4], qn[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = T(acl[k * m.blk + i]);
+        const T sg = T(acl[4 * m.blk + i]);
+        lds_ld4(L.rec(i), q);
+        if (a.stage > 1) load_q(a.qn, c, qn);
+        const T sig_old = a.dt_global ? T(1) : a.sigma[c];
+        cell_update(a, fl, inst, c, q, qn, r, sg, m.vol[c], sig_old);
+        sig_new = dt_key(a, sg, m.vol[c]);
+      }
+    }
   }
-  if (a.last && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
-  if (a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) step_start(a, fl, inst);
+  if (live && a.last && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
+  if (live && a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) step_start(a, fl, inst);
 }
 
 // Spectral sums of the first step: every face carries |V_inf| and c_inf
@@ -2045,9 +2043,11 @@ size_t lds_bytes_pass_b(const DevMesh& m, int order, bool last, size_t tsize) {
          (last ? 5 : 4) * (size_t)m.max_bfaces * tsize;
 }
 
-// cell rows only (the atomic-scatter pass B keeps no face terms in LDS)
-static size_t lds_bytes_cells(const DevMesh& m, int order, size_t tsize) {
-  return lds_cells_bytes(m.blk + m.max_halo, order == 2 ? 12 : 4, tsize, tsize == 8);
+// the atomic-scatter pass B: cell rows, then (8-byte aligned) the block's
+// cells' five FP64 sums (no face terms)
+static size_t lds_bytes_atomic(const DevMesh& m, int order, size_t tsize) {
+  const size_t cells = lds_cells_bytes(m.blk + m.max_halo, order == 2 ? 12 : 4, tsize, tsize == 8);
+  return ((cells + 7) & ~size_t(7)) + 5 * (size_t)m.blk * sizeof(double);
 }
 
 template <typename T>
@@ -2087,13 +2087,10 @@ hipError_t launch_pass_b(const StageArgsT<T>& a0, int order, int ns, int block, 
     if (grid == 0) return hipSuccess;
     const dim3 blk(a.m.blk * kPassBThreadsPerCell);
     if (a.accumulate) {
-      const size_t lds = lds_bytes_cells(a.m, order, sizeof(T));
+      const size_t lds = lds_bytes_atomic(a.m, order, sizeof(T));
       if (order == 1) hipLaunchKernelGGL((k_pass_b_atomic<T, 1, false>), dim3(grid), blk, lds, s, a);
       else if (ns) hipLaunchKernelGGL((k_pass_b_atomic<T, 2, true>), dim3(grid), blk, lds, s, a);
       else hipLaunchKernelGGL((k_pass_b_atomic<T, 2, false>), dim3(grid), blk, lds, s, a);
-      const hipError_t e = hipGetLastError();
-      if (e != hipSuccess) return e;
-      hipLaunchKernelGGL((k_update_atomic<T>), dim3(grid), dim3(a.m.blk), 0, s, a);
       return hipGetLastError();
     }
     const size_t lds = lds_bytes_pass_b(a.m, order, a.last != 0, sizeof(T));

@@ -97,7 +97,7 @@ struct StageArgsT {
   const int32_t* blk_inst;     // per mesh block: instance (batch only, else null)
   const int32_t* active;       // logical block -> mesh block (batch only, else null)
   int32_t nactive;             // active blocks (batch only)
-  int32_t accumulate;          // 1: pass B scatters with atomics, a cell pass updates
+  int32_t accumulate;          // 1: pass B scatters each face's terms with LDS atomics
   int32_t stage;               // 1, 2, 3
   int32_t last;                // the step's last stage (3, or 2 for the two-stage SSP-RK2):
                                // spectral sums for the next dt, GLOBAL maximum, residual
@@ -122,9 +122,6 @@ struct StageArgsT {
                                   // (GLOBAL dt), FP64 bits
   unsigned long long* smax_next;  // per instance: max spectral sum written by stage 3
   T* res;                      // residual (4 per cell)
-  T* acc;                      // accumulate = 1: face sums per cell (4 per cell), zero
-                               // between stages
-  T* sacc;                     // accumulate = 1: spectral sums per cell at stage 3
   double* time;                // per instance: physical time of the uniform dt
 };
 

@@ -77,8 +77,6 @@ struct Engine {
   void* g[3] = {nullptr, nullptr, nullptr};   // stage 1..3 limited gradients (8*ncp each)
   void* sigma = nullptr;                      // ncp
   void* res = nullptr;                        // 4*ncp
-  void* acc = nullptr;                        // accumulate = 1: 4*ncp face sums (zero between stages)
-  void* sacc = nullptr;                       // accumulate = 1: ncp spectral sums
   unsigned long long* smax = nullptr;         // [2][n], FP64 bits
   double* time = nullptr;                     // [n] physical time of the uniform dt
   double* partial = nullptr;                  // [10][nblocks]: residual and conservation sums
@@ -337,8 +335,6 @@ fvk::StageArgsT<T> base_args(Engine* c) {
   a.sigma = static_cast<T*>(c->sigma);
   a.res = static_cast<T*>(c->res);
   a.accumulate = c->opt.accumulate;
-  a.acc = static_cast<T*>(c->acc);
-  a.sacc = static_cast<T*>(c->sacc);
   const size_t n = c->in.size();
   a.smax_cur = c->smax + c->smax_cur * n;
   a.smax_next = c->smax + (1 - c->smax_cur) * n;
@@ -962,12 +958,6 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
   }
   if ((rc = fill(c, c->res, 4 * (int64_t)ncp, 0.0))) return rc;
   if ((rc = fill(c, c->sigma, (int64_t)ncp, 0.0))) return rc;
-  if (c->opt.accumulate) {   // the accumulators start at zero and the cell pass re-zeroes them
-    if ((rc = alloc(&c->acc, 4 * ncp))) return rc;
-    if ((rc = alloc(&c->sacc, ncp))) return rc;
-    if ((rc = fill(c, c->acc, 4 * (int64_t)ncp, 0.0))) return rc;
-    if ((rc = fill(c, c->sacc, (int64_t)ncp, 0.0))) return rc;
-  }
   FVK_HIP(c, hipStreamSynchronize(c->stream));
   return FVK_OK;
 }

@@ -103,9 +103,10 @@ typedef struct fvk_options {
                               1: one thread per cell; 2: one thread per face slot;
                               3: LDS-staged blocks */
   int32_t accumulate;      /* face-to-cell residual accumulation: 0 (default) owner
-                              gather in face order (bit-exact); 1 atomic scatter,
-                              each face once, then a cell-update pass (LDS-staged
-                              shape only; sums in run-time order, so not bit-exact) */
+                              gather in face order (bit-exact); 1 atomic scatter of
+                              each face's terms into the block's LDS accumulators
+                              (LDS-staged shape only; sums in run-time order, so
+                              not bit-exact) */
 } fvk_options;
 
 /* Fill *opt with the defaults above for the given BC kinds. */

@@ -1,9 +1,9 @@
 """Atomic-scatter residual accumulation (accumulate = 1) against the owner
 gather (accumulate = 0, the reference's bits).
 
-The atomic form computes each face once and adds its terms to both cells in
-run-time order, so its sums differ from the gather's by summation order
-only.  Where the scheme is well conditioned (no limiter, or first order) the
+The atomic form scatters each face's terms into LDS accumulators of the
+block's cells with LDS atomics, in run-time order, so its sums differ from the
+gather's by summation order only.  Where the scheme is well conditioned (no limiter, or first order) the
 two FP64 solutions must agree to 1e-10 relative after 100 steps, the bound
 stated for summation-order differences; with the SQUEEZE limiter, whose
 lagged ratio of differences amplifies rounding, only a loose bound holds.
