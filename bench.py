@@ -17,6 +17,10 @@ stream has drained).
   FP32-mixed batch: the FP64 batch, and 256 sequential FP64 runs of the plain
   single-instance path (each instance timed alone over a few steps, priced at
   the steps its FP64 run took); per-instance FP32-mixed errors against FP64.
+  Then, untimed for `value`, the same instances at first order run to the drop
+  in both precisions (at most --conv-steps steps, stall retirement, the
+  FP32-mixed run's stalled instances finished in FP64): the errors at
+  convergence, which the second-order SQUEEZE batch does not reach.
 --config ramp|channel|cylinder|airfoil: one mesh (the 512x256 ramp, the
   64x32 bump channel, the 1024x512 NS O-grid, the 7.8M-cell Delaunay-dual
   airfoil), FP32-mixed and FP64, K steps each (the ramp: to a 1e-8 density
@@ -773,6 +777,44 @@ def batch_errors(args, cases, r, r64, fp64_opts):
     return out
 
 
+def convergence_leg(args, cases, nc, opts32, opts64):
+    """The configured instances at first order, run to the residual drop in
+    both precisions with stall retirement, the FP32-mixed run's stalled
+    instances finished in FP64: errors at convergence, which the second-order
+    SQUEEZE batch does not reach (its lagged limiter stalls the residual in
+    both precisions).  Runs after the measurement; its figures are reported
+    beside it, not in `value`."""
+    cases1 = [(g, inputs.Case(**{**c.__dict__, "poly": 0})) for g, c in cases]
+    a = argparse.Namespace(**vars(args))
+    a.steps, a.check_every, a.warmup, a.kernel_reps = args.conv_steps, 200, 0, 1
+    a.stall_checks, a.stall_factor = 20, 0.5
+    a.finish_fp64, a.reference_drop = True, 0.0
+    log(f"convergence leg: first order, to a {a.drop:g} drop or {a.steps} steps")
+    r32 = run_batch("fp32", a, cases1, nc, opts32)
+    r64 = run_batch("fp64", a, cases1, nc, opts64)
+    e = batch_errors(a, cases1, r32, r64, opts64)
+    r32["batch"].close()
+    r64["batch"].close()
+    fin = e["with_fp64_finish"]
+    return {
+        "workload": "the same 256 instances at first order (poly 0), Euler, Rusanov, LOCAL dt",
+        "why": "the second-order SQUEEZE instances stall above the drop in both precisions",
+        "drop": a.drop, "max_steps": a.steps, "check_every": a.check_every,
+        "stall_retirement": {"checks": a.stall_checks, "factor": a.stall_factor},
+        "fp32_mixed": {"value": r32["value"], "wall_s": round(r32["wall"], 3),
+                       "retired": {"drop": int((r32["why"] == RETIRED_DROP).sum()),
+                                   "stall": int((r32["why"] == RETIRED_STALL).sum())}},
+        "fp64_batched": {"value": r64["value"], "wall_s": round(r64["wall"], 3),
+                         "retired": {"drop": int((r64["why"] == RETIRED_DROP).sum()),
+                                     "stall": int((r64["why"] == RETIRED_STALL).sum())}},
+        "speedup_vs_fp64_batched": round(r32["value"] / r64["value"], 4),
+        "error_at_convergence": e["error_at_convergence"],
+        "with_fp64_finish": {k: fin[k] for k in ("value", "speedup_vs_fp64_batched",
+                                                 "instances_finished", "retired",
+                                                 "error_at_convergence")},
+        "fp64_sensitivity_floor": e["fp64_sensitivity_floor"]}
+
+
 def bench_batch(args):
     name, plan = W.batch(args.instances)
     cases = [(i.grid, i.case) for i in plan]
@@ -854,6 +896,9 @@ def bench_batch(args):
             out["speedup_vs_fp64_batched"] = round(r["value"] / r64["value"], 4)
             fp64_opts = tuned["fp64"]["choice"] if tuned else opts
             out.update(batch_errors(args, cases, r, r64, fp64_opts))
+            if args.conv_steps and not (args.order or args.limiter):
+                out["convergence_first_order"] = convergence_leg(
+                    args, cases, nc, tuned["fp32"]["choice"] if tuned else opts, fp64_opts)
     if not args.no_cpu:
         # a bounded sample: the smallest, the median and the largest instance
         order = np.argsort(nc)
@@ -898,6 +943,10 @@ def main():
                          "batch run to this residual drop (untimed; 0: off)")
     ap.add_argument("--reference-steps", type=int, default=40000,
                     help="batch: step cap of the --reference-drop run")
+    ap.add_argument("--conv-steps", type=int, default=20000,
+                    help="batch: step cap of the convergence leg (the instances at first "
+                         "order, run to the drop in both precisions with stall retirement "
+                         "and the FP64 finish); 0: skip it")
     ap.add_argument("--chunk", type=int, default=None,
                     help="single mesh: steps between residual checks and force samples")
     ap.add_argument("--drop-single", type=float, default=None,
