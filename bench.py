@@ -448,7 +448,7 @@ def bench_single(args):
     opts = {"block_size": args.block_size, "xcd_remap": args.xcd_remap,
             "accumulate": args.accumulate}
     tuned = {}
-    if args.autotune and args.variant in (0, 3):
+    if args.autotune and args.variant in (0, 3, 4):
         from fv2d import autotune
         for prec in precs:
             best, table = autotune.tune_single(
@@ -470,7 +470,8 @@ def bench_single(args):
     algo = algorithmic_bytes(case, nc, nf, s=4 if head == "fp32" else 8)
     chosen = r["solver"].launch_options()
     child = ["--no-cpu", "--no-traffic", "--no-autotune", "--steps", "5", "--warmup", "2",
-             "--kernel-reps", "5", "--config", args.config, "--variant", str(args.variant),
+             "--kernel-reps", "5", "--config", args.config,
+             "--variant", str(chosen["kernel_variant"]),
              "--block-size", str(chosen["block_size"]), "--xcd-remap", str(chosen["xcd_remap"]),
              "--accumulate", str(chosen["accumulate"]), "--precision", head]
     tmp = None
@@ -832,7 +833,7 @@ def bench_batch(args):
     log(f"{name}: {len(cases)} instances, {int(nc.sum())} cells")
     precs = ["fp32", "fp64"] if args.precision == "both" else [args.precision]
     opts = {"block_size": args.block_size, "xcd_remap": args.xcd_remap,
-            "accumulate": args.accumulate}
+            "accumulate": args.accumulate, "kernel_variant": args.variant}
     tuned = {}
     if args.autotune:
         from fv2d import autotune
@@ -858,6 +859,7 @@ def bench_batch(args):
              "--warmup", "0", "--kernel-reps", "3", "--config", "batch",
              "--instances", str(args.instances), "--block-size", str(chosen["block_size"]),
              "--xcd-remap", str(chosen["xcd_remap"]), "--accumulate", str(chosen["accumulate"]),
+             "--variant", str(chosen["kernel_variant"]),
              "--precision", head, "--order", str(args.order)] + \
         (["--limiter", args.limiter] if args.limiter else [])
     roof = roofline(r["per_launch"], algo, head, args, child)
@@ -959,7 +961,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
     ap.add_argument("--variant", type=int, default=0,
-                    help="single mesh: 0 default, 1 cell-parallel, 2 slot-parallel, 3 LDS-staged")
+                    help="0 default, 1 cell-parallel, 2 slot-parallel (single mesh), 3 LDS-staged, "
+                         "4 LDS-staged on a persistent grid")
     ap.add_argument("--block-size", type=int, default=0,
                     help="cells per mesh block (0: the library's automatic choice)")
     ap.add_argument("--xcd-remap", type=int, default=-1,

@@ -34,8 +34,13 @@
 //                  are staged in LDS with coalesced loads; pass B computes
 //                  each face of the block once (one thread per face, two
 //                  threads per cell of the block) into LDS and each cell then
-//                  adds its faces' terms in slot order.
+//                  adds its faces' terms in slot order;
+//   persistent     the LDS-staged blocks walked by as many workgroups as are
+//                  resident at once, each over its XCD's share of the blocks.
+#include <algorithm>
+#include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "fv_kernels.hpp"
 
@@ -78,14 +83,17 @@ __device__ __forceinline__ T max2(T a, T b) { return (a < b) ? b : a; }  // std:
 // A negative argument -G asks for runs of G consecutive logical blocks per
 // XCD, the runs dealt to the XCDs in turn: neighbour blocks still share an L2
 // while every XCD walks the whole block range at the same pace.
-__device__ __forceinline__ int logical_block(int per_xcd) {
-  const int b = blockIdx.x;
+__device__ __forceinline__ int logical_block_of(int b, int per_xcd) {
   if (per_xcd > 0) return (b & 7) * per_xcd + (b >> 3);
   if (per_xcd < 0) {
     const int g = -per_xcd, j = b >> 3;
     return ((j / g) * 8 + (b & 7)) * g + j % g;
   }
   return b;
+}
+
+__device__ __forceinline__ int logical_block(int per_xcd) {
+  return logical_block_of(blockIdx.x, per_xcd);
 }
 
 // Mesh block and instance of logical block lb: through the active-block list
@@ -1500,12 +1508,13 @@ __device__ __forceinline__ void slot_pos(const DevMesh& m, int s, SlotPos<T>& px
   }
 }
 
+// Pass A of logical block lb (one thread per cell of the block)
 template <typename T, bool SQUEEZE, int PF>
-__global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageArgsT<T> a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
+__device__ __forceinline__ void pass_a_tile(const StageArgsT<T>& a, int lb,
+                                            unsigned char* lds_raw) {
   const DevMesh& m = a.m;
   int b, inst;
-  if (!block_of(a, logical_block(a.xcd_blocks), b, inst)) return;   // uniform over the block
+  if (!block_of(a, lb, b, inst)) return;   // uniform over the block
   const FlowT<T>& fl = a.flows[inst];
   const bool squeeze = SQUEEZE && fl.squeeze;
   LdsCells<T, 4, true> L(lds_raw, m.blk + m.max_halo);
@@ -1599,6 +1608,12 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
   write_grad(a, c, gx, gy);
 }
 
+template <typename T, bool SQUEEZE, int PF>
+__global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageArgsT<T> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  pass_a_tile<T, SQUEEZE, PF>(a, logical_block(a.xcd_blocks), lds_raw);
+}
+
 // One block face: owner and neighbour (local indices, or the boundary code)
 // and its geometry -- FP64 for the FP64 path (FaceGeo built once the
 // centroids are in LDS), precomputed FP32 for the FP32 path.
@@ -1660,19 +1675,19 @@ __device__ __forceinline__ void lds_cell(const LdsCells<T, NQ, XY>& L, int l, Ce
   }
 }
 
-template <typename T, int ORDER, bool NS, int PF, bool ROE = false>
-__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
-void k_pass_b_lds(StageArgsT<T> a) {
-  // cell rows, then [4 or 5][max_bfaces] face terms (the spectral-sum term
-  // only at stage 3, where the next step's dt needs it)
-  extern __shared__ __align__(16) unsigned char lds_raw[];
+// Pass B of logical block lb (two threads per cell of the block).  LDS: cell
+// rows, then [4 or 5][max_bfaces] face terms (the spectral-sum term only at
+// stage 3, where the next step's dt needs it)
+template <typename T, int ORDER, bool NS, int PF, bool ROE>
+__device__ __forceinline__ void pass_b_tile(const StageArgsT<T>& a, int lb,
+                                            unsigned char* lds_raw) {
   constexpr int NQ = ORDER == 2 ? 12 : 4;
   constexpr bool XY = !kDelta<T>;
   constexpr bool kLate = FVK_NS_LATE && NS && !kDelta<T>;   // FP64 NS: the leaner prefetch
   constexpr int kFPFB = kLate ? 1 : kFPF;
   const DevMesh& m = a.m;
   int b = 0, inst = 0;
-  const bool live = block_of(a, logical_block(a.xcd_blocks), b, inst);   // uniform
+  const bool live = block_of(a, lb, b, inst);   // uniform
   const FlowT<T>& fl = a.flows[inst];
   double sig_new = 0.0;
   if (live) {
@@ -1774,6 +1789,13 @@ void k_pass_b_lds(StageArgsT<T> a) {
   if (live && a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) step_start(a, fl, inst);
 }
 
+template <typename T, int ORDER, bool NS, int PF, bool ROE = false>
+__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
+void k_pass_b_lds(StageArgsT<T> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  pass_b_tile<T, ORDER, NS, PF, ROE>(a, logical_block(a.xcd_blocks), lds_raw);
+}
+
 // ---------------------------------------------------------------------------
 // Atomic-scatter accumulation (accumulate = 1), the alternative to the owner
 // gather above.  Each block computes the faces of its cells, as the gather
@@ -1790,14 +1812,13 @@ void k_pass_b_lds(StageArgsT<T> a) {
 // reference) by summation order only.
 // ---------------------------------------------------------------------------
 template <typename T, int ORDER, bool NS>
-__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
-void k_pass_b_atomic(StageArgsT<T> a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
+__device__ __forceinline__ void pass_b_atomic_tile(const StageArgsT<T>& a, int lb,
+                                                   unsigned char* lds_raw) {
   constexpr int NQ = ORDER == 2 ? 12 : 4;
   constexpr bool XY = !kDelta<T>;
   const DevMesh& m = a.m;
   int b = 0, inst = 0;
-  const bool live = block_of(a, logical_block(a.xcd_blocks), b, inst);   // uniform
+  const bool live = block_of(a, lb, b, inst);   // uniform
   const FlowT<T>& fl = a.flows[inst];
   double sig_new = 0.0;
   if (live) {
@@ -1860,6 +1881,65 @@ void k_pass_b_atomic(StageArgsT<T> a) {
   }
   if (live && a.last && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
   if (live && a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) step_start(a, fl, inst);
+}
+
+template <typename T, int ORDER, bool NS>
+__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
+void k_pass_b_atomic(StageArgsT<T> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  pass_b_atomic_tile<T, ORDER, NS>(a, logical_block(a.xcd_blocks), lds_raw);
+}
+
+// ---------------------------------------------------------------------------
+// The LDS-staged kernels on a persistent grid (kernel_variant 4): as many
+// workgroups as the device holds at once, a multiple of 8.  Workgroup w runs
+// on XCD w % 8 and walks the tiles (mesh blocks) that the one-workgroup-per-
+// block launch deals to that XCD, in the same order, striding by the XCD's
+// workgroup count -- so the blocks that share an L2 are the same.  The
+// assignment is static: no counters, no atomics, no workgroup waits for
+// another (a kernel boundary still separates the passes and the stages), and
+// each tile's work is that of the one-workgroup-per-block launch, bit for bit.
+// ---------------------------------------------------------------------------
+template <typename T, typename Body>
+__device__ __forceinline__ void tile_loop(const StageArgsT<T>& a, Body body) {
+  const int q = blockIdx.x & 7, per_q = gridDim.x >> 3;
+  for (int k = blockIdx.x >> 3; k < a.tiles_per_xcd; k += per_q) {
+    const int v = k * 8 + q;     // this tile's workgroup in the one-per-block launch
+    if (v < a.tiles_total) body(logical_block_of(v, a.xcd_blocks));
+    __syncthreads();             // the tile's LDS reads are done before the next staging
+  }
+}
+
+// The kernel's argument block, read again from the kernarg segment for every
+// tile.  Left loop-invariant, the compiler hoists the argument loads and
+// everything derived from them out of the tile loop, and the spilled scalars
+// cost registers and occupancy.
+template <typename T>
+__device__ __forceinline__ const StageArgsT<T>& fresh_args() {
+  using CP = const __attribute__((address_space(4))) StageArgsT<T>*;
+  CP p = (CP)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return *(const StageArgsT<T>*)p;
+}
+
+template <typename T, bool SQUEEZE, int PF>
+__global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_persist(StageArgsT<T> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  tile_loop(a, [&](int lb) { pass_a_tile<T, SQUEEZE, PF>(fresh_args<T>(), lb, lds_raw); });
+}
+
+template <typename T, int ORDER, bool NS, int PF, bool ROE = false>
+__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
+void k_pass_b_persist(StageArgsT<T> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  tile_loop(a, [&](int lb) { pass_b_tile<T, ORDER, NS, PF, ROE>(fresh_args<T>(), lb, lds_raw); });
+}
+
+template <typename T, int ORDER, bool NS>
+__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
+void k_pass_b_atomic_persist(StageArgsT<T> a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  tile_loop(a, [&](int lb) { pass_b_atomic_tile<T, ORDER, NS>(fresh_args<T>(), lb, lds_raw); });
 }
 
 // Spectral sums of the first step: every face carries |V_inf| and c_inf
@@ -2051,21 +2131,101 @@ static size_t lds_bytes_atomic(const DevMesh& m, int order, size_t tsize) {
 }
 
 template <typename T>
+using StageKernel = void (*)(StageArgsT<T>);
+
+// Workgroups of a persistent launch: as many as are resident at once (the
+// kernel's occupancy at this LDS size times the CUs), a multiple of 8 so that
+// every XCD has as many, and no more than the tiles.  The occupancy
+// is looked up once per kernel, LDS size and device.
+template <typename T>
+hipError_t persist_grid(StageKernel<T> k, int threads, size_t lds, int tiles, int* grid) {
+  struct Key {
+    const void* k;
+    int threads, dev;
+    size_t lds;
+  };
+  static std::mutex mu;
+  static std::vector<std::pair<Key, int>> cache;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  int resident = 0;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    for (const auto& kv : cache)
+      if (kv.first.k == (const void*)k && kv.first.threads == threads && kv.first.dev == dev &&
+          kv.first.lds == lds)
+        resident = kv.second;
+  }
+  if (resident == 0) {
+    int cus = 0, per_cu = 0;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+      return e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, threads, lds)) !=
+        hipSuccess)
+      return e;
+    resident = std::max(8, per_cu * cus / 8 * 8);
+    std::lock_guard<std::mutex> lock(mu);
+    cache.push_back({Key{(const void*)k, threads, dev, lds}, resident});
+  }
+  *grid = std::min(resident, (tiles + 7) / 8 * 8);
+  return hipSuccess;
+}
+
+// an LDS-staged pass over `tiles` logical blocks: one workgroup per block
+// (k1), or the persistent grid (kp) walking them
+template <typename T>
+hipError_t launch_tiles(StageArgsT<T> a, StageKernel<T> k1, StageKernel<T> kp, int tiles,
+                        int threads, size_t lds, hipStream_t s) {
+  StageKernel<T> k = k1;
+  int grid = tiles;
+  if (a.variant == kVariantPersist) {
+    const hipError_t e = persist_grid(kp, threads, lds, tiles, &grid);
+    if (e != hipSuccess) return e;
+    a.tiles_total = tiles;
+    a.tiles_per_xcd = (tiles + 7) / 8;
+    k = kp;
+  }
+  void* args[] = {&a};
+  return hipLaunchKernel((const void*)k, dim3(grid), dim3(threads), args, lds, s);
+}
+
+template <typename T, bool SQUEEZE, int PF>
+void pick_a(StageKernel<T>* k1, StageKernel<T>* kp) {
+  *k1 = k_pass_a_lds<T, SQUEEZE, PF>;
+  *kp = k_pass_a_persist<T, SQUEEZE, PF>;
+}
+
+template <typename T, int PF, bool ROE>
+void pick_b(int order, int ns, StageKernel<T>* k1, StageKernel<T>* kp) {
+  if (order == 1) {
+    *k1 = k_pass_b_lds<T, 1, false, PF, ROE>;
+    *kp = k_pass_b_persist<T, 1, false, PF, ROE>;
+  } else if (ns) {
+    *k1 = k_pass_b_lds<T, 2, true, PF, ROE>;
+    *kp = k_pass_b_persist<T, 2, true, PF, ROE>;
+  } else {
+    *k1 = k_pass_b_lds<T, 2, false, PF, ROE>;
+    *kp = k_pass_b_persist<T, 2, false, PF, ROE>;
+  }
+}
+
+template <typename T>
 hipError_t launch_pass_a(const StageArgsT<T>& a0, int squeeze, int block, hipStream_t s) {
   StageArgsT<T> a = a0;
   int grid;
-  if (a.variant == kVariantLds) {
+  if (a.variant == kVariantLds || a.variant == kVariantPersist) {
     shape(a, a.active ? a.nactive : a.m.nblocks, &grid, &a.xcd_blocks);
     if (grid == 0) return hipSuccess;
-    const size_t lds = lds_bytes_pass_a(a.m, sizeof(T));
-    const dim3 g(grid), blk(a.m.blk);
+    StageKernel<T> k1, kp;
     if (a.m.max_deg > kPF) {
-      if (squeeze) hipLaunchKernelGGL((k_pass_a_lds<T, true, kPFDeep>), g, blk, lds, s, a);
-      else hipLaunchKernelGGL((k_pass_a_lds<T, false, kPFDeep>), g, blk, lds, s, a);
+      if (squeeze) pick_a<T, true, kPFDeep>(&k1, &kp);
+      else pick_a<T, false, kPFDeep>(&k1, &kp);
     } else {
-      if (squeeze) hipLaunchKernelGGL((k_pass_a_lds<T, true, kPF>), g, blk, lds, s, a);
-      else hipLaunchKernelGGL((k_pass_a_lds<T, false, kPF>), g, blk, lds, s, a);
+      if (squeeze) pick_a<T, true, kPF>(&k1, &kp);
+      else pick_a<T, false, kPF>(&k1, &kp);
     }
+    return launch_tiles(a, k1, kp, grid, a.m.blk, lds_bytes_pass_a(a.m, sizeof(T)), s);
   } else if (a.variant == kVariantSlot) {
     shape(a, a.m.ncp / 64, &grid, &a.xcd_blocks);
     if (squeeze) hipLaunchKernelGGL((k_pass_a_slot<T, true>), dim3(grid), dim3(64 * kRows), 0, s, a);
@@ -2082,37 +2242,33 @@ template <typename T>
 hipError_t launch_pass_b(const StageArgsT<T>& a0, int order, int ns, int block, hipStream_t s) {
   StageArgsT<T> a = a0;
   int grid;
-  if (a.variant == kVariantLds) {
+  if (a.variant == kVariantLds || a.variant == kVariantPersist) {
     shape(a, a.active ? a.nactive : a.m.nblocks, &grid, &a.xcd_blocks);
     if (grid == 0) return hipSuccess;
-    const dim3 blk(a.m.blk * kPassBThreadsPerCell);
+    const int threads = a.m.blk * kPassBThreadsPerCell;
+    StageKernel<T> k1, kp;
     if (a.accumulate) {
-      const size_t lds = lds_bytes_atomic(a.m, order, sizeof(T));
-      if (order == 1) hipLaunchKernelGGL((k_pass_b_atomic<T, 1, false>), dim3(grid), blk, lds, s, a);
-      else if (ns) hipLaunchKernelGGL((k_pass_b_atomic<T, 2, true>), dim3(grid), blk, lds, s, a);
-      else hipLaunchKernelGGL((k_pass_b_atomic<T, 2, false>), dim3(grid), blk, lds, s, a);
-      return hipGetLastError();
-    }
-    const size_t lds = lds_bytes_pass_b(a.m, order, a.last != 0, sizeof(T));
-    if (a.roe) {   // the Roe-flux pass B (an extension)
-      if (a.m.max_deg > kPF) {
-        if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false, kPFDeep, true>), dim3(grid), blk, lds, s, a);
-        else if (ns) hipLaunchKernelGGL((k_pass_b_lds<T, 2, true, kPFDeep, true>), dim3(grid), blk, lds, s, a);
-        else hipLaunchKernelGGL((k_pass_b_lds<T, 2, false, kPFDeep, true>), dim3(grid), blk, lds, s, a);
+      if (order == 1) {
+        k1 = k_pass_b_atomic<T, 1, false>;
+        kp = k_pass_b_atomic_persist<T, 1, false>;
+      } else if (ns) {
+        k1 = k_pass_b_atomic<T, 2, true>;
+        kp = k_pass_b_atomic_persist<T, 2, true>;
       } else {
-        if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false, kPF, true>), dim3(grid), blk, lds, s, a);
-        else if (ns) hipLaunchKernelGGL((k_pass_b_lds<T, 2, true, kPF, true>), dim3(grid), blk, lds, s, a);
-        else hipLaunchKernelGGL((k_pass_b_lds<T, 2, false, kPF, true>), dim3(grid), blk, lds, s, a);
+        k1 = k_pass_b_atomic<T, 2, false>;
+        kp = k_pass_b_atomic_persist<T, 2, false>;
       }
-    } else if (a.m.max_deg > kPF) {
-      if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false, kPFDeep>), dim3(grid), blk, lds, s, a);
-      else if (ns) hipLaunchKernelGGL((k_pass_b_lds<T, 2, true, kPFDeep>), dim3(grid), blk, lds, s, a);
-      else hipLaunchKernelGGL((k_pass_b_lds<T, 2, false, kPFDeep>), dim3(grid), blk, lds, s, a);
-    } else {
-      if (order == 1) hipLaunchKernelGGL((k_pass_b_lds<T, 1, false, kPF>), dim3(grid), blk, lds, s, a);
-      else if (ns) hipLaunchKernelGGL((k_pass_b_lds<T, 2, true, kPF>), dim3(grid), blk, lds, s, a);
-      else hipLaunchKernelGGL((k_pass_b_lds<T, 2, false, kPF>), dim3(grid), blk, lds, s, a);
+      return launch_tiles(a, k1, kp, grid, threads, lds_bytes_atomic(a.m, order, sizeof(T)), s);
     }
+    const bool deep = a.m.max_deg > kPF;
+    if (a.roe) {   // the Roe-flux pass B (an extension)
+      if (deep) pick_b<T, kPFDeep, true>(order, ns, &k1, &kp);
+      else pick_b<T, kPF, true>(order, ns, &k1, &kp);
+    } else {
+      if (deep) pick_b<T, kPFDeep, false>(order, ns, &k1, &kp);
+      else pick_b<T, kPF, false>(order, ns, &k1, &kp);
+    }
+    return launch_tiles(a, k1, kp, grid, threads, lds_bytes_pass_b(a.m, order, a.last != 0, sizeof(T)), s);
   } else if (a.variant == kVariantSlot) {
     shape(a, a.m.ncp / 64, &grid, &a.xcd_blocks);
     const dim3 blk(64 * kRows);

@@ -109,7 +109,7 @@ struct StageArgsT {
   int32_t store_res;           // write the residual (stage 3 of a reported step)
   int32_t xcd_remap;           // XCD block order: 0 off, 1 contiguous ranges, G>1 runs of G
   int32_t xcd_blocks;          // blocks per XCD range, or -G (set by the launcher)
-  int32_t variant;             // kVariantCell, kVariantSlot or kVariantLds
+  int32_t variant;             // kVariantCell, kVariantSlot, kVariantLds or kVariantPersist
   int32_t roe;                 // flux_solver ROE: the Roe-flux pass B (LDS-staged gather only)
   const T* qc;                 // stage input state (4 per cell)
   const T* qp;                 // previous stage's input state (lagged limiter)
@@ -123,11 +123,14 @@ struct StageArgsT {
   unsigned long long* smax_next;  // per instance: max spectral sum written by stage 3
   T* res;                      // residual (4 per cell)
   double* time;                // per instance: physical time of the uniform dt
+  int32_t tiles_total;         // kVariantPersist (set by the launcher): logical blocks
+  int32_t tiles_per_xcd;       //   of the one-workgroup-per-block launch, and per XCD
 };
 
-// launch shapes: one thread per cell; one thread per (cell, face slot); or
-// LDS-staged blocks (block + halo in LDS, one thread per face for the flux)
-enum { kVariantCell = 1, kVariantSlot = 2, kVariantLds = 3 };
+// launch shapes: one thread per cell; one thread per (cell, face slot);
+// LDS-staged blocks (block + halo in LDS, one thread per face for the flux);
+// or the LDS-staged blocks walked by a persistent grid, per XCD
+enum { kVariantCell = 1, kVariantSlot = 2, kVariantLds = 3, kVariantPersist = 4 };
 
 // LDS bytes the LDS-staged kernels need for this mesh and state type size
 // (pass B stages centroids only for FP64: the FP32 kernels read per-face

@@ -221,24 +221,29 @@ inline int32_t shift_boundary(int32_t code, int32_t off) { return ~((~code) + (o
 // SSP-RK3, main.cpp:123-126), or 2 for the two-stage SSP-RK2 extension
 inline int stages_of(const fvk_options& o) { return o.time_scheme == 3 ? 2 : 3; }
 
+// the LDS-staged shapes: one workgroup per block, or a persistent grid
+inline bool lds_staged(int variant) {
+  return variant == fvk::kVariantLds || variant == fvk::kVariantPersist;
+}
+
 int check_options(const fvk_options& o, std::string* msg) {
   if (o.order != 1 && o.order != 2) { *msg = "order must be 1 or 2 (solution_poly_order 0 or 1)"; return FVK_ERR_OPTION; }
   if (o.limiter < 0 || o.limiter > 3) { *msg = "limiter must be NONE, MINMOD, SQUEEZE or 3 (Barth-Jespersen)"; return FVK_ERR_OPTION; }
   if (o.eqn != 1 && o.eqn != 2) { *msg = "eqn must be EULER or NS"; return FVK_ERR_OPTION; }
   if (o.flux_solver != 1 && o.flux_solver != 2) { *msg = "flux_solver must be RUSANOV or ROE"; return FVK_ERR_OPTION; }
-  if (o.flux_solver == 2 && ((o.kernel_variant != 0 && o.kernel_variant != fvk::kVariantLds) || o.accumulate)) {
-    *msg = "flux_solver ROE runs in the LDS-staged gather pass B only (kernel_variant 0 or 3, accumulate 0)";
+  if (o.flux_solver == 2 && ((o.kernel_variant != 0 && !lds_staged(o.kernel_variant)) || o.accumulate)) {
+    *msg = "flux_solver ROE runs in the LDS-staged gather pass B only (kernel_variant 0, 3 or 4, accumulate 0)";
     return FVK_ERR_OPTION;
   }
   if (o.time_scheme < 1 || o.time_scheme > 3) { *msg = "time_scheme must be SSP_RK2, SSP_RK3 or 3 (two-stage SSP-RK2)"; return FVK_ERR_OPTION; }
   if (o.time_step_type < 1 || o.time_step_type > 3) { *msg = "time_step_type must be GLOBAL, LOCAL or 3 (uniform)"; return FVK_ERR_OPTION; }
   if (o.bc1 < -4 || o.bc1 > -1 || o.bc2 < -4 || o.bc2 > -1) { *msg = "BC1/BC2 must be FREESTREAM, SLIP_WALL, EXTRAPOLATION or NO_SLIP_WALL"; return FVK_ERR_OPTION; }
   if (o.precision != FVK_FP64 && o.precision != FVK_FP32_MIXED) { *msg = "precision must be FVK_FP64 or FVK_FP32_MIXED"; return FVK_ERR_OPTION; }
-  if (o.kernel_variant < 0 || o.kernel_variant > 3) { *msg = "kernel_variant must be 0, 1, 2 or 3"; return FVK_ERR_OPTION; }
+  if (o.kernel_variant < 0 || o.kernel_variant > 4) { *msg = "kernel_variant must be 0, 1, 2, 3 or 4"; return FVK_ERR_OPTION; }
   if (o.renumber < 0 || o.renumber > 2) { *msg = "renumber must be 0, 1 or 2"; return FVK_ERR_OPTION; }
   if (o.accumulate != 0 && o.accumulate != 1) { *msg = "accumulate must be 0 (gather) or 1 (atomic scatter)"; return FVK_ERR_OPTION; }
-  if (o.accumulate == 1 && o.kernel_variant != 0 && o.kernel_variant != fvk::kVariantLds) {
-    *msg = "accumulate = 1 needs the LDS-staged kernels (kernel_variant 0 or 3)";
+  if (o.accumulate == 1 && o.kernel_variant != 0 && !lds_staged(o.kernel_variant)) {
+    *msg = "accumulate = 1 needs the LDS-staged kernels (kernel_variant 0, 3 or 4)";
     return FVK_ERR_OPTION;
   }
   if (o.xcd_remap < -1 || o.xcd_remap > 4096) { *msg = "xcd_remap must be -1 (auto), 0, 1 or a run length 2..4096"; return FVK_ERR_OPTION; }
@@ -265,7 +270,7 @@ int check_common(const fvk_options& a, const fvk_options& b, std::string* msg) {
     return FVK_ERR_OPTION;
   }
   if (a.kernel_variant == fvk::kVariantCell || a.kernel_variant == fvk::kVariantSlot) {
-    *msg = "batches run the LDS-staged kernels only (kernel_variant 0 or 3)";
+    *msg = "batches run the LDS-staged kernels only (kernel_variant 0, 3 or 4)";
     return FVK_ERR_OPTION;
   }
   return FVK_OK;
@@ -900,7 +905,7 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
       if ((rc = upload_cat(c, &HM::bf32_dist, &d.bf32_dist))) return rc;
     }
   }
-  if (c->variant == fvk::kVariantLds) {
+  if (lds_staged(c->variant)) {
     // the LDS-staged kernels need a block plus its halo (the largest) in LDS
     int dev = 0;
     hipDeviceProp_t prop;
@@ -912,7 +917,7 @@ int engine_init(Engine* c, int32_t n, const fvk_mesh* meshes, const fvk_options*
     // above what fits in LDS)
     const bool codes_fit = d.blk + d.max_halo < (1 << 14) && d.max_bfaces < (1 << 14);
     if (need > prop.sharedMemPerBlock || !codes_fit) {
-      if (o0.kernel_variant == fvk::kVariantLds || batched || o0.accumulate || o0.flux_solver == 2)
+      if (lds_staged(o0.kernel_variant) || batched || o0.accumulate || o0.flux_solver == 2)
         return set_err(c, FVK_ERR_MESH, "mesh blocks need " + std::to_string(need) +
                                             " bytes of LDS, more than the device allows");
       c->variant = fvk::kVariantCell;

@@ -76,7 +76,7 @@ typedef struct fvk_options {
                               (read_inputs.cpp:89-91) but its Roe flux is an unfinished
                               draft (res.cpp:66-68 leaves the flux at -101); here Roe's
                               flux with Harten's entropy fix, in the LDS-staged gather
-                              pass B only (kernel_variant 0 or 3, accumulate 0) */
+                              pass B only (kernel_variant 0, 3 or 4, accumulate 0) */
   int32_t time_scheme;     /* 1 SSP_RK2 (runs SSP-RK3, main.cpp:123-126), 2 SSP_RK3;
                               3 two-stage SSP-RK2, an extension (the reference's
                               ssp_rk2.cpp is a draft it never builds): Q* = Q + dt R(Q),
@@ -101,11 +101,14 @@ typedef struct fvk_options {
                               blocks, else 8 */
   int32_t kernel_variant;  /* 0: default (3 when the blocks fit in LDS, else 1);
                               1: one thread per cell; 2: one thread per face slot;
-                              3: LDS-staged blocks */
+                              3: LDS-staged blocks, one workgroup per block;
+                              4: the same blocks on a persistent grid (as many
+                              workgroups as are resident at once, each walking a
+                              stride of its XCD's blocks; results identical to 3) */
   int32_t accumulate;      /* face-to-cell residual accumulation: 0 (default) owner
                               gather in face order (bit-exact); 1 atomic scatter of
                               each face's terms into the block's LDS accumulators
-                              (LDS-staged shape only; sums in run-time order, so
+                              (LDS-staged shapes only; sums in run-time order, so
                               not bit-exact) */
 } fvk_options;
 
@@ -226,7 +229,7 @@ int fvk_time_kernels(fvk_ctx* ctx, int32_t reps, double avg_us[2]);
  * whole batch.  No counterpart in the reference, which solves one case per
  * process; each instance's results are those of a single fvk_ctx run of the
  * same mesh and options, bit for bit.  Instances must agree on order, eqn,
- * time_step_type, precision, block_size, kernel_variant (0 or 3) and
+ * time_step_type, precision, block_size, kernel_variant (0, 3 or 4) and
  * xcd_remap.  Steps are replayed from a captured HIP graph as fvk_step
  * describes, counting the cells of all instances; the graph is captured
  * again when the number of active blocks changes (fvk_batch_use_graph).

@@ -7,7 +7,8 @@
   100 steps each) and a 196k-cell polygonal airfoil dual against the oracle
   on the same inputs.
 * Size-independent properties: results do not depend on cell renumbering,
-  block size or the XCD block order; a uniform freestream with only
+  block size, the XCD block order or the launch shape (one workgroup per
+  block, or the persistent grid of kernel_variant 4); a uniform freestream with only
   freestream/extrapolation boundaries stays uniform; conservation.
 """
 import numpy as np
@@ -31,11 +32,12 @@ def _start(solver, orc, g: Golden):
         solver.init_freestream()
 
 
+@pytest.mark.parametrize("variant", [0, 4])
 @pytest.mark.parametrize("name", golden_names())
-def test_matches_reference_fixture(name):
+def test_matches_reference_fixture(name, variant):
     g = Golden(name)
-    print(f"[gpu] fixture {name} cells={g.grid.num_cells}", flush=True)
-    s = Solver(g.grid, g.case)
+    print(f"[gpu] fixture {name} cells={g.grid.num_cells} variant={variant}", flush=True)
+    s = Solver(g.grid, g.case, kernel_variant=variant)
     if g.meta["res_only"]:
         # the residual operator is pinned through one step from the fixture
         # state, against the oracle (itself pinned to the reference's res())
@@ -101,7 +103,7 @@ def _oracle_run(kind, steps):
 
 # the oracle's 100 steps of the 524k-cell NS O-grid take about 90 s on one core
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("variant", [3, 1])
+@pytest.mark.parametrize("variant", [3, 4, 1])
 @pytest.mark.parametrize("kind,steps", [("bump", 100), ("ramp", 100), ("ogrid", 100),
                                         ("airfoil", 100)])
 def test_full_size_against_oracle(kind, steps, variant):
@@ -121,7 +123,8 @@ def test_full_size_against_oracle(kind, steps, variant):
                                   dict(kernel_variant=3, block_size=256),
                                   dict(kernel_variant=3, block_size=64, renumber=2),
                                   dict(xcd_remap=0), dict(xcd_remap=3),
-                                  dict(kernel_variant=2, xcd_remap=16)])
+                                  dict(kernel_variant=2, xcd_remap=16),
+                                  dict(kernel_variant=4), dict(kernel_variant=4, block_size=64)])
 def test_layout_invariance(over):
     g = Golden("ogrid_ns_squeeze_global")
     a = Solver(g.grid, g.case)
