@@ -1508,110 +1508,173 @@ __device__ __forceinline__ void slot_pos(const DevMesh& m, int s, SlotPos<T>& px
   }
 }
 
-// Pass A of logical block lb (one thread per cell of the block)
-template <typename T, bool SQUEEZE, int PF>
-__device__ __forceinline__ void pass_a_tile(const StageArgsT<T>& a, int lb,
-                                            unsigned char* lds_raw) {
-  const DevMesh& m = a.m;
-  int b, inst;
-  if (!block_of(a, lb, b, inst)) return;   // uniform over the block
-  const FlowT<T>& fl = a.flows[inst];
-  const bool squeeze = SQUEEZE && fl.squeeze;
-  LdsCells<T, 4, true> L(lds_raw, m.blk + m.max_halo);
-  const int i = threadIdx.x;
-  const int c = b * m.blk + i;
-  const int deg = m.deg[c];
-  const int base = m.chunk_off[c >> 6] + (c & 63);
-  // prefetch: slot codes and face positions, lagged state, LSQ inverse (the
-  // limiter's data after the gradient loop with FVK_A_LATE >= 1, the LSQ
-  // inverse too with 2, so that fewer values are live through the loop)
-  int lc[PF];
-  SlotPos<T> pfx[PF], pfy[PF];
-  auto load_pf = [&]() {
-#pragma unroll
-    for (int j = 0; j < PF; ++j)
-      if (j < deg) slot_pos<T>(m, base + j * 64, pfx[j], pfy[j]);
-  };
-#pragma unroll
-  for (int j = 0; j < PF; ++j)
-    if (j < deg) lc[j] = m.lcode[base + j * 64];
-  T qp[4], gpx[4], gpy[4];
-  if (FVK_A_LATE < 1) {
-    load_pf();
-    if (squeeze && !fl.bj && deg > 0) load_lagged(a, c, qp, gpx, gpy);
+// ---------------------------------------------------------------------------
+// Launch over the mesh blocks (tiles).  PERSIST = false: one workgroup per
+// block, the kernel body runs once.  PERSIST = true (kernel_variant 4): a
+// persistent grid of as many workgroups as the device holds at once, a
+// multiple of 8; workgroup w runs on XCD w % 8 and walks the tiles that the
+// one-workgroup-per-block launch deals to that XCD, in the same order,
+// striding by the XCD's workgroup count, so the blocks that share an L2 are
+// the same.  The assignment is static: no counters, no atomics, no workgroup
+// waits for another (a kernel boundary still separates the passes and the
+// stages), and each tile's work is that of the per-block launch, bit for bit.
+// The body is written once, in the kernel, for both: moved into a device
+// function it is optimised apart from the kernel and comes back with more
+// registers.
+// ---------------------------------------------------------------------------
+template <bool PERSIST>
+struct TileWalk;
+
+template <>
+struct TileWalk<false> {
+  template <typename T>
+  __device__ explicit TileWalk(const StageArgsT<T>&) {}
+  template <typename T>
+  __device__ int block(const StageArgsT<T>& a) const { return logical_block(a.xcd_blocks); }
+  template <typename T>
+  __device__ bool next(const StageArgsT<T>&) { return false; }
+};
+
+template <>
+struct TileWalk<true> {
+  int k;
+  template <typename T>
+  __device__ explicit TileWalk(const StageArgsT<T>&) : k(blockIdx.x >> 3) {}
+  // the tile's logical block, or one that block_of() turns down
+  template <typename T>
+  __device__ int block(const StageArgsT<T>& a) const {
+    const int v = k * 8 + (blockIdx.x & 7);   // its workgroup in the per-block launch
+    return k < a.tiles_per_xcd && v < a.tiles_total ? logical_block_of(v, a.xcd_blocks)
+                                                    : 0x7fffffff;
   }
-  double ixx = 0, iyy = 0, ixy = 0;
-  if (FVK_A_LATE < 2 && deg > 0) load_lsq_inv(m, c, ixx, iyy, ixy);
-  stage_block(a, b, L);
-  __syncthreads();
-  if (deg == 0) return;
-  T q[4];
-  lds_ld4(L.rec(i), q);
-  double xc, yc;
-  lds_xy(L, i, xc, yc);
-  T sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, qmin[4], qmax[4];
-  init_minmax(qmin, qmax);
-  auto slot = [&](int j, int code) {
-    T px[4], py[4], tmin[4], tmax[4];
-    if (code >= 0) {
-      const int nb = code >> 1;
-      T qn[4];
-      lds_ld4(L.rec(nb), qn);
-      double xn, yn;
-      lds_xy(L, nb, xn, yn);
-      lsq_interior(code & 1, q, xc, yc, qn, xn, yn, px, py, tmin, tmax);
-    } else {
-      // the 16-bit local code holds the kind only; the global code also
-      // names the no-slip ghost's source cell
-      const int s = base + j * 64;
-      lsq_boundary(a, fl, m.code[s], q, xc, yc, m.s_nx[s], m.s_ny[s], m.s_fcx[s], m.s_fcy[s],
-                   px, py, tmin, tmax);
-    }
-    accumulate_lsq(sx, sy, qmin, qmax, px, py, tmin, tmax);
-  };
-#pragma unroll
-  for (int j = 0; j < PF; ++j)
-    if (j < deg) slot(j, lc[j]);
-  for (int j = PF; j < deg; ++j) slot(j, m.lcode[base + j * 64]);
-  if (FVK_A_LATE >= 1) {
-    load_pf();
-    if (squeeze && !fl.bj) load_lagged(a, c, qp, gpx, gpy);
+  template <typename T>
+  __device__ bool next(const StageArgsT<T>& a) {
+    __syncthreads();   // the tile's LDS reads are done before the next staging
+    k += gridDim.x >> 3;
+    return k < a.tiles_per_xcd;
   }
-  if (FVK_A_LATE >= 2) load_lsq_inv(m, c, ixx, iyy, ixy);
-  T gx[4], gy[4];
-  lsq_solve(ixx, iyy, ixy, sx, sy, gx, gy);
-  if (squeeze) {
-    T phi[4] = {1, 1, 1, 1};
-    if (fl.bj) current_as_lagged(q, gx, gy, qp, gpx, gpy);
-    const bool sent = a.sentinel && !fl.bj;
-    auto lim = [&](SlotPos<T> px, SlotPos<T> py) {
-      T ph[4];
-      if constexpr (kDelta<T>) face_phi(sent, fl, px, py, q, qmin, qmax, qp, gpx, gpy, ph);
-      else face_phi(sent, fl, T(px - xc), T(py - yc), q, qmin, qmax, qp, gpx, gpy, ph);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) phi[k] = min2(ph[k], phi[k]);
-    };
-#pragma unroll
-    for (int j = 0; j < PF; ++j)
-      if (j < deg) lim(pfx[j], pfy[j]);
-    for (int j = PF; j < deg; ++j) {
-      SlotPos<T> px, py;
-      slot_pos<T>(m, base + j * 64, px, py);
-      lim(px, py);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      gx[k] = gx[k] * phi[k];
-      gy[k] = gy[k] * phi[k];
-    }
+};
+
+// The argument block a tile reads.  Per block: the kernel's.  Persistent:
+// read again from the kernarg segment for every tile -- left loop-invariant,
+// the compiler hoists the argument loads and everything derived from them out
+// of the tile loop, and the spilled scalars cost registers and occupancy.
+template <bool PERSIST, typename T>
+__device__ __forceinline__ const StageArgsT<T>& tile_args(const StageArgsT<T>& args) {
+  if constexpr (!PERSIST) {
+    return args;
+  } else {
+    using CP = const __attribute__((address_space(4))) StageArgsT<T>*;
+    CP p = (CP)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *(const StageArgsT<T>*)p;
   }
-  write_grad(a, c, gx, gy);
 }
 
-template <typename T, bool SQUEEZE, int PF>
-__global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageArgsT<T> a) {
+// Pass A, one thread per cell of the block
+template <typename T, bool SQUEEZE, int PF, bool PERSIST = false>
+__global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageArgsT<T> args) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_a_tile<T, SQUEEZE, PF>(a, logical_block(a.xcd_blocks), lds_raw);
+  TileWalk<PERSIST> walk(args);
+  do {
+    const StageArgsT<T>& a = tile_args<PERSIST>(args);
+    do {   // one tile; `break` ends it
+      const DevMesh& m = a.m;
+      int b, inst;
+      if (!block_of(a, walk.block(a), b, inst)) break;   // uniform over the block
+      const FlowT<T>& fl = a.flows[inst];
+      const bool squeeze = SQUEEZE && fl.squeeze;
+      LdsCells<T, 4, true> L(lds_raw, m.blk + m.max_halo);
+      const int i = threadIdx.x;
+      const int c = b * m.blk + i;
+      const int deg = m.deg[c];
+      const int base = m.chunk_off[c >> 6] + (c & 63);
+      // prefetch: slot codes and face positions, lagged state, LSQ inverse (the
+      // limiter's data after the gradient loop with FVK_A_LATE >= 1, the LSQ
+      // inverse too with 2, so that fewer values are live through the loop)
+      int lc[PF];
+      SlotPos<T> pfx[PF], pfy[PF];
+      auto load_pf = [&]() {
+#pragma unroll
+        for (int j = 0; j < PF; ++j)
+          if (j < deg) slot_pos<T>(m, base + j * 64, pfx[j], pfy[j]);
+      };
+#pragma unroll
+      for (int j = 0; j < PF; ++j)
+        if (j < deg) lc[j] = m.lcode[base + j * 64];
+      T qp[4], gpx[4], gpy[4];
+      if (FVK_A_LATE < 1) {
+        load_pf();
+        if (squeeze && !fl.bj && deg > 0) load_lagged(a, c, qp, gpx, gpy);
+      }
+      double ixx = 0, iyy = 0, ixy = 0;
+      if (FVK_A_LATE < 2 && deg > 0) load_lsq_inv(m, c, ixx, iyy, ixy);
+      stage_block(a, b, L);
+      __syncthreads();
+      if (deg == 0) break;
+      T q[4];
+      lds_ld4(L.rec(i), q);
+      double xc, yc;
+      lds_xy(L, i, xc, yc);
+      T sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, qmin[4], qmax[4];
+      init_minmax(qmin, qmax);
+      auto slot = [&](int j, int code) {
+        T px[4], py[4], tmin[4], tmax[4];
+        if (code >= 0) {
+          const int nb = code >> 1;
+          T qn[4];
+          lds_ld4(L.rec(nb), qn);
+          double xn, yn;
+          lds_xy(L, nb, xn, yn);
+          lsq_interior(code & 1, q, xc, yc, qn, xn, yn, px, py, tmin, tmax);
+        } else {
+          // the 16-bit local code holds the kind only; the global code also
+          // names the no-slip ghost's source cell
+          const int s = base + j * 64;
+          lsq_boundary(a, fl, m.code[s], q, xc, yc, m.s_nx[s], m.s_ny[s], m.s_fcx[s], m.s_fcy[s],
+                       px, py, tmin, tmax);
+        }
+        accumulate_lsq(sx, sy, qmin, qmax, px, py, tmin, tmax);
+      };
+#pragma unroll
+      for (int j = 0; j < PF; ++j)
+        if (j < deg) slot(j, lc[j]);
+      for (int j = PF; j < deg; ++j) slot(j, m.lcode[base + j * 64]);
+      if (FVK_A_LATE >= 1) {
+        load_pf();
+        if (squeeze && !fl.bj) load_lagged(a, c, qp, gpx, gpy);
+      }
+      if (FVK_A_LATE >= 2) load_lsq_inv(m, c, ixx, iyy, ixy);
+      T gx[4], gy[4];
+      lsq_solve(ixx, iyy, ixy, sx, sy, gx, gy);
+      if (squeeze) {
+        T phi[4] = {1, 1, 1, 1};
+        if (fl.bj) current_as_lagged(q, gx, gy, qp, gpx, gpy);
+        const bool sent = a.sentinel && !fl.bj;
+        auto lim = [&](SlotPos<T> px, SlotPos<T> py) {
+          T ph[4];
+          if constexpr (kDelta<T>) face_phi(sent, fl, px, py, q, qmin, qmax, qp, gpx, gpy, ph);
+          else face_phi(sent, fl, T(px - xc), T(py - yc), q, qmin, qmax, qp, gpx, gpy, ph);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) phi[k] = min2(ph[k], phi[k]);
+        };
+#pragma unroll
+        for (int j = 0; j < PF; ++j)
+          if (j < deg) lim(pfx[j], pfy[j]);
+        for (int j = PF; j < deg; ++j) {
+          SlotPos<T> px, py;
+          slot_pos<T>(m, base + j * 64, px, py);
+          lim(px, py);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          gx[k] = gx[k] * phi[k];
+          gy[k] = gy[k] * phi[k];
+        }
+      }
+      write_grad(a, c, gx, gy);
+    } while (false);
+  } while (walk.next(args));
 }
 
 // One block face: owner and neighbour (local indices, or the boundary code)
@@ -1675,125 +1738,123 @@ __device__ __forceinline__ void lds_cell(const LdsCells<T, NQ, XY>& L, int l, Ce
   }
 }
 
-// Pass B of logical block lb (two threads per cell of the block).  LDS: cell
-// rows, then [4 or 5][max_bfaces] face terms (the spectral-sum term only at
-// stage 3, where the next step's dt needs it)
-template <typename T, int ORDER, bool NS, int PF, bool ROE>
-__device__ __forceinline__ void pass_b_tile(const StageArgsT<T>& a, int lb,
-                                            unsigned char* lds_raw) {
-  constexpr int NQ = ORDER == 2 ? 12 : 4;
-  constexpr bool XY = !kDelta<T>;
-  constexpr bool kLate = FVK_NS_LATE && NS && !kDelta<T>;   // FP64 NS: the leaner prefetch
-  constexpr int kFPFB = kLate ? 1 : kFPF;
-  const DevMesh& m = a.m;
-  int b = 0, inst = 0;
-  const bool live = block_of(a, lb, b, inst);   // uniform
-  const FlowT<T>& fl = a.flows[inst];
-  double sig_new = 0.0;
-  if (live) {
-    LdsCells<T, NQ, XY> L(lds_raw, m.blk + m.max_halo);
-    T* lf = reinterpret_cast<T*>(lds_raw + L.bytes());
-    const int fstride = m.max_bfaces;
-    const int fb = m.bf_ptr[b], nfb = m.bf_ptr[b + 1] - fb;
-    // threads [0, blk) own the block's cells; all threads share the faces
-    const int i = threadIdx.x;
-    const bool cell_thread = i < m.blk;
-    const int c = b * m.blk + (cell_thread ? i : 0);
-    const int deg = cell_thread ? m.deg[c] : 0;
-    const int base = m.chunk_off[c >> 6] + (c & 63);
-    // prefetch: this thread's faces, its cell's slot-to-face map and the
-    // cell data of the stage update.  The FP64 NS kernel, whose face terms
-    // need the most registers, prefetches one face and loads the cell data
-    // after the face phase, so that fewer values stay live through it.
-    FaceRec<T, NS> fr[kFPFB];
+// Pass B, two threads per cell of the block.  LDS: cell rows, then [4 or
+// 5][max_bfaces] face terms (the spectral-sum term only at stage 3, where the
+// next step's dt needs it)
+template <typename T, int ORDER, bool NS, int PF, bool ROE = false, bool PERSIST = false>
+__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
+void k_pass_b_lds(StageArgsT<T> args) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  TileWalk<PERSIST> walk(args);
+  do {
+    const StageArgsT<T>& a = tile_args<PERSIST>(args);
+    constexpr int NQ = ORDER == 2 ? 12 : 4;
+    constexpr bool XY = !kDelta<T>;
+    constexpr bool kLate = FVK_NS_LATE && NS && !kDelta<T>;   // FP64 NS: the leaner prefetch
+    constexpr int kFPFB = kLate ? 1 : kFPF;
+    const DevMesh& m = a.m;
+    int b = 0, inst = 0;
+    const bool live = block_of(a, walk.block(a), b, inst);   // uniform
+    const FlowT<T>& fl = a.flows[inst];
+    double sig_new = 0.0;
+    if (live) {
+      LdsCells<T, NQ, XY> L(lds_raw, m.blk + m.max_halo);
+      T* lf = reinterpret_cast<T*>(lds_raw + L.bytes());
+      const int fstride = m.max_bfaces;
+      const int fb = m.bf_ptr[b], nfb = m.bf_ptr[b + 1] - fb;
+      // threads [0, blk) own the block's cells; all threads share the faces
+      const int i = threadIdx.x;
+      const bool cell_thread = i < m.blk;
+      const int c = b * m.blk + (cell_thread ? i : 0);
+      const int deg = cell_thread ? m.deg[c] : 0;
+      const int base = m.chunk_off[c >> 6] + (c & 63);
+      // prefetch: this thread's faces, its cell's slot-to-face map and the
+      // cell data of the stage update.  The FP64 NS kernel, whose face terms
+      // need the most registers, prefetches one face and loads the cell data
+      // after the face phase, so that fewer values stay live through it.
+      FaceRec<T, NS> fr[kFPFB];
 #pragma unroll
-    for (int t = 0; t < kFPFB; ++t) {
-      const int e = i + t * blockDim.x;
-      if (e < nfb) load_face(m, fb + e, fr[t]);
-    }
-    int sf[PF];
-#pragma unroll
-    for (int j = 0; j < PF; ++j)
-      if (j < deg) sf[j] = m.sface[base + j * 64];
-    T qn[4] = {0, 0, 0, 0}, sig_old = T(1);
-    double V = 1.0;
-    if (!kLate && deg > 0) {
-      if (a.stage > 1) load_q(a.qn, c, qn);
-      V = m.vol[c];
-      if (!a.dt_global) sig_old = a.sigma[c];
-    }
-    stage_block(a, b, L);
-    __syncthreads();
-    // one thread per face of the block
-    auto face = [&](int e, const FaceRec<T, NS>& f) {
-      T x[4], st;
-      CellQG<T> c1, c2;
-      lds_cell<T, ORDER>(L, f.ia, c1);
-      if (f.ib >= 0) {
-        lds_cell<T, ORDER>(L, f.ib, c2);
-        FaceGeo<T> g;
-        if constexpr (kDelta<T>) g = f.g;
-        else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, &c2, -1);
-        face_flux<ORDER, NS, ROE>(a, fl, c1, &c2, -1, g, x, st);
-      } else {
-        const int kind = (~f.ib) & 3;
-        FaceGeo<T> g;
-        if constexpr (kDelta<T>) g = f.g;
-        else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, nullptr, kind);
-        face_flux<ORDER, NS, ROE>(a, fl, c1, (const CellQG<T>*)nullptr, kind, g, x, st);
+      for (int t = 0; t < kFPFB; ++t) {
+        const int e = i + t * blockDim.x;
+        if (e < nfb) load_face(m, fb + e, fr[t]);
       }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) lf[k * fstride + e] = x[k];
-      if (a.last) lf[4 * fstride + e] = st;
-    };
-#pragma unroll
-    for (int t = 0; t < kFPFB; ++t) {
-      const int e = i + t * blockDim.x;
-      if (e < nfb) face(e, fr[t]);
-    }
-    for (int e = i + kFPFB * blockDim.x; e < nfb; e += blockDim.x) {
-      FaceRec<T, NS> f;
-      load_face(m, fb + e, f);
-      face(e, f);
-    }
-    if (kLate && deg > 0) {
-      if (a.stage > 1) load_q(a.qn, c, qn);
-      V = m.vol[c];
-      if (!a.dt_global) sig_old = a.sigma[c];
-    }
-    __syncthreads();
-    // one thread per cell: add the face terms in slot order (res.cpp:74-102)
-    if (deg > 0) {
-      T r[4] = {0, 0, 0, 0}, sg = 0;
-      auto add = [&](int code) {
-        const int e = code >> 1;
-        const bool own = (code & 1) != 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const T x = lf[k * fstride + e];
-          r[k] = r[k] + (own ? x : -x);
-        }
-        if (a.last) sg += lf[4 * fstride + e];
-      };
+      int sf[PF];
 #pragma unroll
       for (int j = 0; j < PF; ++j)
-        if (j < deg) add(sf[j]);
-      for (int j = PF; j < deg; ++j) add(m.sface[base + j * 64]);
-      T q[4];
-      lds_ld4(L.rec(i), q);
-      cell_update(a, fl, inst, c, q, qn, r, sg, V, sig_old);
-      sig_new = dt_key(a, sg, V);
+        if (j < deg) sf[j] = m.sface[base + j * 64];
+      T qn[4] = {0, 0, 0, 0}, sig_old = T(1);
+      double V = 1.0;
+      if (!kLate && deg > 0) {
+        if (a.stage > 1) load_q(a.qn, c, qn);
+        V = m.vol[c];
+        if (!a.dt_global) sig_old = a.sigma[c];
+      }
+      stage_block(a, b, L);
+      __syncthreads();
+      // one thread per face of the block
+      auto face = [&](int e, const FaceRec<T, NS>& f) {
+        T x[4], st;
+        CellQG<T> c1, c2;
+        lds_cell<T, ORDER>(L, f.ia, c1);
+        if (f.ib >= 0) {
+          lds_cell<T, ORDER>(L, f.ib, c2);
+          FaceGeo<T> g;
+          if constexpr (kDelta<T>) g = f.g;
+          else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, &c2, -1);
+          face_flux<ORDER, NS, ROE>(a, fl, c1, &c2, -1, g, x, st);
+        } else {
+          const int kind = (~f.ib) & 3;
+          FaceGeo<T> g;
+          if constexpr (kDelta<T>) g = f.g;
+          else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, nullptr, kind);
+          face_flux<ORDER, NS, ROE>(a, fl, c1, (const CellQG<T>*)nullptr, kind, g, x, st);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lf[k * fstride + e] = x[k];
+        if (a.last) lf[4 * fstride + e] = st;
+      };
+#pragma unroll
+      for (int t = 0; t < kFPFB; ++t) {
+        const int e = i + t * blockDim.x;
+        if (e < nfb) face(e, fr[t]);
+      }
+      for (int e = i + kFPFB * blockDim.x; e < nfb; e += blockDim.x) {
+        FaceRec<T, NS> f;
+        load_face(m, fb + e, f);
+        face(e, f);
+      }
+      if (kLate && deg > 0) {
+        if (a.stage > 1) load_q(a.qn, c, qn);
+        V = m.vol[c];
+        if (!a.dt_global) sig_old = a.sigma[c];
+      }
+      __syncthreads();
+      // one thread per cell: add the face terms in slot order (res.cpp:74-102)
+      if (deg > 0) {
+        T r[4] = {0, 0, 0, 0}, sg = 0;
+        auto add = [&](int code) {
+          const int e = code >> 1;
+          const bool own = (code & 1) != 0;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const T x = lf[k * fstride + e];
+            r[k] = r[k] + (own ? x : -x);
+          }
+          if (a.last) sg += lf[4 * fstride + e];
+        };
+#pragma unroll
+        for (int j = 0; j < PF; ++j)
+          if (j < deg) add(sf[j]);
+        for (int j = PF; j < deg; ++j) add(m.sface[base + j * 64]);
+        T q[4];
+        lds_ld4(L.rec(i), q);
+        cell_update(a, fl, inst, c, q, qn, r, sg, V, sig_old);
+        sig_new = dt_key(a, sg, V);
+      }
     }
-  }
-  if (live && a.last && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
-  if (live && a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) step_start(a, fl, inst);
-}
-
-template <typename T, int ORDER, bool NS, int PF, bool ROE = false>
-__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
-void k_pass_b_lds(StageArgsT<T> a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_b_tile<T, ORDER, NS, PF, ROE>(a, logical_block(a.xcd_blocks), lds_raw);
+    if (live && a.last && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
+    if (live && a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) step_start(a, fl, inst);
+  } while (walk.next(args));
 }
 
 // ---------------------------------------------------------------------------
@@ -1811,135 +1872,81 @@ void k_pass_b_lds(StageArgsT<T> a) {
 // terms in run-time order, so results differ from the gather (and the
 // reference) by summation order only.
 // ---------------------------------------------------------------------------
-template <typename T, int ORDER, bool NS>
-__device__ __forceinline__ void pass_b_atomic_tile(const StageArgsT<T>& a, int lb,
-                                                   unsigned char* lds_raw) {
-  constexpr int NQ = ORDER == 2 ? 12 : 4;
-  constexpr bool XY = !kDelta<T>;
-  const DevMesh& m = a.m;
-  int b = 0, inst = 0;
-  const bool live = block_of(a, lb, b, inst);   // uniform
-  const FlowT<T>& fl = a.flows[inst];
-  double sig_new = 0.0;
-  if (live) {
-    LdsCells<T, NQ, XY> L(lds_raw, m.blk + m.max_halo);
-    // [5][blk]: the face sums and the spectral sum of each cell of the block
-    double* acl = reinterpret_cast<double*>(lds_raw + ((L.bytes() + 7) & ~size_t(7)));
-    const int fb = m.bf_ptr[b], nfb = m.bf_ptr[b + 1] - fb;
-    const int i = threadIdx.x;
-    if (i < m.blk) {
+template <typename T, int ORDER, bool NS, bool PERSIST = false>
+__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
+void k_pass_b_atomic(StageArgsT<T> args) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  TileWalk<PERSIST> walk(args);
+  do {
+    const StageArgsT<T>& a = tile_args<PERSIST>(args);
+    constexpr int NQ = ORDER == 2 ? 12 : 4;
+    constexpr bool XY = !kDelta<T>;
+    const DevMesh& m = a.m;
+    int b = 0, inst = 0;
+    const bool live = block_of(a, walk.block(a), b, inst);   // uniform
+    const FlowT<T>& fl = a.flows[inst];
+    double sig_new = 0.0;
+    if (live) {
+      LdsCells<T, NQ, XY> L(lds_raw, m.blk + m.max_halo);
+      // [5][blk]: the face sums and the spectral sum of each cell of the block
+      double* acl = reinterpret_cast<double*>(lds_raw + ((L.bytes() + 7) & ~size_t(7)));
+      const int fb = m.bf_ptr[b], nfb = m.bf_ptr[b + 1] - fb;
+      const int i = threadIdx.x;
+      if (i < m.blk) {
 #pragma unroll
-      for (int k = 0; k < 5; ++k) acl[k * m.blk + i] = 0.0;
-    }
-    stage_block(a, b, L);
-    __syncthreads();
-    for (int e = i; e < nfb; e += blockDim.x) {
-      FaceRec<T, NS> f;
-      load_face(m, fb + e, f);
-      CellQG<T> c1, c2;
-      lds_cell<T, ORDER>(L, f.ia, c1);
-      T x[4], st;
-      if (f.ib >= 0) {
-        lds_cell<T, ORDER>(L, f.ib, c2);
-        FaceGeo<T> g;
-        if constexpr (kDelta<T>) g = f.g;
-        else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, &c2, -1);
-        face_flux<ORDER, NS>(a, fl, c1, &c2, -1, g, x, st);
-        if (f.ib < m.blk) {
+        for (int k = 0; k < 5; ++k) acl[k * m.blk + i] = 0.0;
+      }
+      stage_block(a, b, L);
+      __syncthreads();
+      for (int e = i; e < nfb; e += blockDim.x) {
+        FaceRec<T, NS> f;
+        load_face(m, fb + e, f);
+        CellQG<T> c1, c2;
+        lds_cell<T, ORDER>(L, f.ia, c1);
+        T x[4], st;
+        if (f.ib >= 0) {
+          lds_cell<T, ORDER>(L, f.ib, c2);
+          FaceGeo<T> g;
+          if constexpr (kDelta<T>) g = f.g;
+          else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, &c2, -1);
+          face_flux<ORDER, NS>(a, fl, c1, &c2, -1, g, x, st);
+          if (f.ib < m.blk) {
 #pragma unroll
-          for (int k = 0; k < 4; ++k) atomicAdd(acl + k * m.blk + f.ib, -double(x[k]));
-          if (a.last) atomicAdd(acl + 4 * m.blk + f.ib, double(st));
+            for (int k = 0; k < 4; ++k) atomicAdd(acl + k * m.blk + f.ib, -double(x[k]));
+            if (a.last) atomicAdd(acl + 4 * m.blk + f.ib, double(st));
+          }
+        } else {
+          const int kind = (~f.ib) & 3;
+          FaceGeo<T> g;
+          if constexpr (kDelta<T>) g = f.g;
+          else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, nullptr, kind);
+          face_flux<ORDER, NS>(a, fl, c1, (const CellQG<T>*)nullptr, kind, g, x, st);
         }
-      } else {
-        const int kind = (~f.ib) & 3;
-        FaceGeo<T> g;
-        if constexpr (kDelta<T>) g = f.g;
-        else g = geo_from<T, NS>(f.nx, f.ny, f.A, f.fx, f.fy, c1, nullptr, kind);
-        face_flux<ORDER, NS>(a, fl, c1, (const CellQG<T>*)nullptr, kind, g, x, st);
-      }
-      if (f.ia < m.blk) {
+        if (f.ia < m.blk) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) atomicAdd(acl + k * m.blk + f.ia, double(x[k]));
-        if (a.last) atomicAdd(acl + 4 * m.blk + f.ia, double(st));
+          for (int k = 0; k < 4; ++k) atomicAdd(acl + k * m.blk + f.ia, double(x[k]));
+          if (a.last) atomicAdd(acl + 4 * m.blk + f.ia, double(st));
+        }
+      }
+      __syncthreads();
+      if (i < m.blk) {
+        const int c = b * m.blk + i;
+        if (m.deg[c] > 0) {
+          T r[4], q[4], qn[4] = {0, 0, 0, 0};
+#pragma unroll
+          for (int k = 0; k < 4; ++k) r[k] = T(acl[k * m.blk + i]);
+          const T sg = T(acl[4 * m.blk + i]);
+          lds_ld4(L.rec(i), q);
+          if (a.stage > 1) load_q(a.qn, c, qn);
+          const T sig_old = a.dt_global ? T(1) : a.sigma[c];
+          cell_update(a, fl, inst, c, q, qn, r, sg, m.vol[c], sig_old);
+          sig_new = dt_key(a, sg, m.vol[c]);
+        }
       }
     }
-    __syncthreads();
-    if (i < m.blk) {
-      const int c = b * m.blk + i;
-      if (m.deg[c] > 0) {
-        T r[4], q[4], qn[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) r[k] = T(acl[k * m.blk + i]);
-        const T sg = T(acl[4 * m.blk + i]);
-        lds_ld4(L.rec(i), q);
-        if (a.stage > 1) load_q(a.qn, c, qn);
-        const T sig_old = a.dt_global ? T(1) : a.sigma[c];
-        cell_update(a, fl, inst, c, q, qn, r, sg, m.vol[c], sig_old);
-        sig_new = dt_key(a, sg, m.vol[c]);
-      }
-    }
-  }
-  if (live && a.last && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
-  if (live && a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) step_start(a, fl, inst);
-}
-
-template <typename T, int ORDER, bool NS>
-__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
-void k_pass_b_atomic(StageArgsT<T> a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  pass_b_atomic_tile<T, ORDER, NS>(a, logical_block(a.xcd_blocks), lds_raw);
-}
-
-// ---------------------------------------------------------------------------
-// The LDS-staged kernels on a persistent grid (kernel_variant 4): as many
-// workgroups as the device holds at once, a multiple of 8.  Workgroup w runs
-// on XCD w % 8 and walks the tiles (mesh blocks) that the one-workgroup-per-
-// block launch deals to that XCD, in the same order, striding by the XCD's
-// workgroup count -- so the blocks that share an L2 are the same.  The
-// assignment is static: no counters, no atomics, no workgroup waits for
-// another (a kernel boundary still separates the passes and the stages), and
-// each tile's work is that of the one-workgroup-per-block launch, bit for bit.
-// ---------------------------------------------------------------------------
-template <typename T, typename Body>
-__device__ __forceinline__ void tile_loop(const StageArgsT<T>& a, Body body) {
-  const int q = blockIdx.x & 7, per_q = gridDim.x >> 3;
-  for (int k = blockIdx.x >> 3; k < a.tiles_per_xcd; k += per_q) {
-    const int v = k * 8 + q;     // this tile's workgroup in the one-per-block launch
-    if (v < a.tiles_total) body(logical_block_of(v, a.xcd_blocks));
-    __syncthreads();             // the tile's LDS reads are done before the next staging
-  }
-}
-
-// The kernel's argument block, read again from the kernarg segment for every
-// tile.  Left loop-invariant, the compiler hoists the argument loads and
-// everything derived from them out of the tile loop, and the spilled scalars
-// cost registers and occupancy.
-template <typename T>
-__device__ __forceinline__ const StageArgsT<T>& fresh_args() {
-  using CP = const __attribute__((address_space(4))) StageArgsT<T>*;
-  CP p = (CP)__builtin_amdgcn_kernarg_segment_ptr();
-  asm volatile("" : "+s"(p));
-  return *(const StageArgsT<T>*)p;
-}
-
-template <typename T, bool SQUEEZE, int PF>
-__global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_persist(StageArgsT<T> a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  tile_loop(a, [&](int lb) { pass_a_tile<T, SQUEEZE, PF>(fresh_args<T>(), lb, lds_raw); });
-}
-
-template <typename T, int ORDER, bool NS, int PF, bool ROE = false>
-__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
-void k_pass_b_persist(StageArgsT<T> a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  tile_loop(a, [&](int lb) { pass_b_tile<T, ORDER, NS, PF, ROE>(fresh_args<T>(), lb, lds_raw); });
-}
-
-template <typename T, int ORDER, bool NS>
-__global__ __launch_bounds__(kMaxBlock * kPassBThreadsPerCell, kMinBlocksB<T>)
-void k_pass_b_atomic_persist(StageArgsT<T> a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  tile_loop(a, [&](int lb) { pass_b_atomic_tile<T, ORDER, NS>(fresh_args<T>(), lb, lds_raw); });
+    if (live && a.last && a.dt_global) block_max_to(sig_new, a.smax_next + inst);
+    if (live && a.stage == 1 && threadIdx.x == 0 && b == fl.first_block) step_start(a, fl, inst);
+  } while (walk.next(args));
 }
 
 // Spectral sums of the first step: every face carries |V_inf| and c_inf
@@ -2193,20 +2200,20 @@ hipError_t launch_tiles(StageArgsT<T> a, StageKernel<T> k1, StageKernel<T> kp, i
 template <typename T, bool SQUEEZE, int PF>
 void pick_a(StageKernel<T>* k1, StageKernel<T>* kp) {
   *k1 = k_pass_a_lds<T, SQUEEZE, PF>;
-  *kp = k_pass_a_persist<T, SQUEEZE, PF>;
+  *kp = k_pass_a_lds<T, SQUEEZE, PF, true>;
 }
 
 template <typename T, int PF, bool ROE>
 void pick_b(int order, int ns, StageKernel<T>* k1, StageKernel<T>* kp) {
   if (order == 1) {
     *k1 = k_pass_b_lds<T, 1, false, PF, ROE>;
-    *kp = k_pass_b_persist<T, 1, false, PF, ROE>;
+    *kp = k_pass_b_lds<T, 1, false, PF, ROE, true>;
   } else if (ns) {
     *k1 = k_pass_b_lds<T, 2, true, PF, ROE>;
-    *kp = k_pass_b_persist<T, 2, true, PF, ROE>;
+    *kp = k_pass_b_lds<T, 2, true, PF, ROE, true>;
   } else {
     *k1 = k_pass_b_lds<T, 2, false, PF, ROE>;
-    *kp = k_pass_b_persist<T, 2, false, PF, ROE>;
+    *kp = k_pass_b_lds<T, 2, false, PF, ROE, true>;
   }
 }
 
@@ -2250,13 +2257,13 @@ hipError_t launch_pass_b(const StageArgsT<T>& a0, int order, int ns, int block, 
     if (a.accumulate) {
       if (order == 1) {
         k1 = k_pass_b_atomic<T, 1, false>;
-        kp = k_pass_b_atomic_persist<T, 1, false>;
+        kp = k_pass_b_atomic<T, 1, false, true>;
       } else if (ns) {
         k1 = k_pass_b_atomic<T, 2, true>;
-        kp = k_pass_b_atomic_persist<T, 2, true>;
+        kp = k_pass_b_atomic<T, 2, true, true>;
       } else {
         k1 = k_pass_b_atomic<T, 2, false>;
-        kp = k_pass_b_atomic_persist<T, 2, false>;
+        kp = k_pass_b_atomic<T, 2, false, true>;
       }
       return launch_tiles(a, k1, kp, grid, threads, lds_bytes_atomic(a.m, order, sizeof(T)), s);
     }
