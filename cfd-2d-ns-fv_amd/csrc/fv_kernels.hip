@@ -2173,6 +2173,7 @@ hipError_t persist_grid(StageKernel<T> k, int threads, size_t lds, int tiles, in
       return e;
     resident = std::max(8, per_cu * cus / 8 * 8);
     std::lock_guard<std::mutex> lock(mu);
+    if (cache.size() >= 256) cache.clear();   // many meshes in one process: start again
     cache.push_back({Key{(const void*)k, threads, dev, lds}, resident});
   }
   *grid = std::min(resident, (tiles + 7) / 8 * 8);
