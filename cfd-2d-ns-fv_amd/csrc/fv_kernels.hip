@@ -197,6 +197,58 @@ __device__ __forceinline__ void st4(double* p, const double v[4]) {
   reinterpret_cast<d2v*>(p)[1] = y;
 }
 
+// Non-temporal access for the streams that a launch reads or writes once, so
+// that they do not displace from the L2 the cell data that neighbouring
+// blocks read again as their halo.  Build option FVK_NT, a bit mask, measured
+// per launch on MI355X (256-instance batch, ramp, NS cylinder):
+//   1  pass-B loads of the face records, slot map and stage-update data:
+//      FP32-mixed pass B 3-14% slower;
+//   2  FP32-mixed stores of the new state, spectral sums, residual and
+//      gradients: pass A 3.6% and pass B 2.6% faster on the batch;
+//   4  pass-A loads of the slot data, LSQ inverse and lagged state: FP64
+//      pass A 16-25% slower;
+//   8  FP64 stores of the new state, spectral sums and residual: pass B
+//      1-2% faster;
+//  16  FP64 gradient stores (four 16-byte stores per cell): FP64 pass A 27%
+//      slower on the batch.
+#ifndef FVK_NT
+#define FVK_NT 10
+#endif
+template <int BIT, typename P>
+__device__ __forceinline__ P sld(const P* p) {
+  if constexpr ((FVK_NT & BIT) != 0) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <int BIT, typename P>
+__device__ __forceinline__ void sst(P* p, P v) {
+  if constexpr ((FVK_NT & BIT) != 0) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+template <int BIT>
+__device__ __forceinline__ void ld4s(const float* p, float v[4]) {
+  const f4v x = sld<BIT>(reinterpret_cast<const f4v*>(p));
+  v[0] = x[0]; v[1] = x[1]; v[2] = x[2]; v[3] = x[3];
+}
+template <int BIT>
+__device__ __forceinline__ void ld4s(const double* p, double v[4]) {
+  const d2v x = sld<BIT>(reinterpret_cast<const d2v*>(p));
+  const d2v y = sld<BIT>(reinterpret_cast<const d2v*>(p) + 1);
+  v[0] = x[0]; v[1] = x[1]; v[2] = y[0]; v[3] = y[1];
+}
+template <int BIT>
+__device__ __forceinline__ void st4s(float* p, const float v[4]) {
+  f4v x;
+  x[0] = v[0]; x[1] = v[1]; x[2] = v[2]; x[3] = v[3];
+  sst<BIT>(reinterpret_cast<f4v*>(p), x);
+}
+template <int BIT>
+__device__ __forceinline__ void st4s(double* p, const double v[4]) {
+  d2v x, y;
+  x[0] = v[0]; x[1] = v[1]; y[0] = v[2]; y[1] = v[3];
+  sst<BIT>(reinterpret_cast<d2v*>(p), x);
+  sst<BIT>(reinterpret_cast<d2v*>(p) + 1, y);
+}
+
 template <typename T>
 __device__ __forceinline__ void load_q(const T* __restrict__ qa, int c, T q[4]) {
   ld4(qa + 4 * (size_t)c, q);
@@ -955,10 +1007,11 @@ __device__ __forceinline__ void cell_update(const StageArgsT<T>& a, const FlowT<
 #pragma unroll
     for (int k = 0; k < 4; ++k) out[k] = ca * qn[k] + cb * q[k] + cb * dt * r[k];
   }
-  st4(a.qout + 4 * (size_t)c, out);
+  constexpr int kNT = kDelta<T> ? 2 : 8;
+  st4s<kNT>(a.qout + 4 * (size_t)c, out);
   if (a.last) {
-    a.sigma[c] = sg;
-    if (a.store_res) st4(a.res + 4 * (size_t)c, r);
+    sst<kNT>(a.sigma + c, sg);
+    if (a.store_res) st4s<kNT>(a.res + 4 * (size_t)c, r);
   }
 }
 
@@ -979,13 +1032,13 @@ __device__ __forceinline__ void cell_update_ld(const StageArgsT<T>& a, const Flo
 __device__ __forceinline__ void load_lsq_inv(const DevMesh& m, int c, double& ixx, double& iyy,
                                              double& ixy) {
   if (m.ixx32) {
-    ixx = m.ixx32[c];
-    iyy = m.iyy32[c];
-    ixy = m.ixy32[c];
+    ixx = sld<4>(m.ixx32 + c);
+    iyy = sld<4>(m.iyy32 + c);
+    ixy = sld<4>(m.ixy32 + c);
   } else {
-    ixx = m.ixx[c];
-    iyy = m.iyy[c];
-    ixy = m.ixy[c];
+    ixx = sld<4>(m.ixx + c);
+    iyy = sld<4>(m.iyy + c);
+    ixy = sld<4>(m.ixy + c);
   }
 }
 
@@ -1009,8 +1062,9 @@ __device__ __forceinline__ void lsq_solve(double ixx, double iyy, double ixy, co
 template <typename T>
 __device__ __forceinline__ void write_grad(const StageArgsT<T>& a, int c, const T gx[4],
                                            const T gy[4]) {
-  st4(a.g + 8 * (size_t)c, gx);
-  st4(a.g + 8 * (size_t)c + 4, gy);
+  constexpr int kNT = kDelta<T> ? 2 : 16;
+  st4s<kNT>(a.g + 8 * (size_t)c, gx);
+  st4s<kNT>(a.g + 8 * (size_t)c + 4, gy);
 }
 
 template <typename T>
@@ -1018,8 +1072,9 @@ __device__ __forceinline__ void load_lagged(const StageArgsT<T>& a, int c, T qp[
                                             T gpy[4]) {
   // the buffers always hold finite values (zero-filled at creation), and
   // face_phi ignores them while the sentinel is armed
-  load_q(a.qp, c, qp);
-  load_g(a.gp, c, gpx, gpy);
+  ld4s<4>(a.qp + 4 * (size_t)c, qp);
+  ld4s<4>(a.gp + 8 * (size_t)c, gpx);
+  ld4s<4>(a.gp + 8 * (size_t)c + 4, gpy);
 }
 
 // The Barth-Jespersen limiter (limiter 3, an extension: T. J. Barth and D. C.
@@ -1500,11 +1555,11 @@ using SlotPos = typename std::conditional<kDelta<T>, float, double>::type;
 template <typename T>
 __device__ __forceinline__ void slot_pos(const DevMesh& m, int s, SlotPos<T>& px, SlotPos<T>& py) {
   if constexpr (kDelta<T>) {
-    px = m.s_dfx32[s];
-    py = m.s_dfy32[s];
+    px = sld<4>(m.s_dfx32 + s);
+    py = sld<4>(m.s_dfy32 + s);
   } else {
-    px = m.s_fcx[s];
-    py = m.s_fcy[s];
+    px = sld<4>(m.s_fcx + s);
+    py = sld<4>(m.s_fcy + s);
   }
 }
 
@@ -1601,7 +1656,7 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
       };
 #pragma unroll
       for (int j = 0; j < PF; ++j)
-        if (j < deg) lc[j] = m.lcode[base + j * 64];
+        if (j < deg) lc[j] = sld<4>(m.lcode + base + j * 64);
       T qp[4], gpx[4], gpy[4];
       if (FVK_A_LATE < 1) {
         load_pf();
@@ -1639,7 +1694,7 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
 #pragma unroll
       for (int j = 0; j < PF; ++j)
         if (j < deg) slot(j, lc[j]);
-      for (int j = PF; j < deg; ++j) slot(j, m.lcode[base + j * 64]);
+      for (int j = PF; j < deg; ++j) slot(j, sld<4>(m.lcode + base + j * 64));
       if (FVK_A_LATE >= 1) {
         load_pf();
         if (squeeze && !fl.bj) load_lagged(a, c, qp, gpx, gpy);
@@ -1694,29 +1749,29 @@ struct FaceRec<float, NS> {
 
 template <typename T, bool NS>
 __device__ __forceinline__ void load_face(const DevMesh& m, int e, FaceRec<T, NS>& f) {
-  f.ia = m.bf_a[e];
-  f.ib = m.bf_b[e];
+  f.ia = sld<1>(m.bf_a + e);
+  f.ib = sld<1>(m.bf_b + e);
   if constexpr (kDelta<T>) {
-    f.g.nx = m.bf32_nx[e];
-    f.g.ny = m.bf32_ny[e];
-    f.g.A = m.bf32_area[e];
-    f.g.d1x = m.bf32_d1x[e];
-    f.g.d1y = m.bf32_d1y[e];
-    f.g.d2x = m.bf32_d2x[e];
-    f.g.d2y = m.bf32_d2y[e];
+    f.g.nx = sld<1>(m.bf32_nx + e);
+    f.g.ny = sld<1>(m.bf32_ny + e);
+    f.g.A = sld<1>(m.bf32_area + e);
+    f.g.d1x = sld<1>(m.bf32_d1x + e);
+    f.g.d1y = sld<1>(m.bf32_d1y + e);
+    f.g.d2x = sld<1>(m.bf32_d2x + e);
+    f.g.d2y = sld<1>(m.bf32_d2y + e);
     if (NS) {
-      f.g.lx = m.bf32_lx[e];
-      f.g.ly = m.bf32_ly[e];
-      f.g.dist = m.bf32_dist[e];
+      f.g.lx = sld<1>(m.bf32_lx + e);
+      f.g.ly = sld<1>(m.bf32_ly + e);
+      f.g.dist = sld<1>(m.bf32_dist + e);
     } else {
       f.g.lx = f.g.ly = f.g.dist = 0.0f;
     }
   } else {
-    f.nx = m.bf_nx[e];
-    f.ny = m.bf_ny[e];
-    f.A = m.bf_area[e];
-    f.fx = m.bf_fcx[e];
-    f.fy = m.bf_fcy[e];
+    f.nx = sld<1>(m.bf_nx + e);
+    f.ny = sld<1>(m.bf_ny + e);
+    f.A = sld<1>(m.bf_area + e);
+    f.fx = sld<1>(m.bf_fcx + e);
+    f.fy = sld<1>(m.bf_fcy + e);
   }
 }
 
@@ -1781,13 +1836,13 @@ void k_pass_b_lds(StageArgsT<T> args) {
       int sf[PF];
 #pragma unroll
       for (int j = 0; j < PF; ++j)
-        if (j < deg) sf[j] = m.sface[base + j * 64];
+        if (j < deg) sf[j] = sld<1>(m.sface + base + j * 64);
       T qn[4] = {0, 0, 0, 0}, sig_old = T(1);
       double V = 1.0;
       if (!kLate && deg > 0) {
-        if (a.stage > 1) load_q(a.qn, c, qn);
-        V = m.vol[c];
-        if (!a.dt_global) sig_old = a.sigma[c];
+        if (a.stage > 1) ld4s<1>(a.qn + 4 * (size_t)c, qn);
+        V = sld<1>(m.vol + c);
+        if (!a.dt_global) sig_old = sld<1>(a.sigma + c);
       }
       stage_block(a, b, L);
       __syncthreads();
@@ -1824,9 +1879,9 @@ void k_pass_b_lds(StageArgsT<T> args) {
         face(e, f);
       }
       if (kLate && deg > 0) {
-        if (a.stage > 1) load_q(a.qn, c, qn);
-        V = m.vol[c];
-        if (!a.dt_global) sig_old = a.sigma[c];
+        if (a.stage > 1) ld4s<1>(a.qn + 4 * (size_t)c, qn);
+        V = sld<1>(m.vol + c);
+        if (!a.dt_global) sig_old = sld<1>(a.sigma + c);
       }
       __syncthreads();
       // one thread per cell: add the face terms in slot order (res.cpp:74-102)
@@ -1845,7 +1900,7 @@ void k_pass_b_lds(StageArgsT<T> args) {
 #pragma unroll
         for (int j = 0; j < PF; ++j)
           if (j < deg) add(sf[j]);
-        for (int j = PF; j < deg; ++j) add(m.sface[base + j * 64]);
+        for (int j = PF; j < deg; ++j) add(sld<1>(m.sface + base + j * 64));
         T q[4];
         lds_ld4(L.rec(i), q);
         cell_update(a, fl, inst, c, q, qn, r, sg, V, sig_old);
