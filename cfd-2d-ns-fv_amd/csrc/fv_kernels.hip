@@ -210,7 +210,10 @@ __device__ __forceinline__ void st4(double* p, const double v[4]) {
 //   8  FP64 stores of the new state, spectral sums and residual: pass B
 //      1-2% faster;
 //  16  FP64 gradient stores (four 16-byte stores per cell): FP64 pass A 27%
-//      slower on the batch.
+//      slower on the batch;
+//  32  pass-B loads of the stage-update data alone: FP32-mixed pass B 1%
+//      faster on the batch, 2-4% slower on the ramp and the cylinder;
+//  64  pass-A loads of the lagged state alone: pass A 2-14% slower.
 #ifndef FVK_NT
 #define FVK_NT 10
 #endif
@@ -1072,9 +1075,9 @@ __device__ __forceinline__ void load_lagged(const StageArgsT<T>& a, int c, T qp[
                                             T gpy[4]) {
   // the buffers always hold finite values (zero-filled at creation), and
   // face_phi ignores them while the sentinel is armed
-  ld4s<4>(a.qp + 4 * (size_t)c, qp);
-  ld4s<4>(a.gp + 8 * (size_t)c, gpx);
-  ld4s<4>(a.gp + 8 * (size_t)c + 4, gpy);
+  ld4s<64>(a.qp + 4 * (size_t)c, qp);
+  ld4s<64>(a.gp + 8 * (size_t)c, gpx);
+  ld4s<64>(a.gp + 8 * (size_t)c + 4, gpy);
 }
 
 // The Barth-Jespersen limiter (limiter 3, an extension: T. J. Barth and D. C.
@@ -1840,9 +1843,9 @@ void k_pass_b_lds(StageArgsT<T> args) {
       T qn[4] = {0, 0, 0, 0}, sig_old = T(1);
       double V = 1.0;
       if (!kLate && deg > 0) {
-        if (a.stage > 1) ld4s<1>(a.qn + 4 * (size_t)c, qn);
-        V = sld<1>(m.vol + c);
-        if (!a.dt_global) sig_old = sld<1>(a.sigma + c);
+        if (a.stage > 1) ld4s<32>(a.qn + 4 * (size_t)c, qn);
+        V = sld<32>(m.vol + c);
+        if (!a.dt_global) sig_old = sld<32>(a.sigma + c);
       }
       stage_block(a, b, L);
       __syncthreads();
@@ -1879,9 +1882,9 @@ void k_pass_b_lds(StageArgsT<T> args) {
         face(e, f);
       }
       if (kLate && deg > 0) {
-        if (a.stage > 1) ld4s<1>(a.qn + 4 * (size_t)c, qn);
-        V = sld<1>(m.vol + c);
-        if (!a.dt_global) sig_old = sld<1>(a.sigma + c);
+        if (a.stage > 1) ld4s<32>(a.qn + 4 * (size_t)c, qn);
+        V = sld<32>(m.vol + c);
+        if (!a.dt_global) sig_old = sld<32>(a.sigma + c);
       }
       __syncthreads();
       // one thread per cell: add the face terms in slot order (res.cpp:74-102)
