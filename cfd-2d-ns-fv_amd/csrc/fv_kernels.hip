@@ -209,13 +209,16 @@ __device__ __forceinline__ void st4(double* p, const double v[4]) {
 //      pass A 16-25% slower;
 //   8  FP64 stores of the new state, spectral sums and residual: pass B
 //      1-2% faster;
-//  16  FP64 gradient stores (four 16-byte stores per cell): FP64 pass A 27%
-//      slower on the batch;
+//  16  FP64 gradient stores of whole 1 KB ranges per wave (FVK_A_GW): FP64
+//      pass A 5% faster on the batch;
 //  32  pass-B loads of the stage-update data alone: FP32-mixed pass B 1%
 //      faster on the batch, 2-4% slower on the ramp and the cylinder;
-//  64  pass-A loads of the lagged state alone: pass A 2-14% slower.
+//  64  pass-A loads of the lagged state alone: pass A 2-14% slower;
+// 128  FP64 gradient stores of one record per lane (four 16-byte stores 64
+//      bytes apart: the cell- and slot-parallel shapes, and pass A with
+//      FVK_A_GW = 0): FP64 pass A 27% slower on the batch.
 #ifndef FVK_NT
-#define FVK_NT 10
+#define FVK_NT 26
 #endif
 template <int BIT, typename P>
 __device__ __forceinline__ P sld(const P* p) {
@@ -1065,32 +1068,9 @@ __device__ __forceinline__ void lsq_solve(double ixx, double iyy, double ixy, co
 template <typename T>
 __device__ __forceinline__ void write_grad(const StageArgsT<T>& a, int c, const T gx[4],
                                            const T gy[4]) {
-  constexpr int kNT = kDelta<T> ? 2 : 16;
+  constexpr int kNT = kDelta<T> ? 2 : 128;
   st4s<kNT>(a.g + 8 * (size_t)c, gx);
   st4s<kNT>(a.g + 8 * (size_t)c + 4, gy);
-}
-
-// The gradient records (32 bytes per cell) of a wave's 64 consecutive cells,
-// first cell c0, with two 16-byte stores per lane that each cover a
-// contiguous 1 KB: the lanes exchange halves through wavefront shuffles, so
-// that lane l writes word 64k + l (cell 32k + l/2, gx or gy by the parity of
-// l).  All 64 lanes take part.
-__device__ __forceinline__ void write_grad_wave(float* g, int c0, const float gx[4],
-                                                const float gy[4]) {
-  const int l = threadIdx.x & 63;
-  float* base = g + 8 * (size_t)c0;
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int src = 32 * k + (l >> 1);
-    float w[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const float x = __shfl(gx[j], src);
-      const float y = __shfl(gy[j], src);
-      w[j] = (l & 1) ? y : x;
-    }
-    st4s<2>(base + 4 * (64 * k + l), w);
-  }
 }
 
 template <typename T>
@@ -1458,10 +1438,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_pass_b_slot(StageArgsT<T> a) {
 //   FVK_NS_LATE the FP64 NS pass B prefetches one face (not kFPF) and loads
 //               the stage-update data after its face phase: 124 VGPRs
 //               instead of 154, 4 waves per SIMD instead of 3;
-//   FVK_A_GW    the FP32-mixed pass A writes each wave's gradient records with
-//               stores that each cover a contiguous 1 KB (the lanes exchange
-//               halves through shuffles): 327 -> 318 us per launch on the
-//               256-instance batch, with the non-temporal stores;
+//   FVK_A_GW    pass A writes each wave's gradient records through LDS, with
+//               stores that each cover a contiguous 1 KB: on the 256-instance
+//               batch FP32-mixed 327 -> 318 us per launch, FP64 637 -> 590 us
+//               (560 us with the non-temporal stores);
 //   FVK_A_LATE  pass A loads the limiter's data (lagged state and gradient,
 //               face positions) after the gradient loop (1), the LSQ
 //               inverse too (2), or all before the staging barrier (0).  1
@@ -1551,6 +1531,32 @@ __device__ __forceinline__ void lds_st4(T* p, const T v[4]) {
 }
 
 // centroid of staged cell l
+// The gradient records of a wave's 64 consecutive cells, first cell c0
+// (FVK_A_GW): each lane puts its cell's record (32 or 64 bytes) in the wave's
+// own LDS area, then writes the 16-byte words 64k + l of the wave's range, so
+// that each store covers a contiguous 1 KB where per-lane stores of the
+// records would leave partial lines.  The area is the wave's alone, and a
+// wave's LDS accesses complete in order.  All 64 lanes take part.
+template <typename T>
+__device__ __forceinline__ void write_grad_lds(T* g, int c0, const T gx[4], const T gy[4],
+                                               T* lw) {
+  const int l = threadIdx.x & 63;
+  lds_st4(lw + 8 * l, gx);
+  lds_st4(lw + 8 * l + 4, gy);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  constexpr int kNT = kDelta<T> ? 2 : 16;
+  constexpr int kWords = 8 * (int)sizeof(T) / 16;   // 16-byte words per record
+  constexpr int kPer = 16 / (int)sizeof(T);         // values per word
+  T* base = g + 8 * (size_t)c0;
+#pragma unroll
+  for (int k = 0; k < kWords; ++k) {
+    const int w = 64 * k + l;
+    const f4v x = *reinterpret_cast<const f4v*>(lw + kPer * w);
+    sst<kNT>(reinterpret_cast<f4v*>(base + kPer * w), x);
+  }
+}
+
 template <typename T, int NQ, bool XY>
 __device__ __forceinline__ void lds_xy(const LdsCells<T, NQ, XY>& L, int l, double& x, double& y) {
   const double2 c = *reinterpret_cast<const double2*>(L.xy + 2 * l);
@@ -1699,10 +1705,9 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
       if (FVK_A_LATE < 2 && deg > 0) load_lsq_inv(m, c, ixx, iyy, ixy);
       stage_block(a, b, L);
       __syncthreads();
-      // FVK_A_GW (FP32-mixed): each wave writes its cells' gradient records
-      // together, so the padding cells (no faces) stay to the store and
-      // write zeros
-      constexpr bool kGw = kDelta<T> && FVK_A_GW;
+      // FVK_A_GW: each wave writes its cells' gradient records together, so
+      // the padding cells (no faces) stay to the store and write zeros
+      constexpr bool kGw = FVK_A_GW != 0;
       if (!kGw && deg == 0) break;
       T gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0};
       if (deg > 0) {
@@ -1766,8 +1771,11 @@ __global__ __launch_bounds__(kMaxBlock, kMinBlocksA<T>) void k_pass_a_lds(StageA
           }
         }
       }
-      if constexpr (kGw) write_grad_wave(a.g, c - (i & 63), gx, gy);
-      else write_grad(a, c, gx, gy);
+      if constexpr (kGw)
+        write_grad_lds(a.g, c - (i & 63), gx, gy,
+                       reinterpret_cast<T*>(lds_raw + L.bytes()) + 8 * (i & ~63));
+      else
+        write_grad(a, c, gx, gy);
     } while (false);
   } while (walk.next(args));
 }
@@ -2217,7 +2225,8 @@ size_t lds_cells_bytes(int stride, int nq, size_t tsize, bool xy = true) {
 }  // namespace
 
 size_t lds_bytes_pass_a(const DevMesh& m, size_t tsize) {
-  return lds_cells_bytes(m.blk + m.max_halo, 4, tsize);
+  return lds_cells_bytes(m.blk + m.max_halo, 4, tsize) +
+         (FVK_A_GW ? 8 * (size_t)m.blk * tsize : 0);   // the waves' record areas
 }
 
 size_t lds_bytes_pass_b(const DevMesh& m, int order, bool last, size_t tsize) {
